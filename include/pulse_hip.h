@@ -1,0 +1,285 @@
+/*
+ * pulse_hip.h -- C ABI of the MI355X (gfx950) path-tracing hot path of pulse.
+ *
+ * The library replaces the wgpu dispatch of the `path_trace` compute shader
+ * (src/path_tracer/node.rs:40-119 -> src/path_tracer/path_tracer.wgsl:58-124)
+ * with hand-written HIP kernels.  It consumes the scene exactly as the
+ * reference's host side lays it out: the 13 bindings of bind group 0
+ * (src/scene/bindings.wgsl:14-28, built in src/scene/mod.rs:911-1090) in
+ * encase/std430 byte layout (src/scene/types.wgsl:1-74).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: PULSE_OK (0) or a negative
+ *     PULSE_E_* code.  pulse_last_error() returns a thread-local message.
+ *   - No exception, abort or torch type crosses this boundary.
+ *   - Device pointers are plain `void*`/`float*` obtained from hipMalloc (or
+ *     pulse_device_alloc); host pointers are marked `host` in comments.
+ *   - Calls on one context are externally serialized (one render-graph node
+ *     drives one context, src/path_tracer/mod.rs:60-77).  Work is ordered on
+ *     the context's stream; pulse_sync() waits for it.
+ */
+#ifndef PULSE_HIP_H
+#define PULSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define PULSE_OK 0
+#define PULSE_E_INVALID_ARG -1     /* null pointer, zero size, bad enum      */
+#define PULSE_E_HIP -2             /* a HIP runtime call failed              */
+#define PULSE_E_LAYOUT -3          /* buffer byte size not a multiple of the */
+                                   /* element size, or an index out of range */
+#define PULSE_E_STACK_OVERFLOW -4  /* traversal stack exceeded (device flag) */
+#define PULSE_E_ITERATION_CAP -5   /* traversal hit the 100000 loop cap      */
+#define PULSE_E_NO_SCENE -6        /* trace called before scene upload       */
+#define PULSE_E_UNSUPPORTED -7     /* parameter combination not supported    */
+#define PULSE_E_BUILD -8           /* host BVH builder failed                */
+
+/* ------------------------------------------------------------------------ */
+/* Scene buffer layout (bind group 0).  Sizes are asserted in the sources.   */
+/* ------------------------------------------------------------------------ */
+
+/* binding 0: SceneUniform, src/scene/types.wgsl:3-6 (8 B) */
+typedef struct pulse_scene_uniform {
+    uint32_t instance_count;
+    uint32_t light_count;
+} pulse_scene_uniform;
+
+/* binding 1: Primitive, types.wgsl:23-27; PulsePrimitive, scene/mod.rs:344-347.
+ * Object-space vertices in original mesh order; vec3 at stride 16 (48 B). */
+typedef struct pulse_primitive {
+    float p_first[3];
+    float _pad0;
+    float p_second[3];
+    float _pad1;
+    float p_third[3];
+    float _pad2;
+} pulse_primitive;
+
+/* binding 2: TriangleData, types.wgsl:29-36; PulseTriangleData,
+ * scene/mod.rs:338-342 (80 B). */
+typedef struct pulse_triangle_data {
+    float n_first[3];
+    float _pad0;
+    float n_second[3];
+    float _pad1;
+    float n_third[3];
+    float _pad2;
+    float uv_first[2];
+    float uv_second[2];
+    float uv_third[2];
+    float _pad3[2];
+} pulse_triangle_data;
+
+/* binding 4: BLASNode, types.wgsl:38-43; PulseBLASNode, blas.rs:5-13 (32 B).
+ * tri_count > 0: leaf over triangle_indices[a_or_first_tri .. +tri_count];
+ * tri_count == 0: interior with children a_or_first_tri and a_or_first_tri+1.
+ * Indices are local to the mesh (node_offset/index_offset added per instance). */
+typedef struct pulse_blas_node {
+    float aabb_min[3];
+    uint32_t a_or_first_tri;
+    float aabb_max[3];
+    uint32_t tri_count;
+} pulse_blas_node;
+
+/* binding 5: TLASNode, types.wgsl:45-50; PulseTLASNode, tlas.rs:4-10 (32 B). */
+typedef struct pulse_tlas_node {
+    float aabb_min[3];
+    uint32_t a_or_first_instance;
+    float aabb_max[3];
+    uint32_t instance_count;
+} pulse_tlas_node;
+
+/* binding 7: MeshInstance, types.wgsl:52-60; PulseMeshInstance +
+ * PulseMeshIndex, scene/mod.rs:460-466, 527-533 (160 B).  Matrices are
+ * column-major (glam Mat4 / WGSL mat4x4f). */
+typedef struct pulse_mesh_instance {
+    float object_world[16];
+    float world_object[16];
+    uint32_t triangle_offset;
+    uint32_t triangle_count;
+    uint32_t index_offset;
+    uint32_t node_offset;
+    uint32_t material_index;
+    uint32_t _pad[3];
+} pulse_mesh_instance;
+
+/* binding 8: Material, types.wgsl:62-68; PulseMaterial, scene/mod.rs:240-247 (48 B). */
+typedef struct pulse_material {
+    float base_color[4];
+    float emissive[4];
+    float perceptual_roughness;
+    float reflectance;
+    float metallic;
+    float _pad;
+} pulse_material;
+
+/* binding 12: LightDataIndex, types.wgsl:70-74; PulseLightDataIndex,
+ * scene/mod.rs:547-551 (8 B). */
+typedef struct pulse_light_data_index {
+    uint32_t cdf_offset;
+    uint32_t mesh_instance_index;
+} pulse_light_data_index;
+
+/* A host buffer view (bytes must be a multiple of the element size). */
+typedef struct pulse_buffer_view {
+    const void *data; /* host */
+    size_t bytes;
+} pulse_buffer_view;
+
+/* The 13 bindings of bind group 0 in binding order
+ * (src/scene/mod.rs:1035-1088). */
+typedef struct pulse_scene_desc {
+    pulse_scene_uniform uniform;                   /* 0  */
+    pulse_buffer_view primitives;                  /* 1  pulse_primitive[]       */
+    pulse_buffer_view triangle_data;               /* 2  pulse_triangle_data[]   */
+    pulse_buffer_view triangle_indices;            /* 3  uint32_t[]              */
+    pulse_buffer_view blas_nodes;                  /* 4  pulse_blas_node[]       */
+    pulse_buffer_view tlas_nodes;                  /* 5  pulse_tlas_node[]       */
+    pulse_buffer_view instance_indices;            /* 6  uint32_t[]              */
+    pulse_buffer_view instances;                   /* 7  pulse_mesh_instance[]   */
+    pulse_buffer_view materials;                   /* 8  pulse_material[]        */
+    pulse_buffer_view light_emission_strength_cdf; /* 9  float[] (unused, Q4)    */
+    pulse_buffer_view light_triangle_area_cdfs;    /* 10 float[]                 */
+    pulse_buffer_view light_mesh_areas;            /* 11 float[]                 */
+    pulse_buffer_view light_indices;               /* 12 pulse_light_data_index[] */
+} pulse_scene_desc;
+
+/* ------------------------------------------------------------------------ */
+/* View (bind group 1) and parameters                                        */
+/* ------------------------------------------------------------------------ */
+
+/* The parts of Bevy's `View` uniform the kernel reads
+ * (path_tracer.wgsl:51, 70, 72) plus PathTracerUniform width/height
+ * (path_tracer.wgsl:45-49). */
+typedef struct pulse_camera {
+    float inverse_view_proj[16]; /* column-major */
+    float world_position[3];
+    uint32_t width;
+    uint32_t height;
+} pulse_camera;
+
+/* Path-loop modes (one path per pixel per sample). */
+#define PULSE_MODE_REF_GGX 0        /* path_trace, path_tracer.wgsl:58-124          */
+#define PULSE_MODE_REF_NEE_LAMBERT 1 /* path_traceBBB, path_tracer.wgsl:128-202, with */
+                                     /* a camera primary ray instead of the G-buffer */
+#define PULSE_MODE_GGX_NEE 2        /* GGX sampling + NEE; emissive at depth 0 only */
+
+/* NEE evaluators for PULSE_MODE_GGX_NEE. */
+#define PULSE_NEE_LAMBERT 0 /* sample_direct_light, utilities.wgsl:500-558     */
+#define PULSE_NEE_GGX 1     /* sample_direct_light_ggx, utilities.wgsl:562-655 */
+
+typedef struct pulse_params {
+    uint32_t accumulation_count; /* frame index n of the running mean (Q18)       */
+    uint32_t spp;                /* samples in this call, applied sequentially    */
+                                 /* with accumulation counts n .. n+spp-1         */
+    uint32_t max_depth;          /* path_trace uses 5 (path_tracer.wgsl:80)       */
+    uint32_t mode;               /* PULSE_MODE_*                                  */
+    uint32_t nee_eval;           /* PULSE_NEE_* (mode GGX_NEE only)               */
+    uint32_t rr_start_depth;     /* Russian roulette from this depth on (ref: 0)  */
+    uint32_t flags;              /* PULSE_FLAG_*                                  */
+    uint32_t _reserved;
+} pulse_params;
+
+#define PULSE_FLAG_COLLECT_STATS 1u /* count rays per stage into pulse_stats */
+#define PULSE_FLAG_STAGE_TIMING 2u  /* hipEvent timing per wavefront stage   */
+
+/* Ray / hit records of the debug trace entry point (RayHitRecord,
+ * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */
+typedef struct pulse_ray {
+    float origin[3];
+    float dir[3];
+    float t_max; /* initial record.t; the reference uses 1e30 */
+    uint32_t _pad;
+} pulse_ray;
+
+typedef struct pulse_hit {
+    float t;
+    uint32_t instance_index;
+    uint32_t triangle_index; /* BVH-order local index (utilities.wgsl:277) */
+    float u;
+    float v;
+} pulse_hit;
+
+/* Per-frame counters (exact ray counts come from the queue lengths). */
+typedef struct pulse_stats {
+    uint64_t primary_rays;
+    uint64_t extension_rays;   /* all closest-hit rays incl. primary       */
+    uint64_t incoherent_rays;  /* extension rays at depth >= 1 + shadow    */
+    uint64_t shadow_rays;
+    uint32_t error_flags;      /* bit 0 stack overflow, bit 1 iteration cap */
+    uint32_t stack_entries;    /* traversal stack depth used by the kernels */
+    double ms_total;           /* whole trace call (stage timing flag)     */
+    double ms_primary_extend;  /* depth-0 closest-hit stage                */
+    double ms_incoherent;      /* depth>=1 closest-hit + shadow stages     */
+    double ms_shade;           /* generate + shade + accumulate stages     */
+} pulse_stats;
+
+/* ------------------------------------------------------------------------ */
+/* Context, memory and scene                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct pulse_ctx pulse_ctx;
+
+const char *pulse_last_error(void);
+const char *pulse_version(void);
+
+/* Creates a context on HIP device `device` with its own stream. */
+int pulse_create(int device, pulse_ctx **out);
+int pulse_destroy(pulse_ctx *ctx);
+/* The context's stream as a hipStream_t. */
+void *pulse_stream(pulse_ctx *ctx);
+int pulse_sync(pulse_ctx *ctx);
+
+/* Device memory helpers for callers without a HIP binding (ctypes, FFI). */
+int pulse_device_alloc(pulse_ctx *ctx, size_t bytes, void **out_device_ptr);
+int pulse_device_free(pulse_ctx *ctx, void *device_ptr);
+int pulse_copy_h2d(pulse_ctx *ctx, void *dst_device, const void *src_host, size_t bytes);
+int pulse_copy_d2h(pulse_ctx *ctx, void *dst_host, const void *src_device, size_t bytes);
+int pulse_memset_d(pulse_ctx *ctx, void *dst_device, int value, size_t bytes);
+
+/* Replaces queue_scene_bind_group (src/scene/mod.rs:911-1090): copies the 13
+ * host buffers into device buffers owned by the context, validates sizes and
+ * index ranges, and derives the traversal-ordered triangle copy.  Re-upload
+ * only when the scene changed; `generation` is returned by
+ * pulse_scene_generation so callers can skip identical uploads. */
+int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *desc);
+int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out);
+
+/* ------------------------------------------------------------------------ */
+/* Hot path                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces PulsePathTracerNode::run's dispatch of `path_trace`
+ * (src/path_tracer/node.rs:104-116).  Renders params->spp samples per pixel
+ * and folds each into `accum_rgba` (device, width*height*4 floats, rgba32f,
+ * row-major) with the reference's running mean (path_tracer.wgsl:116-123).
+ * Asynchronous on the context stream; errors raised on the device are
+ * reported by pulse_sync / pulse_get_stats. */
+int pulse_trace(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params,
+                float *accum_rgba);
+
+/* Closest hit for a batch of world-space rays (trace_ray,
+ * utilities.wgsl:112-114 -> traverse_tlas 127-188 -> traverse_blas
+ * 259-326).  rays/hits are device arrays of n records. */
+int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n);
+
+/* Occlusion for a batch of rays: occluded[i] = 1 iff the closest hit of ray i
+ * has t < t_max (the shadow test of utilities.wgsl:517-525 with its
+ * threshold |pl - origin| - 0.005 passed in t_max). */
+int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occluded, uint32_t n);
+
+/* Counters of the last pulse_trace call (synchronizes the stream). */
+int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PULSE_HIP_H */

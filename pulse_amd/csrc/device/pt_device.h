@@ -1,0 +1,607 @@
+// pt_device.h -- device-side building blocks of the gfx950 path-tracing hot
+// path: scene view, f32 vector helpers, PCG, slab and Moeller-Trumbore tests,
+// two-level (TLAS -> BLAS) stack traversal with an LDS stack, GGX sampling
+// and next-event-estimation terms.
+//
+// Every arithmetic expression follows the reference WGSL operation by
+// operation (src/utilities.wgsl, src/path_tracer/path_tracer.wgsl) and the
+// library is compiled with -ffp-contract=off, so results are those of the
+// reference's f32 arithmetic with IEEE division/sqrt; transcendentals come
+// from pt_math.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pulse_hip.h"
+
+#pragma clang force_cuda_host_device begin
+#include "../pt_math.h"
+#pragma clang force_cuda_host_device end
+
+namespace pt {
+
+constexpr float kPI = 3.14159265358f;     // utilities.wgsl:32
+constexpr float kTWO_PI = 6.28318530718f; // utilities.wgsl:34
+constexpr float kINV_PI = 0.31830988618f; // utilities.wgsl:35
+constexpr float kTFar = 1e30f;            // path_tracer.wgsl:74
+constexpr int kMaxIterations = 100000;    // utilities.wgsl:137, 271
+
+constexpr uint32_t kErrStackOverflow = 1u;
+constexpr uint32_t kErrIterationCap = 2u;
+
+// Device view of the uploaded scene.  Besides the reference buffers it holds
+// two traversal-ordered copies made at upload time (identical values, no
+// index indirection in the inner loop):
+//   tri_bvh[3*k .. 3*k+2]  = {p0.xyz, e1.x}, {e1.y, e1.z, e2.x, e2.y}, {e2.z,0,0,0}
+//   nrm_bvh[3*k .. 3*k+2]  = n_first, n_second, n_third
+// for triangle-index slot k, with e1 = p1 - p0 and e2 = p2 - p0 evaluated in
+// f32 exactly as ray_triangle_intersect does (utilities.wgsl:432-433).
+struct DevScene {
+    const float4 *tri_bvh;
+    const float4 *nrm_bvh;
+    const pulse_primitive *primitives;  // original order (NEE, utilities.wgsl:511)
+    const float4 *blas_nodes;           // 2 float4 per node
+    const float4 *tlas_nodes;           // 2 float4 per node
+    const uint32_t *instance_indices;
+    const pulse_mesh_instance *instances;
+    const pulse_material *materials;
+    const float *light_cdfs;
+    const float *light_areas;
+    const pulse_light_data_index *lights;
+    uint32_t instance_count;
+    uint32_t light_count;
+    uint32_t *error_flags;
+};
+
+// ---------------------------------------------------------------------------
+// f32 vectors with the reference's evaluation order
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 mul(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ V3 scale(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(V3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return divs(a, length(a)); }
+__device__ __forceinline__ float max3(V3 v) { return fmaxf(fmaxf(v.x, v.y), v.z); }
+__device__ __forceinline__ V3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+// Column-major mat4 * (p, w) with ((c0*x + c1*y) + c2*z) + c3*w per row.
+__device__ __forceinline__ void mat_vec(const float *m, V3 p, float w, float h[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = m[r] * p.x + m[4 + r] * p.y + m[8 + r] * p.z + m[12 + r] * w;
+}
+// transform_position / transform_direction, utilities.wgsl:751-759
+__device__ __forceinline__ V3 transform_position(const float *m, V3 p) {
+    float h[4];
+    mat_vec(m, p, 1.0f, h);
+    return mk(h[0] / h[3], h[1] / h[3], h[2] / h[3]);
+}
+__device__ __forceinline__ V3 transform_direction(const float *m, V3 p) {
+    float h[4];
+    mat_vec(m, p, 0.0f, h);
+    return mk(h[0], h[1], h[2]);
+}
+
+// ---------------------------------------------------------------------------
+// R1: PCG hash, utilities.wgsl:67-83
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rand_u(uint32_t &state) {
+    state = state * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float rand_f(uint32_t &state) {
+    return static_cast<float>(rand_u(state)) * __uint_as_float(0x2f800004u);
+}
+__device__ __forceinline__ uint32_t pixel_seed(uint32_t pixel_index, uint32_t acc) {
+    return pixel_index * 1235243u + acc * 5817321u;
+}
+
+// ---------------------------------------------------------------------------
+// R5: slab test, utilities.wgsl:405-426 (true division per axis)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float slab(V3 o, V3 d, float rec_t, float4 lo, float4 hi) {
+    float tx1 = (lo.x - o.x) / d.x;
+    float tx2 = (hi.x - o.x) / d.x;
+    float tmin = fminf(tx1, tx2);
+    float tmax = fmaxf(tx1, tx2);
+    float ty1 = (lo.y - o.y) / d.y;
+    float ty2 = (hi.y - o.y) / d.y;
+    tmin = fmaxf(tmin, fminf(ty1, ty2));
+    tmax = fminf(tmax, fmaxf(ty1, ty2));
+    float tz1 = (lo.z - o.z) / d.z;
+    float tz2 = (hi.z - o.z) / d.z;
+    tmin = fmaxf(tmin, fminf(tz1, tz2));
+    tmax = fminf(tmax, fmaxf(tz1, tz2));
+    return (tmax >= tmin && tmin < rec_t && tmax > 0.0f) ? tmin : kTFar;
+}
+
+// Object-space hit record of one BLAS traversal.
+struct ObjHit {
+    float t;
+    uint32_t tri;
+    float u, v;
+    bool found;
+};
+
+// R6: Moeller-Trumbore, utilities.wgsl:430-460, on a traversal-ordered
+// triangle (p0, e1, e2).
+__device__ __forceinline__ void mt(V3 o, V3 d, const float4 *tri, uint32_t slot, uint32_t tri_local, ObjHit &h) {
+    float4 A = tri[3 * slot];
+    float4 B = tri[3 * slot + 1];
+    float4 C = tri[3 * slot + 2];
+    V3 p0 = mk(A.x, A.y, A.z);
+    V3 e1 = mk(A.w, B.x, B.y);
+    V3 e2 = mk(B.z, B.w, C.x);
+    V3 hv = cross(d, e2);
+    float a = dot(e1, hv);
+    if (fabsf(a) < 0.0001f) return;
+    float f = 1.0f / a;
+    V3 s = sub(o, p0);
+    float u = f * dot(s, hv);
+    if (u < 0.0f || u > 1.0f) return;
+    V3 q = cross(s, e1);
+    float v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return;
+    float t = f * dot(e2, q);
+    if (t > 0.001f && t < h.t) {
+        h.t = t;
+        h.tri = tri_local;
+        h.u = u;
+        h.v = v;
+        h.found = true;
+    }
+}
+
+// Per-thread stack in LDS, entry i at base[i * stride] (consecutive lanes in
+// consecutive banks).
+struct LdsStack {
+    uint32_t *base;
+    int stride;
+    int capacity;
+    __device__ __forceinline__ void put(int i, uint32_t v) { base[i * stride] = v; }
+    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+};
+
+// Slack of the object-space culling bound derived from a world-space bound:
+// covers the rounding of the world->object->world round trip of the hit
+// point that the reference's world-t conversion performs.
+__device__ __forceinline__ float world_bound_slack(V3 o, float t) {
+    float m = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), t));
+    return m * 0x1p-12f + 0x1p-20f;
+}
+
+// R4: traverse_blas (utilities.wgsl:259-326) for one instance.  `world_bound`
+// is an upper bound on the world t that can still matter (the current
+// closest hit, or the occlusion threshold); it is turned into a conservative
+// object-space bound so far nodes are culled, which cannot change which
+// triangle wins (anything culled converts to a world t >= world_bound).
+// Returns the reference's world t of this instance's closest object-space hit
+// (+inf if none) and the object-space record.
+__device__ __forceinline__ float traverse_blas(const DevScene &sc, V3 o, V3 d, uint32_t inst, float world_bound,
+                                               LdsStack &stk, int sp_base, ObjHit &h) {
+    const pulse_mesh_instance &mi = sc.instances[inst];
+    float wo[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) wo[i] = mi.world_object[i];
+    const uint32_t node_off = mi.node_offset;
+    const uint32_t idx_off = mi.index_offset;
+    V3 oo = transform_position(wo, o);
+    V3 dd_raw = transform_direction(wo, d);
+    float dlen = length(dd_raw);
+    V3 od = divs(dd_raw, dlen);
+
+    h.t = kTFar;
+    h.found = false;
+    h.tri = 0;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    if (world_bound < kTFar) {
+        float wb = world_bound + world_bound_slack(o, world_bound);
+        float ob = wb * dlen * (1.0f + 0x1p-10f);
+        if (ob < kTFar) h.t = ob;
+    }
+
+    int sp = sp_base;
+    uint32_t node = 0;
+    float4 n_lo = sc.blas_nodes[2 * (node_off + 0)];
+    float4 n_hi = sc.blas_nodes[2 * (node_off + 0) + 1];
+    int iteration = 0;
+    while (iteration < kMaxIterations) {
+        iteration += 1;
+        uint32_t first = __float_as_uint(n_lo.w);
+        uint32_t count = __float_as_uint(n_hi.w);
+        if (count > 0u) {
+            for (uint32_t i = 0; i < count; ++i) mt(oo, od, sc.tri_bvh, idx_off + first + i, first + i, h);
+            if (sp == sp_base) break;
+            sp -= 1;
+            node = stk.get(sp);
+            n_lo = sc.blas_nodes[2 * (node_off + node)];
+            n_hi = sc.blas_nodes[2 * (node_off + node) + 1];
+            continue;
+        }
+        uint32_t a = first, b = first + 1u;
+        const float4 *pa = sc.blas_nodes + 2 * (node_off + a);
+        float4 a_lo = pa[0], a_hi = pa[1], b_lo = pa[2], b_hi = pa[3];
+        float da = slab(oo, od, h.t, a_lo, a_hi);
+        float db = slab(oo, od, h.t, b_lo, b_hi);
+        if (da > db) {
+            float tf = da; da = db; db = tf;
+            uint32_t ti = a; a = b; b = ti;
+            float4 t0 = a_lo; a_lo = b_lo; b_lo = t0;
+            float4 t1 = a_hi; a_hi = b_hi; b_hi = t1;
+        }
+        if (da == kTFar) {
+            if (sp == sp_base) break;
+            sp -= 1;
+            node = stk.get(sp);
+            n_lo = sc.blas_nodes[2 * (node_off + node)];
+            n_hi = sc.blas_nodes[2 * (node_off + node) + 1];
+        } else {
+            node = a;
+            n_lo = a_lo;
+            n_hi = a_hi;
+            if (db != kTFar) {
+                if (sp < stk.capacity) {
+                    stk.put(sp, b);
+                    sp += 1;
+                } else {
+                    atomicOr(sc.error_flags, kErrStackOverflow);
+                }
+            }
+        }
+    }
+    if (iteration >= kMaxIterations) atomicOr(sc.error_flags, kErrIterationCap);
+
+    // World t through the hit point (utilities.wgsl:318-320).  With no hit
+    // the reference converts t = 1e30, which overflows to +inf.
+    if (!h.found) h.t = kTFar;
+    float ow[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
+    V3 hit_obj = add(oo, scale(h.t, od));
+    V3 hit_world = transform_position(ow, hit_obj);
+    return length(sub(hit_world, o));
+}
+
+// R3: traverse_tlas (utilities.wgsl:127-188).  Closest hit: rec_t/ids are
+// updated in place.  Shadow (any_hit) mode: returns true as soon as an
+// instance's world t is below `threshold` (the reference's closest-hit test
+// `t < threshold`, utilities.wgsl:523, decided without finishing the closest
+// hit; see the equivalence argument in traverse_blas).
+template <bool ANY_HIT>
+__device__ __forceinline__ bool traverse_tlas(const DevScene &sc, V3 o, V3 d, float &rec_t, uint32_t &rec_inst,
+                                              uint32_t &rec_tri, float &rec_u, float &rec_v, float threshold,
+                                              LdsStack &stk) {
+    float4 r_lo = sc.tlas_nodes[0];
+    float4 r_hi = sc.tlas_nodes[1];
+    if (__float_as_uint(r_lo.w) == 0u && __float_as_uint(r_hi.w) == 0u) return false;
+    // Box tests cull by the current closest hit (closest) or by a slack
+    // bound above the threshold (any-hit).
+    float cull_t = rec_t;
+    if (ANY_HIT) cull_t = threshold + world_bound_slack(o, threshold);
+
+    int sp = 0;
+    uint32_t node = 0;
+    float4 n_lo = r_lo, n_hi = r_hi;
+    int iteration = 0;
+    while (iteration < kMaxIterations) {
+        iteration += 1;
+        uint32_t first = __float_as_uint(n_lo.w);
+        uint32_t count = __float_as_uint(n_hi.w);
+        if (count > 0u) {
+            for (uint32_t i = 0; i < count; ++i) {
+                uint32_t inst = sc.instance_indices[first + i];
+                ObjHit h;
+                float new_t = traverse_blas(sc, o, d, inst, ANY_HIT ? cull_t : rec_t, stk, sp, h);
+                if (ANY_HIT) {
+                    if (new_t < threshold) return true;
+                } else if (new_t < rec_t) {
+                    rec_t = new_t;
+                    rec_inst = inst;
+                    rec_tri = h.tri;
+                    rec_u = h.u;
+                    rec_v = h.v;
+                    cull_t = rec_t;
+                }
+            }
+            if (sp == 0) break;
+            sp -= 1;
+            node = stk.get(sp);
+            n_lo = sc.tlas_nodes[2 * node];
+            n_hi = sc.tlas_nodes[2 * node + 1];
+            continue;
+        }
+        uint32_t a = first, b = first + 1u;
+        const float4 *pa = sc.tlas_nodes + 2 * a;
+        float4 a_lo = pa[0], a_hi = pa[1], b_lo = pa[2], b_hi = pa[3];
+        float da = slab(o, d, cull_t, a_lo, a_hi);
+        float db = slab(o, d, cull_t, b_lo, b_hi);
+        if (da > db) {
+            float tf = da; da = db; db = tf;
+            uint32_t ti = a; a = b; b = ti;
+            float4 t0 = a_lo; a_lo = b_lo; b_lo = t0;
+            float4 t1 = a_hi; a_hi = b_hi; b_hi = t1;
+        }
+        if (da == kTFar) {
+            if (sp == 0) break;
+            sp -= 1;
+            node = stk.get(sp);
+            n_lo = sc.tlas_nodes[2 * node];
+            n_hi = sc.tlas_nodes[2 * node + 1];
+        } else {
+            node = a;
+            n_lo = a_lo;
+            n_hi = a_hi;
+            if (db != kTFar) {
+                if (sp < stk.capacity) {
+                    stk.put(sp, b);
+                    sp += 1;
+                } else {
+                    atomicOr(sc.error_flags, kErrStackOverflow);
+                }
+            }
+        }
+    }
+    if (iteration >= kMaxIterations) atomicOr(sc.error_flags, kErrIterationCap);
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// R8: GGX terms and sampling, utilities.wgsl:468-476, 719-769, 787-872
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V3 fresnel_schlick(float cos_theta, V3 f0) {
+    float p = pt_pow5f(1.0f - cos_theta);
+    return mk(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+}
+__device__ __forceinline__ float D_GGX(float NdotM, float roughness) {
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float NdotM2 = NdotM * NdotM;
+    float b = (NdotM2 * (a2 - 1.0f) + 1.0f);
+    return a2 * kINV_PI / (b * b);
+}
+__device__ __forceinline__ float G1(float NdotO, float roughness) {
+    float a = roughness * roughness;
+    float k = a / 2.0f;
+    return NdotO / (NdotO * (1.0f - k) + k);
+}
+__device__ __forceinline__ float G_smith(float NdotO, float NdotI, float roughness) {
+    return G1(NdotI, roughness) * G1(NdotO, roughness);
+}
+
+struct Material {
+    V3 base, emissive;
+    float roughness, reflectance, metallic;
+};
+__device__ __forceinline__ Material load_material(const pulse_material &m) {
+    Material r;
+    r.base = ld3(m.base_color);
+    r.emissive = ld3(m.emissive);
+    r.roughness = m.perceptual_roughness;
+    r.reflectance = m.reflectance;
+    r.metallic = m.metallic;
+    return r;
+}
+__device__ __forceinline__ V3 material_f0(const Material &m) {
+    float f = 0.16f * m.reflectance * m.reflectance;
+    float om = 1.0f - m.metallic;
+    return mk(f * om + m.base.x * m.metallic, f * om + m.base.y * m.metallic, f * om + m.base.z * m.metallic);
+}
+
+// sample_cosine_hemisphere, utilities.wgsl:468-476
+__device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float e0, float e1) {
+    float cos_theta = 2.0f * e0 - 1.0f;
+    float phi = 2.0f * kPI * e1;
+    float sin_theta = __builtin_sqrtf(fmaxf(1.0f - cos_theta * cos_theta, 0.0f));
+    float sin_phi = pt_sinf(phi);
+    float cos_phi = pt_cosf(phi);
+    V3 dir = normalize(mk(sin_theta * cos_phi, cos_theta, sin_theta * sin_phi));
+    return normalize(add(n, dir));
+}
+
+struct Sample {
+    V3 wi, reflectance;
+};
+
+// importance_sample_ggx_d, utilities.wgsl:816-872
+__device__ __forceinline__ Sample importance_sample_ggx_d(V3 n, V3 wo, const Material &m, uint32_t &rng) {
+    V3 f0 = material_f0(m);
+    float NdotO = dot(n, wo);
+    V3 F0 = fresnel_schlick(NdotO, f0);
+    float F0_max = max3(F0);
+    float e0 = rand_f(rng);
+    float e1 = rand_f(rng);
+    float e2 = rand_f(rng);
+    V3 brdf, wi;
+    float pdf_s = 0.0f, pdf_d = 0.0f;
+    if (e0 <= F0_max) {
+        float a = m.roughness * m.roughness;
+        float a2 = a * a;
+        float theta = pt_acosf(__builtin_sqrtf((1.0f - e1) / (e1 * (a2 - 1.0f) + 1.0f)));
+        float phi = e2 * kTWO_PI;
+        // orthonormal_from_normal, utilities.wgsl:719-729
+        V3 e_two = n;
+        V3 e_three;
+        if (!(length(sub(n, mk(1.0f, 0.0f, 0.0f))) < 0.001f)) {
+            e_three = normalize(cross(e_two, mk(1.0f, 0.0f, 0.0f)));
+        } else {
+            e_three = normalize(cross(e_two, mk(0.0f, 1.0f, 0.0f)));
+        }
+        V3 e_one = normalize(cross(e_two, e_three));
+        // spherical_to_cartesian_in_on, utilities.wgsl:732-739: A + (B + C)
+        float sin_theta = pt_sinf(theta);
+        float sp = pt_sinf(phi), cp = pt_cosf(phi), ct = pt_cosf(theta);
+        V3 A = mk((e_one.x * sin_theta) * sp, (e_one.y * sin_theta) * sp, (e_one.z * sin_theta) * sp);
+        V3 B = mk(e_two.x * ct, e_two.y * ct, e_two.z * ct);
+        V3 C = mk((e_three.x * sin_theta) * cp, (e_three.y * sin_theta) * cp, (e_three.z * sin_theta) * cp);
+        V3 wm = add(A, add(B, C));
+        // reflect, utilities.wgsl:767-769
+        float k = 2.0f * dot(wo, wm);
+        wi = sub(scale(k, wm), wo);
+        float NdotM = dot(n, wm);
+        float NdotI = dot(n, wi);
+        float OdotM = dot(wo, wm);
+        V3 F = fresnel_schlick(OdotM, f0);
+        float D = D_GGX(NdotM, m.roughness);
+        float G = G_smith(NdotO, NdotI, m.roughness);
+        float denom = 4.0f * NdotO * NdotI;
+        brdf = mk(F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom);
+        pdf_s = D * NdotM / (4.0f * OdotM);
+    } else {
+        wi = sample_cosine_hemisphere(n, e1, e2);
+        float NdotI = dot(n, wi);
+        brdf = mk(m.base.x * kPI * NdotI, m.base.y * kPI * NdotI, m.base.z * kPI * NdotI);
+        pdf_d = kPI * NdotI;
+    }
+    float pdf = ((1.0f - F0_max) * pdf_d) + (F0_max * pdf_s);
+    if (pdf < 0.0001f) pdf = 1.0f;
+    Sample s;
+    s.wi = wi;
+    s.reflectance = divs(brdf, pdf);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// R10-R12: NEE setup and unoccluded contribution
+// ---------------------------------------------------------------------------
+// sample_light_triangle_area_cdf (utilities.wgsl:677-691) as a binary search
+// for the same index: the reference's loop (mid = l + (r - l) = r) scans from
+// the top for the last entry <= e and returns min(that + 1, count - 1); the
+// predicate cdf <= e holds on a prefix of the CDF (running sums of
+// non-negative areas; NaN entries, if any, form a suffix and compare false),
+// so the partition point found here is the same index.
+__device__ __forceinline__ uint32_t cdf_pick(const float *cdf, float e, uint32_t count) {
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (cdf[mid] <= e) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return min(lo, count - 1u);
+}
+
+struct NeeSample {
+    V3 origin, dir;     // shadow ray
+    float threshold;    // occluded iff closest t < threshold
+    V3 contrib;         // direct light if unoccluded (before * throughput)
+};
+
+// sample_direct_light / sample_direct_light_ggx (utilities.wgsl:500-558,
+// 562-655) up to the occlusion test, plus the contribution they return when
+// the light is visible.  Consumes 4 random numbers.
+__device__ __forceinline__ NeeSample nee_sample(const DevScene &sc, V3 p0, V3 n0, const Material &m, V3 wo, bool ggx,
+                                                uint32_t &rng) {
+    NeeSample ns;
+    uint32_t light_index = rand_u(rng) % sc.light_count;
+    pulse_light_data_index ldi = sc.lights[light_index];
+    const pulse_mesh_instance &mi = sc.instances[ldi.mesh_instance_index];
+    float e0 = rand_f(rng);
+    float e1 = rand_f(rng);
+    float e2 = rand_f(rng);
+    uint32_t prim_index = cdf_pick(sc.light_cdfs + ldi.cdf_offset, e0, mi.triangle_count);
+    const pulse_primitive &prim = sc.primitives[mi.triangle_offset + prim_index];
+    V3 q0 = ld3(prim.p_first), q1 = ld3(prim.p_second), q2 = ld3(prim.p_third);
+    // sample_triangle_uniformly, utilities.wgsl:695-703
+    V3 sa = sub(q1, q0), sb = sub(q2, q0);
+    V3 pl_obj;
+    if (e1 + e2 < 1.0f) {
+        pl_obj = add(add(q0, scale(e1, sa)), scale(e2, sb));
+    } else {
+        pl_obj = add(add(q0, scale(1.0f - e1, sa)), scale(1.0f - e2, sb));
+    }
+    float ow[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
+    V3 pl = transform_position(ow, pl_obj);
+    V3 to_light = sub(pl, p0);
+    ns.origin = add(p0, scale(0.001f, n0));
+    ns.dir = normalize(to_light);
+    ns.threshold = length(sub(pl, ns.origin)) - 0.005f;
+
+    // light normal from the object-space triangle (utilities.wgsl:533-538)
+    V3 nl = normalize(cross(sa, sb));
+    if (dot(nl, neg(ns.dir)) < 0.0f) nl = neg(nl);
+    float light_pdf = 1.0f / static_cast<float>(sc.light_count);
+    float triangle_pdf = 1.0f / sc.light_areas[light_index];
+    float pdf = light_pdf * triangle_pdf;
+    float cos_r = dot(ns.dir, n0);
+    float cos_e = dot(neg(ns.dir), nl);
+    V3 brdf;
+    if (!ggx) {
+        brdf = mk(m.base.x * kINV_PI, m.base.y * kINV_PI, m.base.z * kINV_PI);
+    } else {
+        V3 f0 = material_f0(m);
+        float NdotO = dot(n0, wo);
+        V3 wi = to_light;  // unnormalized, as in utilities.wgsl:639
+        V3 wm = normalize(add(scale(0.5f, wi), scale(0.5f, wo)));
+        float NdotM = dot(n0, wm);
+        float NdotI = dot(n0, wi);
+        float OdotM = dot(wo, wm);
+        V3 F = fresnel_schlick(OdotM, f0);
+        float D = D_GGX(NdotM, m.roughness);
+        float G = G_smith(NdotO, NdotI, m.roughness);
+        float denom = 4.0f * NdotO * NdotI;
+        brdf = mk(F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom);
+    }
+    V3 le = ld3(sc.materials[mi.material_index].emissive);
+    float dsq = dot(sub(n0, nl), sub(n0, nl));
+    V3 dl;
+    dl.x = brdf.x * le.x * cos_r * cos_e / pdf / dsq;
+    dl.y = brdf.y * le.y * cos_r * cos_e / pdf / dsq;
+    dl.z = brdf.z * le.z * cos_r * cos_e / pdf / dsq;
+    ns.contrib = mk(fmaxf(dl.x, 0.0f), fmaxf(dl.y, 0.0f), fmaxf(dl.z, 0.0f));
+    return ns;
+}
+
+// ---------------------------------------------------------------------------
+// R2 / R7: primary ray and hit shading
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void primary_ray(const pulse_camera &cam, uint32_t x, uint32_t y, V3 &o, V3 &d) {
+    float uvx = static_cast<float>(x) / static_cast<float>(cam.width);
+    float uvy = static_cast<float>(y) / static_cast<float>(cam.height);
+    float cx = uvx * 2.0f - 1.0f;
+    float cy = uvy * 2.0f - 1.0f;
+    float h[4];
+    mat_vec(cam.inverse_view_proj, mk(cx, -cy, 1.0f), 1.0f, h);
+    o = ld3(cam.world_position);
+    d = normalize(sub(mk(h[0] / h[3], h[1] / h[3], h[2] / h[3]), o));
+}
+
+struct HitInfo {
+    V3 normal, position;
+    uint32_t material_index;
+};
+// path_tracer.wgsl:89-98
+__device__ __forceinline__ HitInfo shade_hit(const DevScene &sc, V3 o, V3 d, float t, uint32_t inst, uint32_t tri,
+                                             float u, float v) {
+    const pulse_mesh_instance &mi = sc.instances[inst];
+    uint32_t slot = mi.index_offset + tri;
+    float4 a = sc.nrm_bvh[3 * slot], b = sc.nrm_bvh[3 * slot + 1], c = sc.nrm_bvh[3 * slot + 2];
+    float w = 1.0f - (u + v);
+    V3 nrm = add(add(scale(w, mk(a.x, a.y, a.z)), scale(u, mk(b.x, b.y, b.z))), scale(v, mk(c.x, c.y, c.z)));
+    float ow[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
+    HitInfo h;
+    h.normal = normalize(transform_direction(ow, nrm));
+    h.position = add(o, scale(t, d));
+    h.material_index = mi.material_index;
+    return h;
+}
+
+}  // namespace pt

@@ -1,0 +1,592 @@
+// pulse_hip.hip -- context, scene upload, launch sequencing and the C ABI of
+// libpulse_hip (declared in include/pulse_hip.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device/wavefront.h"
+#include "host/common.h"
+
+using namespace pt;
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            PULSE_FAIL(PULSE_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                              \
+    } while (0)
+
+namespace {
+
+// One device allocation that grows on demand.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && ptr) return PULSE_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        size_t b = std::max<size_t>(bytes, 256);
+        HIP_TRY(hipMalloc(&ptr, b));
+        cap = b;
+        return PULSE_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+enum SceneBuf {
+    B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
+    B_CDFS, B_AREAS, B_LIGHTS, B_COUNT
+};
+
+enum WfBuf {
+    W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_HIT_V, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C,
+    W_COUNTERS, W_COUNT
+};
+
+}  // namespace
+
+struct pulse_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    DevBuf scene[B_COUNT];
+    DevBuf wf[W_COUNT];
+    DevBuf errors;  // u32 device error word
+    size_t wf_pixels = 0;
+    bool has_scene = false;
+    uint64_t generation = 0;
+    pulse_scene_uniform uniform{};
+    uint32_t stack_entries = 0;  // traversal stack capacity in use
+    uint32_t tlas_depth = 0, blas_depth = 0;
+    pulse_stats stats{};
+    uint32_t last_traces = 0;
+    hipEvent_t ev[3 * kMaxTraces + 8] = {};
+    bool events_created = false;
+};
+
+namespace {
+
+int set_device(pulse_ctx *ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return PULSE_OK;
+}
+
+// Smallest compiled stack size that holds tlas_depth + blas_depth entries:
+// a depth-first traversal that pushes at most one far child per level never
+// holds more entries than the depth of the node it is at.
+uint32_t pick_stack(uint32_t need) {
+    const uint32_t sizes[] = {16, 24, 32, 48, 64, 96, 128};
+    for (uint32_t s : sizes)
+        if (s >= need) return s;
+    return 0;
+}
+
+// Depth of the BLAS rooted at node_offset `root` (child indices are local to
+// the mesh), checking every child index and leaf triangle range.
+bool walk_blas(const pulse_blas_node *nodes, size_t n, uint32_t root, uint32_t tri_count, uint32_t &depth_out) {
+    uint32_t best = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> st;
+    st.push_back({0u, 0u});
+    size_t visited = 0;
+    while (!st.empty()) {
+        auto [local, d] = st.back();
+        st.pop_back();
+        uint64_t ni = static_cast<uint64_t>(root) + local;
+        if (ni >= n || ++visited > n) return false;
+        best = std::max(best, d);
+        const pulse_blas_node &x = nodes[ni];
+        if (x.tri_count == 0) {
+            if (x.a_or_first_tri == 0 || static_cast<uint64_t>(root) + x.a_or_first_tri + 1 >= n) return false;
+            st.push_back({x.a_or_first_tri, d + 1});
+            st.push_back({x.a_or_first_tri + 1, d + 1});
+        } else if (static_cast<uint64_t>(x.a_or_first_tri) + x.tri_count > tri_count) {
+            return false;
+        }
+    }
+    depth_out = best;
+    return true;
+}
+
+// Depth of the TLAS, checking child indices and leaf instance ranges.
+bool walk_tlas(const pulse_tlas_node *nodes, size_t n, size_t n_iidx, uint32_t &depth_out) {
+    uint32_t best = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> st;
+    st.push_back({0u, 0u});
+    size_t visited = 0;
+    while (!st.empty()) {
+        auto [ni, d] = st.back();
+        st.pop_back();
+        if (ni >= n || ++visited > n) return false;
+        best = std::max(best, d);
+        const pulse_tlas_node &x = nodes[ni];
+        if (x.instance_count == 0) {
+            if (x.a_or_first_instance == 0 || static_cast<uint64_t>(x.a_or_first_instance) + 1 >= n) return false;
+            st.push_back({x.a_or_first_instance, d + 1});
+            st.push_back({x.a_or_first_instance + 1, d + 1});
+        } else if (static_cast<uint64_t>(x.a_or_first_instance) + x.instance_count > n_iidx) {
+            return false;
+        }
+    }
+    depth_out = best;
+    return true;
+}
+
+DevScene dev_scene(pulse_ctx *ctx) {
+    DevScene s;
+    s.tri_bvh = ctx->scene[B_TRI_BVH].as<float4>();
+    s.nrm_bvh = ctx->scene[B_NRM_BVH].as<float4>();
+    s.primitives = ctx->scene[B_PRIMS].as<pulse_primitive>();
+    s.blas_nodes = ctx->scene[B_BLAS].as<float4>();
+    s.tlas_nodes = ctx->scene[B_TLAS].as<float4>();
+    s.instance_indices = ctx->scene[B_INST_IDX].as<uint32_t>();
+    s.instances = ctx->scene[B_INSTANCES].as<pulse_mesh_instance>();
+    s.materials = ctx->scene[B_MATERIALS].as<pulse_material>();
+    s.light_cdfs = ctx->scene[B_CDFS].as<float>();
+    s.light_areas = ctx->scene[B_AREAS].as<float>();
+    s.lights = ctx->scene[B_LIGHTS].as<pulse_light_data_index>();
+    s.instance_count = ctx->uniform.instance_count;
+    s.light_count = ctx->uniform.light_count;
+    s.error_flags = ctx->errors.as<uint32_t>();
+    return s;
+}
+
+constexpr int kTraceBlock = 64;  // one wave per block: LDS stack granularity
+
+template <int STACK>
+void launch_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
+    uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    hipLaunchKernelGGL((k_trace_rays<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+                       dev_scene(ctx), rays, hits, n);
+}
+template <int STACK>
+void launch_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ, uint32_t n) {
+    uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    hipLaunchKernelGGL((k_shadow_rays<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+                       dev_scene(ctx), rays, occ, n);
+}
+template <int STACK>
+void launch_extend(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int q, int k) {
+    uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
+    hipLaunchKernelGGL((k_extend<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+                       dev_scene(ctx), wb, q, k);
+}
+template <int STACK>
+void launch_shadow(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int k) {
+    uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
+    hipLaunchKernelGGL((k_shadow<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+                       dev_scene(ctx), wb, k);
+}
+
+#define PULSE_STACK_DISPATCH(fn, ctx, ...)                  \
+    switch ((ctx)->stack_entries) {                         \
+        case 16: fn<16>(ctx, __VA_ARGS__); break;           \
+        case 24: fn<24>(ctx, __VA_ARGS__); break;           \
+        case 32: fn<32>(ctx, __VA_ARGS__); break;           \
+        case 48: fn<48>(ctx, __VA_ARGS__); break;           \
+        case 64: fn<64>(ctx, __VA_ARGS__); break;           \
+        case 96: fn<96>(ctx, __VA_ARGS__); break;           \
+        default: fn<128>(ctx, __VA_ARGS__); break;          \
+    }
+
+int ensure_wavefront(pulse_ctx *ctx, size_t npix) {
+    if (npix <= ctx->wf_pixels) return PULSE_OK;
+    const size_t f4 = sizeof(float4) * npix;
+    int rc;
+    for (int b : {W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C})
+        if ((rc = ctx->wf[b].ensure(f4)) != PULSE_OK) return rc;
+    if ((rc = ctx->wf[W_HIT_V].ensure(sizeof(float) * npix)) != PULSE_OK) return rc;
+    if ((rc = ctx->wf[W_COUNTERS].ensure(sizeof(uint32_t) * 2 * (kMaxTraces + 1))) != PULSE_OK) return rc;
+    ctx->wf_pixels = npix;
+    return PULSE_OK;
+}
+
+WfBuffers wf_buffers(pulse_ctx *ctx) {
+    WfBuffers wb;
+    wb.ray_o[0] = ctx->wf[W_RAY_O0].as<float4>();
+    wb.ray_o[1] = ctx->wf[W_RAY_O1].as<float4>();
+    wb.ray_d[0] = ctx->wf[W_RAY_D0].as<float4>();
+    wb.ray_d[1] = ctx->wf[W_RAY_D1].as<float4>();
+    wb.hit = ctx->wf[W_HIT].as<float4>();
+    wb.hit_v = ctx->wf[W_HIT_V].as<float>();
+    wb.state_t = ctx->wf[W_STATE_T].as<float4>();
+    wb.state_c = ctx->wf[W_STATE_C].as<float4>();
+    wb.sh_o = ctx->wf[W_SH_O].as<float4>();
+    wb.sh_d = ctx->wf[W_SH_D].as<float4>();
+    wb.sh_c = ctx->wf[W_SH_C].as<float4>();
+    wb.ext_count = ctx->wf[W_COUNTERS].as<uint32_t>();
+    wb.sh_count = wb.ext_count + (kMaxTraces + 1);
+    return wb;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *pulse_version(void) { return "pulse_hip 0.1 (gfx950)"; }
+
+int pulse_create(int device, pulse_ctx **out) {
+    if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: null out");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: device %d of %d", device, n);
+    auto *ctx = new pulse_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        PULSE_FAIL(PULSE_E_HIP, "pulse_create: %s", hipGetErrorString(e));
+    }
+    if (ctx->errors.ensure(sizeof(uint32_t)) != PULSE_OK || hipMemset(ctx->errors.ptr, 0, sizeof(uint32_t)) != hipSuccess) {
+        pulse_destroy(ctx);
+        PULSE_FAIL(PULSE_E_HIP, "pulse_create: cannot allocate the error word");
+    }
+    *out = ctx;
+    return PULSE_OK;
+}
+
+int pulse_destroy(pulse_ctx *ctx) {
+    if (!ctx) return PULSE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->scene) b.release();
+    for (auto &b : ctx->wf) b.release();
+    ctx->errors.release();
+    if (ctx->events_created)
+        for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PULSE_OK;
+}
+
+void *pulse_stream(pulse_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+int pulse_sync(pulse_ctx *ctx) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sync: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
+    if (err & kErrIterationCap) PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached the 100000-iteration cap");
+    return PULSE_OK;
+}
+
+int pulse_device_alloc(pulse_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out || bytes == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_alloc: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(out, bytes));
+    return PULSE_OK;
+}
+int pulse_device_free(pulse_ctx *ctx, void *p) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_free: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (p) HIP_TRY(hipFree(p));
+    return PULSE_OK;
+}
+int pulse_copy_h2d(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_h2d: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PULSE_OK;
+}
+int pulse_copy_d2h(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_d2h: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PULSE_OK;
+}
+int pulse_memset_d(pulse_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!ctx || (bytes && !dst)) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_memset_d: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return PULSE_OK;
+}
+
+int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
+    if (!ctx || !d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    auto count_of = [](const pulse_buffer_view &v, size_t elem, const char *name, size_t &n) -> int {
+        if (v.bytes % elem != 0)
+            PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: %s has %zu bytes, not a multiple of %zu", name, v.bytes, elem);
+        if (v.bytes && !v.data) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: %s data is null", name);
+        n = v.bytes / elem;
+        return PULSE_OK;
+    };
+    size_t n_prim, n_tdata, n_tidx, n_blas, n_tlas, n_iidx, n_inst, n_mat, n_scdf, n_cdf, n_area, n_light;
+    if ((rc = count_of(d->primitives, sizeof(pulse_primitive), "primitives", n_prim)) ||
+        (rc = count_of(d->triangle_data, sizeof(pulse_triangle_data), "triangle_data", n_tdata)) ||
+        (rc = count_of(d->triangle_indices, 4, "triangle_indices", n_tidx)) ||
+        (rc = count_of(d->blas_nodes, sizeof(pulse_blas_node), "blas_nodes", n_blas)) ||
+        (rc = count_of(d->tlas_nodes, sizeof(pulse_tlas_node), "tlas_nodes", n_tlas)) ||
+        (rc = count_of(d->instance_indices, 4, "instance_indices", n_iidx)) ||
+        (rc = count_of(d->instances, sizeof(pulse_mesh_instance), "instances", n_inst)) ||
+        (rc = count_of(d->materials, sizeof(pulse_material), "materials", n_mat)) ||
+        (rc = count_of(d->light_emission_strength_cdf, 4, "light_emission_strength_cdf", n_scdf)) ||
+        (rc = count_of(d->light_triangle_area_cdfs, 4, "light_triangle_area_cdfs", n_cdf)) ||
+        (rc = count_of(d->light_mesh_areas, 4, "light_mesh_areas", n_area)) ||
+        (rc = count_of(d->light_indices, sizeof(pulse_light_data_index), "light_indices", n_light)))
+        return rc;
+    (void)n_scdf;
+    if (n_tlas == 0) PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: tlas_nodes is empty (needs at least the root)");
+    if (d->uniform.instance_count > n_inst)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: instance_count %u > %zu instances", d->uniform.instance_count, n_inst);
+    if (d->uniform.light_count > n_light || d->uniform.light_count > n_area)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: light_count %u exceeds the light tables", d->uniform.light_count);
+    if (n_tdata != n_prim) PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: %zu triangle_data for %zu primitives", n_tdata, n_prim);
+
+    const auto *prims = static_cast<const pulse_primitive *>(d->primitives.data);
+    const auto *tdata = static_cast<const pulse_triangle_data *>(d->triangle_data.data);
+    const auto *tidx = static_cast<const uint32_t *>(d->triangle_indices.data);
+    const auto *blas = static_cast<const pulse_blas_node *>(d->blas_nodes.data);
+    const auto *tlas = static_cast<const pulse_tlas_node *>(d->tlas_nodes.data);
+    const auto *iidx = static_cast<const uint32_t *>(d->instance_indices.data);
+    const auto *inst = static_cast<const pulse_mesh_instance *>(d->instances.data);
+    const auto *lights = static_cast<const pulse_light_data_index *>(d->light_indices.data);
+
+    // --- validate references and build the traversal-ordered copies ---
+    std::vector<float4> tri_bvh(3 * std::max<size_t>(n_tidx, 1), make_float4(0, 0, 0, 0));
+    std::vector<float4> nrm_bvh(3 * std::max<size_t>(n_tidx, 1), make_float4(0, 0, 0, 0));
+    std::vector<int64_t> slot_owner(n_tidx, -1);  // triangle_offset that filled a slot
+    uint32_t blas_depth = 0;
+    std::vector<uint32_t> seen_roots;
+    for (size_t i = 0; i < n_inst; ++i) {
+        const pulse_mesh_instance &mi = inst[i];
+        if (mi.material_index >= n_mat)
+            PULSE_FAIL(PULSE_E_LAYOUT, "instance %zu: material_index %u >= %zu", i, mi.material_index, n_mat);
+        if (static_cast<uint64_t>(mi.triangle_offset) + mi.triangle_count > n_prim ||
+            static_cast<uint64_t>(mi.index_offset) + mi.triangle_count > n_tidx || mi.node_offset >= n_blas ||
+            mi.triangle_count == 0)
+            PULSE_FAIL(PULSE_E_LAYOUT, "instance %zu: triangle/index/node range outside the buffers", i);
+        if (std::find(seen_roots.begin(), seen_roots.end(), mi.node_offset) == seen_roots.end()) {
+            seen_roots.push_back(mi.node_offset);
+            uint32_t dep = 0;
+            if (!walk_blas(blas, n_blas, mi.node_offset, mi.triangle_count, dep))
+                PULSE_FAIL(PULSE_E_LAYOUT, "instance %zu: BLAS at node_offset %u has an out-of-range child or leaf", i,
+                           mi.node_offset);
+            blas_depth = std::max(blas_depth, dep);
+        }
+        for (uint32_t k = 0; k < mi.triangle_count; ++k) {
+            size_t slot = mi.index_offset + k;
+            if (slot_owner[slot] == static_cast<int64_t>(mi.triangle_offset)) continue;
+            if (slot_owner[slot] != -1)
+                PULSE_FAIL(PULSE_E_LAYOUT, "triangle_indices slot %zu is shared by meshes with different triangle_offset", slot);
+            slot_owner[slot] = mi.triangle_offset;
+            uint32_t local = tidx[slot];
+            if (local >= mi.triangle_count)
+                PULSE_FAIL(PULSE_E_LAYOUT, "triangle_indices[%zu] = %u >= mesh triangle count %u", slot, local, mi.triangle_count);
+            const pulse_primitive &p = prims[mi.triangle_offset + local];
+            // e1 = p1 - p0, e2 = p2 - p0 in f32 (utilities.wgsl:432-433)
+            float e1x = p.p_second[0] - p.p_first[0], e1y = p.p_second[1] - p.p_first[1], e1z = p.p_second[2] - p.p_first[2];
+            float e2x = p.p_third[0] - p.p_first[0], e2y = p.p_third[1] - p.p_first[1], e2z = p.p_third[2] - p.p_first[2];
+            tri_bvh[3 * slot] = make_float4(p.p_first[0], p.p_first[1], p.p_first[2], e1x);
+            tri_bvh[3 * slot + 1] = make_float4(e1y, e1z, e2x, e2y);
+            tri_bvh[3 * slot + 2] = make_float4(e2z, 0.f, 0.f, 0.f);
+            const pulse_triangle_data &t = tdata[mi.triangle_offset + local];
+            nrm_bvh[3 * slot] = make_float4(t.n_first[0], t.n_first[1], t.n_first[2], 0.f);
+            nrm_bvh[3 * slot + 1] = make_float4(t.n_second[0], t.n_second[1], t.n_second[2], 0.f);
+            nrm_bvh[3 * slot + 2] = make_float4(t.n_third[0], t.n_third[1], t.n_third[2], 0.f);
+        }
+    }
+    uint32_t tlas_depth = 0;
+    bool empty_tlas = tlas[0].a_or_first_instance == 0 && tlas[0].instance_count == 0;
+    if (!empty_tlas) {
+        if (!walk_tlas(tlas, n_tlas, n_iidx, tlas_depth))
+            PULSE_FAIL(PULSE_E_LAYOUT, "tlas_nodes: child index or instance range out of range");
+        for (size_t k = 0; k < n_iidx; ++k)
+            if (iidx[k] >= n_inst) PULSE_FAIL(PULSE_E_LAYOUT, "instance_indices[%zu] = %u >= %zu", k, iidx[k], n_inst);
+    }
+    for (size_t l = 0; l < d->uniform.light_count; ++l) {
+        if (lights[l].mesh_instance_index >= n_inst)
+            PULSE_FAIL(PULSE_E_LAYOUT, "light %zu: mesh_instance_index out of range", l);
+        const pulse_mesh_instance &mi = inst[lights[l].mesh_instance_index];
+        if (static_cast<uint64_t>(lights[l].cdf_offset) + mi.triangle_count > n_cdf)
+            PULSE_FAIL(PULSE_E_LAYOUT, "light %zu: cdf range outside light_triangle_area_cdfs", l);
+    }
+    uint32_t need = tlas_depth + blas_depth + 1;
+    uint32_t stack = pick_stack(need);
+    if (stack == 0)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
+                   tlas_depth, blas_depth);
+
+    // --- copy to device ---
+    auto up = [&](int b, const void *src, size_t bytes) -> int {
+        int r = ctx->scene[b].ensure(bytes);
+        if (r) return r;
+        if (bytes) HIP_TRY(hipMemcpy(ctx->scene[b].ptr, src, bytes, hipMemcpyHostToDevice));
+        return PULSE_OK;
+    };
+    if ((rc = up(B_PRIMS, prims, d->primitives.bytes)) || (rc = up(B_TRI_BVH, tri_bvh.data(), tri_bvh.size() * sizeof(float4))) ||
+        (rc = up(B_NRM_BVH, nrm_bvh.data(), nrm_bvh.size() * sizeof(float4))) || (rc = up(B_BLAS, blas, d->blas_nodes.bytes)) ||
+        (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) || (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes)) ||
+        (rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_MATERIALS, d->materials.data, d->materials.bytes)) ||
+        (rc = up(B_CDFS, d->light_triangle_area_cdfs.data, d->light_triangle_area_cdfs.bytes)) ||
+        (rc = up(B_AREAS, d->light_mesh_areas.data, d->light_mesh_areas.bytes)) ||
+        (rc = up(B_LIGHTS, lights, d->light_indices.bytes)))
+        return rc;
+    ctx->uniform = d->uniform;
+    ctx->stack_entries = stack;
+    ctx->tlas_depth = tlas_depth;
+    ctx->blas_depth = blas_depth;
+    ctx->has_scene = true;
+    ctx->generation += 1;
+    return PULSE_OK;
+}
+
+int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out) {
+    if (!ctx || !out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_generation: null argument");
+    *out = ctx->generation;
+    return PULSE_OK;
+}
+
+int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
+    if (!ctx || (n && (!rays || !hits))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_rays: bad argument");
+    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_rays: no scene uploaded");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (n == 0) return PULSE_OK;
+    PULSE_STACK_DISPATCH(launch_trace_rays, ctx, rays, hits, n);
+    HIP_TRY(hipGetLastError());
+    return PULSE_OK;
+}
+
+int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occluded, uint32_t n) {
+    if (!ctx || (n && (!rays || !occluded))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_shadow_rays: bad argument");
+    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_shadow_rays: no scene uploaded");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (n == 0) return PULSE_OK;
+    PULSE_STACK_DISPATCH(launch_shadow_rays, ctx, rays, occluded, n);
+    HIP_TRY(hipGetLastError());
+    return PULSE_OK;
+}
+
+int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
+    if (!ctx || !cam || !prm || !accum_rgba) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: null argument");
+    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace: no scene uploaded");
+    if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: empty image");
+    if (prm->mode > PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown mode %u", prm->mode);
+    if (prm->nee_eval > PULSE_NEE_GGX) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown nee_eval %u", prm->nee_eval);
+    uint32_t traces = prm->mode == PULSE_MODE_REF_NEE_LAMBERT ? prm->max_depth + 1u : prm->max_depth;
+    if (traces > static_cast<uint32_t>(kMaxTraces))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace: max_depth %u above %d", prm->max_depth, kMaxTraces - 1);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const size_t npix = static_cast<size_t>(cam->width) * cam->height;
+    if (npix > 0xffffffffull / 4) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace: image too large");
+    if ((rc = ensure_wavefront(ctx, npix)) != PULSE_OK) return rc;
+    WfBuffers wb = wf_buffers(ctx);
+    const bool timing = (prm->flags & PULSE_FLAG_STAGE_TIMING) != 0;
+    if (timing && !ctx->events_created) {
+        for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+        ctx->events_created = true;
+    }
+
+    FrameArgs fa;
+    fa.cam = *cam;
+    fa.mode = prm->mode;
+    fa.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
+    fa.max_traces = traces;
+    fa.rr_start = prm->rr_start_depth;
+    const uint32_t cap = static_cast<uint32_t>(npix);
+    const uint32_t tiles = ((cam->width + 7) / 8) * ((cam->height + 7) / 8);
+    const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
+    const uint32_t shade_grid = (cap + kShadeBlock - 1) / kShadeBlock;
+
+    ctx->stats = pulse_stats{};
+    double ms_primary = 0, ms_incoherent = 0, ms_total = 0;
+    uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
+    std::vector<uint32_t> counts(2 * (kMaxTraces + 1));
+    for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
+        fa.accumulation_count = prm->accumulation_count + s;
+        HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * 2 * (kMaxTraces + 1), ctx->stream));
+        int e = 0;
+        if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+        hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, ctx->stream, wb, fa);
+        int q = 0;
+        for (uint32_t k = 0; k < traces; ++k) {
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+            PULSE_STACK_DISPATCH(launch_extend, ctx, wb, cap, q, static_cast<int>(k));
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+            hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, q,
+                               static_cast<int>(k));
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+            PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, cap, static_cast<int>(k));
+            q ^= 1;
+        }
+        if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+        hipLaunchKernelGGL(k_accumulate, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, wb,
+                           reinterpret_cast<float4 *>(accum_rgba), cap, fa.accumulation_count);
+        if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
+        HIP_TRY(hipGetLastError());
+        if (timing || (prm->flags & PULSE_FLAG_COLLECT_STATS)) {
+            HIP_TRY(hipMemcpyAsync(counts.data(), wb.ext_count, sizeof(uint32_t) * counts.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            for (uint32_t k = 0; k < traces; ++k) {
+                ext_total += counts[k];
+                sh_total += counts[kMaxTraces + 1 + k];
+            }
+            ext_primary += counts[0];
+        }
+        if (timing) {
+            auto el = [&](int a, int b) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]);
+                return static_cast<double>(ms);
+            };
+            ms_total += el(0, e - 1);
+            // events: 0 gen, then per k: [ext start, shade start, shadow start], end-of-shadow = next ext start
+            for (uint32_t k = 0; k < traces; ++k) {
+                int ext0 = 1 + 3 * static_cast<int>(k);
+                int shade0 = ext0 + 1, shadow0 = ext0 + 2, next0 = ext0 + 3;
+                double t_ext = el(ext0, shade0), t_sh = el(shadow0, next0);
+                if (k == 0) ms_primary += t_ext;
+                else ms_incoherent += t_ext;
+                ms_incoherent += t_sh;
+            }
+        }
+    }
+    ctx->stats.primary_rays = ext_primary;
+    ctx->stats.extension_rays = ext_total;
+    ctx->stats.shadow_rays = sh_total;
+    ctx->stats.incoherent_rays = (ext_total - ext_primary) + sh_total;
+    ctx->stats.stack_entries = ctx->stack_entries;
+    ctx->stats.ms_total = ms_total;
+    ctx->stats.ms_primary_extend = ms_primary;
+    ctx->stats.ms_incoherent = ms_incoherent;
+    ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
+    return PULSE_OK;
+}
+
+int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out) {
+    if (!ctx || !out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_stats: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    *out = ctx->stats;
+    out->error_flags = err;
+    return PULSE_OK;
+}
+
+}  // extern "C"
