@@ -1,0 +1,225 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+Workload (default): 64 displaced subdivision-6 icospheres (81,920 triangles
+each; 5.24M unique, 65.5M instanced triangles), 800 instances with random
+rigid transforms in a 100^3 box, 32 emissive instances; 1920x1080, 1 sample
+per pixel per step, 4 bounces (max_depth 4) with next event estimation
+(GGX sampling + Lambert NEE).  One step = one frame: generate, 4x(extend,
+shade, shadow), accumulate, with the scene already resident in HBM.
+
+value = incoherent Mrays/s: extension rays at depth >= 1 plus shadow rays,
+divided by the summed time of those stages (hipEvents around each stage on
+the library's stream, exact counts from the queue lengths).
+
+Prints one JSON line.  --gpus N > 1 is not supported on this build.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "incoherent Mrays/s + fwd+adjoint ms/iter vs plain fwd, 1 MI355X, 1080p"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of the traversal kernels, per unit of work.  Node pairs are
+# fetched together (both children of an interior node: 64 B); the TLAS root is
+# read once per ray and each entered BLAS reads its root (32 B), the instance
+# index (4 B) and the instance fields used (world_object 64 B, object_world
+# 64 B, node/index offsets 8 B); a triangle test reads the traversal-ordered
+# triangle record (48 B).
+B_NODE_PAIR = 64
+B_ROOT = 32
+B_INSTANCE = 32 + 4 + 64 + 64 + 8
+B_TRIANGLE = 48
+B_RAY_EXTEND = 32 + 20     # ray in, hit out
+B_RAY_SHADOW = 48 + 16     # ray + threshold + colour in, colour out
+
+
+def bytes_per_ray(work: dict, io: int) -> float:
+    r = max(work["rays"], 1)
+    return (B_ROOT * r + B_NODE_PAIR * (work["tlas_interior"] + work["blas_interior"])
+            + B_INSTANCE * work["instances_entered"] + B_TRIANGLE * work["tri_tests"]) / r + io
+
+
+def cpu_baseline(scene, cam, prm, seconds: float):
+    """The f32 oracle (scalar C++ restatement) timed on the host cores on a
+    bounded sample of rows of the same frame; also counts the traversal work
+    of the kernels' algorithm for the bytes model."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import OracleScene
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    orc = OracleScene(scene)
+    acc = np.zeros((cam.height, cam.width, 4), np.float32)
+    rows, ms_total = 0, 0.0
+    tot = {"rays": 0, "primary": 0, "shadow": 0}
+    step = max(1, threads // 2)
+    row = 0
+    while row < cam.height and ms_total < seconds * 1000.0:
+        r1 = min(cam.height, row + step)
+        _, c, ms = orc.render(cam, prm, accum=acc, threads=threads, rows=(row, r1))
+        ms_total += ms
+        tot["rays"] += c["rays"]
+        tot["primary"] += c["primary_rays"]
+        tot["shadow"] += c["shadow_rays"]
+        rows += r1 - row
+        row = r1
+        step = min(step * 2, 64)
+    incoherent = tot["rays"] - tot["primary"]
+    # work counters of the kernels' traversal on a few rows of the sample
+    orc.set_cull_world(True)
+    r0 = cam.height // 2
+    _, cw, _ = orc.render(cam, prm, accum=np.zeros_like(acc), threads=threads, rows=(r0, min(cam.height, r0 + 16)))
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    base = {
+        "value": incoherent / (ms_total / 1e3) / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rows} of {cam.height} rows of the same frame ({rows * cam.width} paths, {tot['rays']} rays, "
+                  f"{incoherent} incoherent) in {ms_total / 1e3:.1f} s with {threads} threads on {cpu_model}; "
+                  f"value = incoherent rays / whole render time",
+        "all_rays_mrays_per_s": tot["rays"] / (ms_total / 1e3) / 1e6,
+    }
+    return base, cw["by_category"]
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--subdiv", type=int, default=6)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--max-depth", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard pixel rows or views across GPUs, see the design notes).", file=sys.stderr)
+        return 2
+
+    from pulse_amd import _native as N
+    from pulse_amd import procedural as P
+    from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
+
+    t0 = time.perf_counter()
+    scene, camf = P.icosphere_field(subdiv=args.subdiv, threads=min(16, os.cpu_count() or 1))
+    host_build_s = time.perf_counter() - t0
+    cam = camf(args.width, args.height)
+    print(f"bench: scene built in {host_build_s:.1f} s: {scene.info['unique_triangles']} unique / "
+          f"{scene.info['instanced_triangles']} instanced triangles", file=sys.stderr, flush=True)
+
+    dev = Device(0)
+    t0 = time.perf_counter()
+    dev.upload_scene(scene)
+    dev.sync()
+    upload_s = time.perf_counter() - t0
+    node = PulsePathTracerNode(dev, args.width, args.height)
+    flags = N.FLAG_STAGE_TIMING | N.FLAG_COLLECT_STATS
+    frame = 0
+    for _ in range(args.warmup):
+        node.run(cam, make_params(accumulation_count=frame, max_depth=args.max_depth, mode=N.MODE_GGX_NEE, flags=flags))
+        frame += 1
+    dev.sync()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+
+    agg = {k: 0.0 for k in ["incoherent_rays", "extension_rays", "primary_rays", "shadow_rays", "ms_total",
+                            "ms_incoherent", "ms_primary_extend", "ms_extend_bounce", "ms_shadow", "ms_shade",
+                            "extend_launches", "shadow_launches"]}
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        node.run(cam, make_params(accumulation_count=frame, max_depth=args.max_depth, mode=N.MODE_GGX_NEE, flags=flags))
+        frame += 1
+        st = dev.stats()
+        if st.error_flags:
+            raise RuntimeError(f"device error flags {st.error_flags}")
+        for k in agg:
+            agg[k] += getattr(st, k)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = agg["incoherent_rays"] / (agg["ms_incoherent"] / 1e3) / 1e6
+    all_rays = agg["extension_rays"] + agg["shadow_rays"]
+    img = node.readback()
+    print(f"bench: {args.steps} frames, {ms_per_step:.2f} ms/frame, {value:.1f} incoherent Mrays/s", file=sys.stderr,
+          flush=True)
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural displaced icospheres, seeded)",
+        "config": {
+            "workload": f"icosphere_field: 64 BLAS x subdiv-{args.subdiv} displaced icospheres, 800 instances, "
+                        f"32 emissive, {args.width}x{args.height}, 1 spp/step, max_depth {args.max_depth}, "
+                        "GGX sampling + Lambert NEE, Russian roulette",
+            "unique_triangles": scene.info["unique_triangles"],
+            "instanced_triangles": scene.info["instanced_triangles"],
+            "width": args.width, "height": args.height, "spp_per_step": 1, "max_depth": args.max_depth,
+        },
+        "rays_per_step": {k: agg[k] / args.steps for k in ["primary_rays", "extension_rays", "shadow_rays",
+                                                            "incoherent_rays"]},
+        "stage_ms_per_step": {k: round(agg[k] / args.steps, 3) for k in ["ms_total", "ms_primary_extend",
+                                                                         "ms_extend_bounce", "ms_shadow", "ms_shade"]},
+        "all_rays_mrays_per_s": round(all_rays / wall / 1e6, 2),
+        "host_build_s": round(host_build_s, 2),
+        "upload_s": round(upload_s, 2),
+        "image_mean": float(np.mean(img[..., :3])),
+    }
+
+    if not args.no_cpu:
+        prm = make_params(accumulation_count=0, max_depth=args.max_depth, mode=N.MODE_GGX_NEE)
+        base, work = cpu_baseline(scene, cam, prm, args.cpu_seconds)
+        result["cpu_baseline"] = base
+        bpr = {"primary": bytes_per_ray(work["primary"], B_RAY_EXTEND),
+               "bounce": bytes_per_ray(work["bounce"], B_RAY_EXTEND),
+               "shadow": bytes_per_ray(work["shadow"], B_RAY_SHADOW)}
+        # dominant kernel by total stage time
+        cands = {
+            "k_extend (depth>=1)": (agg["ms_extend_bounce"], agg["extension_rays"] - agg["primary_rays"], bpr["bounce"],
+                                    agg["extend_launches"] - args.steps),
+            "k_shadow": (agg["ms_shadow"], agg["shadow_rays"], bpr["shadow"], agg["shadow_launches"]),
+            "k_extend (depth 0)": (agg["ms_primary_extend"], agg["primary_rays"], bpr["primary"], args.steps),
+        }
+        name, (ms, rays, b, launches) = max(cands.items(), key=lambda kv: kv[1][0])
+        achieved = rays * b / (ms / 1e3) / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                              "avg_launch_ms": round(ms / max(launches, 1), 4),
+                              "bytes_per_ray": {k: round(v, 1) for k, v in bpr.items()}}
+    print(json.dumps(result))
+    dev.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

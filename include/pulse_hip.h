@@ -208,18 +208,23 @@ typedef struct pulse_hit {
     float v;
 } pulse_hit;
 
-/* Per-frame counters (exact ray counts come from the queue lengths). */
+/* Per-call counters (exact ray counts come from the queue lengths), summed
+ * over the samples of the call.  Stage times need PULSE_FLAG_STAGE_TIMING. */
 typedef struct pulse_stats {
-    uint64_t primary_rays;
+    uint64_t primary_rays;     /* closest-hit rays at trace index 0        */
     uint64_t extension_rays;   /* all closest-hit rays incl. primary       */
     uint64_t incoherent_rays;  /* extension rays at depth >= 1 + shadow    */
     uint64_t shadow_rays;
     uint32_t error_flags;      /* bit 0 stack overflow, bit 1 iteration cap */
     uint32_t stack_entries;    /* traversal stack depth used by the kernels */
-    double ms_total;           /* whole trace call (stage timing flag)     */
-    double ms_primary_extend;  /* depth-0 closest-hit stage                */
-    double ms_incoherent;      /* depth>=1 closest-hit + shadow stages     */
+    double ms_total;           /* generate .. accumulate, all samples      */
+    double ms_primary_extend;  /* closest-hit stage at trace index 0       */
+    double ms_incoherent;      /* ms_extend_bounce + ms_shadow             */
     double ms_shade;           /* generate + shade + accumulate stages     */
+    double ms_extend_bounce;   /* closest-hit stages at trace index >= 1   */
+    double ms_shadow;          /* occlusion stages                         */
+    uint32_t extend_launches;  /* closest-hit stage launches               */
+    uint32_t shadow_launches;  /* occlusion stage launches                 */
 } pulse_stats;
 
 /* ------------------------------------------------------------------------ */

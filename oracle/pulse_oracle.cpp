@@ -46,6 +46,7 @@ void export_counters(const std::vector<Counters> &per_thread, uint64_t *out) {
     uint64_t v[11] = {c.tlas_interior, c.tlas_leaf, c.blas_interior, c.blas_leaf, c.instances_entered,
                       c.tri_tests, c.rays, c.max_stack, c.iteration_cap_hits, c.shadow_rays, c.primary_rays};
     std::memcpy(out, v, sizeof(v));
+    std::memcpy(out + 11, c.cat, sizeof(c.cat));
 }
 
 }  // namespace
@@ -101,6 +102,8 @@ void oracle_scene_set_cdf_linear_scan(oracle_scene *sc, int on) {
     reinterpret_cast<OwnedScene *>(sc)->s.cdf_linear_scan = on != 0;
 }
 
+void oracle_scene_set_cull_world(oracle_scene *sc, int on) { reinterpret_cast<OwnedScene *>(sc)->s.cull_world = on != 0; }
+
 static const Scene &S(oracle_scene *sc) { return reinterpret_cast<OwnedScene *>(sc)->s; }
 
 // Closest hit per ray, f32.  counters: 11 u64 (see export_counters).
@@ -155,6 +158,10 @@ int oracle_shadow_rays_f32(oracle_scene *sc, const pulse_ray *rays, uint32_t *oc
         r.origin = load3<float>(rays[i].origin);
         r.dir = load3<float>(rays[i].dir);
         r.t = kTFar;
+        if (s.cull_world) {
+            occluded[i] = traverse_tlas_anyhit(s, r, rays[i].t_max, pc[tid]) ? 1u : 0u;
+            return;
+        }
         traverse_tlas(s, r, pc[tid]);
         occluded[i] = r.t < rays[i].t_max ? 1u : 0u;
     });

@@ -164,6 +164,12 @@ struct Scene {
     const pulse_light_data_index *light_indices = nullptr;
     size_t n_primitives = 0, n_instances = 0, n_tlas = 0, n_blas = 0;
     bool cdf_linear_scan = false;  // true: run the reference's O(n) scan as written
+    // true (f32 only): the kernels' traversal -- each BLAS starts from a
+    // conservative object-space bound derived from the current world-space
+    // closest hit (or occlusion threshold) and shadow rays stop at the first
+    // occluding instance.  Results must equal the reference traversal; the
+    // work counters then describe what the kernels execute.
+    bool cull_world = false;
 };
 
 // Per-ray traversal counters (the algorithmic-bytes model counts these).
@@ -179,7 +185,12 @@ struct Counters {
     uint64_t iteration_cap_hits = 0;
     uint64_t shadow_rays = 0;     // NEE occlusion rays
     uint64_t primary_rays = 0;    // depth-0 closest-hit rays
+    // Work per ray category (0 primary, 1 bounce, 2 shadow):
+    // {tlas_interior, tlas_leaf, blas_interior, blas_leaf, instances, tri_tests, rays}
+    uint64_t cat[3][7] = {};
     void add(const Counters &o) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 7; ++b) cat[a][b] += o.cat[a][b];
         tlas_interior += o.tlas_interior;
         tlas_leaf += o.tlas_leaf;
         blas_interior += o.blas_interior;
@@ -191,6 +202,23 @@ struct Counters {
         iteration_cap_hits += o.iteration_cap_hits;
         shadow_rays += o.shadow_rays;
         primary_rays += o.primary_rays;
+    }
+};
+
+// Adds the traversal work done while it is alive to category `k`.
+struct CatScope {
+    Counters &c;
+    int k;
+    uint64_t b[7];
+    CatScope(Counters &c_, int k_) : c(c_), k(k_) { snap(b); }
+    void snap(uint64_t *v) const {
+        v[0] = c.tlas_interior; v[1] = c.tlas_leaf; v[2] = c.blas_interior; v[3] = c.blas_leaf;
+        v[4] = c.instances_entered; v[5] = c.tri_tests; v[6] = c.rays;
+    }
+    ~CatScope() {
+        uint64_t a[7];
+        snap(a);
+        for (int i = 0; i < 7; ++i) c.cat[k][i] += a[i] - b[i];
     }
 };
 
@@ -263,14 +291,38 @@ constexpr int kMaxIterations = 100000;  // utilities.wgsl:137, 271
 constexpr int kOracleStack = 256;       // reference: 32 (no overflow check)
 
 // traverse_blas, utilities.wgsl:259-326
+// World t computed by the last traverse_blas call (any-hit bookkeeping).
+inline thread_local double ray_world_t_last = 0.0;
+
+// Conservative object-space bound for a world-space bound (culling mode):
+// the world bound is padded by 2^-12 of max(|origin|, bound) + 2^-20 to cover
+// the rounding of the reference's world-t conversion, converted with the
+// length of the transformed direction and padded by 2^-10.
+inline float cull_bound(V3<float> o, float world_bound, float dlen) {
+    if (!(world_bound < kTFar)) return kTFar;
+    float m = std::fmax(std::fmax(std::fabs(o.x), std::fabs(o.y)), std::fmax(std::fabs(o.z), world_bound));
+    float wb = world_bound + (m * 0x1p-12f + 0x1p-20f);
+    float ob = wb * dlen * (1.0f + 0x1p-10f);
+    return ob < kTFar ? ob : kTFar;
+}
+
 template <class R>
-void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c) {
+void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c, R world_bound = sentinel<R>()) {
     c.instances_entered++;
     const pulse_mesh_instance &mi = s.instances[inst];
     Ray<R> ro;
     ro.origin = transform_position<R>(mi.world_object, ray.origin);
-    ro.dir = normalize(transform_direction<R>(mi.world_object, ray.dir));
+    V3<R> draw = transform_direction<R>(mi.world_object, ray.dir);
+    ro.dir = normalize(draw);
     ro.t = sentinel<R>();
+    bool culling = false;
+    if constexpr (std::is_same<R, float>::value) {
+        if (s.cull_world) {
+            ro.t = cull_bound(ray.origin, world_bound, length(draw));
+            culling = true;
+        }
+    }
+    const R start_t = ro.t;
 
     uint32_t node_index = 0;
     uint32_t stack[kOracleStack];
@@ -316,9 +368,11 @@ void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c) {
     // miss (t = 1e30) overflows to +inf here and never updates; f64 does not
     // overflow, so the miss is tested explicitly (same result).
     if (!std::is_same<R, float>::value && ro.t == sentinel<R>()) return;
+    if (culling && ro.t == start_t) ro.t = sentinel<R>();  // nothing below the bound: as a miss
     V3<R> hit_obj = ro.origin + ro.t * ro.dir;
     V3<R> hit_world = transform_position<R>(mi.object_world, hit_obj);
     R new_t = length(hit_world - ray.origin);
+    ray_world_t_last = static_cast<double>(new_t);
     if (new_t < ray.t) {
         ray.t = new_t;
         ray.instance_index = ro.instance_index;
@@ -343,7 +397,7 @@ void traverse_tlas(const Scene &s, Ray<R> &ray, Counters &c) {
         if (node.instance_count > 0u) {
             c.tlas_leaf++;
             for (uint32_t i = 0; i < node.instance_count; ++i) {
-                traverse_blas(s, ray, s.instance_indices[node.a_or_first_instance + i], c);
+                traverse_blas(s, ray, s.instance_indices[node.a_or_first_instance + i], c, ray.t);
             }
             if (sp == 0) break;
             sp -= 1;
@@ -372,6 +426,55 @@ void traverse_tlas(const Scene &s, Ray<R> &ray, Counters &c) {
         }
     }
     if (iteration >= kMaxIterations) c.iteration_cap_hits++;
+}
+
+// The kernels' occlusion query (culling mode): TLAS boxes culled at a padded
+// threshold, each entered BLAS searched for its closest hit below the padded
+// bound, stop at the first instance whose world t is below the threshold.
+// Equals `closest t < threshold` of the full traversal.
+inline bool traverse_tlas_anyhit(const Scene &s, Ray<float> &ray, float threshold, Counters &c) {
+    c.rays++;
+    if (s.tlas_nodes[0].a_or_first_instance == 0u && s.tlas_nodes[0].instance_count == 0u) return false;
+    float m = std::fmax(std::fmax(std::fabs(ray.origin.x), std::fabs(ray.origin.y)),
+                        std::fmax(std::fabs(ray.origin.z), threshold));
+    const float cull_t = threshold + (m * 0x1p-12f + 0x1p-20f);
+    ray.t = cull_t;
+    uint32_t node_index = 0;
+    uint32_t stack[kOracleStack];
+    int sp = 0;
+    int iteration = 0;
+    while (iteration < kMaxIterations) {
+        iteration += 1;
+        const pulse_tlas_node &node = s.tlas_nodes[node_index];
+        if (node.instance_count > 0u) {
+            c.tlas_leaf++;
+            for (uint32_t i = 0; i < node.instance_count; ++i) {
+                Ray<float> probe = ray;
+                probe.t = kTFar;
+                traverse_blas(s, probe, s.instance_indices[node.a_or_first_instance + i], c, cull_t);
+                if (ray_world_t_last < static_cast<double>(threshold)) return true;
+            }
+            if (sp == 0) break;
+            node_index = stack[--sp];
+            continue;
+        }
+        c.tlas_interior++;
+        uint32_t a = node.a_or_first_instance, b = a + 1u;
+        float da = ray_aabb_intersect(ray, s.tlas_nodes[a].aabb_min, s.tlas_nodes[a].aabb_max);
+        float db = ray_aabb_intersect(ray, s.tlas_nodes[b].aabb_min, s.tlas_nodes[b].aabb_max);
+        if (da > db) {
+            std::swap(da, db);
+            std::swap(a, b);
+        }
+        if (da == kTFar) {
+            if (sp == 0) break;
+            node_index = stack[--sp];
+        } else {
+            node_index = a;
+            if (db != kTFar) stack[sp++] = b;
+        }
+    }
+    return false;
 }
 
 // Brute force closest hit: every instance, every triangle of its mesh, with
@@ -690,9 +793,14 @@ ShadowSetup<R> nee_setup(const Scene &s, V3<R> p0, V3<R> n0, uint32_t &rng, Deci
 template <class R>
 bool nee_occluded(const Scene &s, ShadowSetup<R> &st, Counters &c, DecisionTape &tape) {
     c.shadow_rays++;
+    CatScope scope(c, 2);
     Ray<R> r = st.ray;
+    R threshold = length(st.pl - st.ray.origin) - R(0.005f);
+    if constexpr (std::is_same<R, float>::value) {
+        if (s.cull_world) return tape.take_bool(traverse_tlas_anyhit(s, r, threshold, c));
+    }
     traverse_tlas(s, r, c);
-    return tape.take_bool(r.t < length(st.pl - st.ray.origin) - R(0.005f));
+    return tape.take_bool(r.t < threshold);
 }
 
 // Light normal from the object-space triangle, flipped to face the receiver
@@ -821,7 +929,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
     if (prm.mode == PULSE_MODE_REF_NEE_LAMBERT) {
         // path_traceBBB (path_tracer.wgsl:128-202) with the G-buffer sample
         // replaced by the camera ray's first hit.
-        trace_closest(s, ray, c, tape);
+        { CatScope scope(c, 0); trace_closest(s, ray, c, tape); }
         if (ray.t >= sentinel<R>()) return color;
         HitShading<R> h0 = shade_hit(s, ray);
         MaterialR<R> m0 = load_material<R>(s.materials[h0.material_index]);
@@ -838,7 +946,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
         throughput = m0.base;
         ray = next;
         for (uint32_t depth = 0; depth < prm.max_depth; ++depth) {
-            trace_closest(s, ray, c, tape);
+            { CatScope scope(c, 1); trace_closest(s, ray, c, tape); }
             if (ray.t >= sentinel<R>()) break;
             HitShading<R> h = shade_hit(s, ray);
             MaterialR<R> m = load_material<R>(s.materials[h.material_index]);
@@ -858,7 +966,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
     }
 
     for (uint32_t depth = 0; depth < prm.max_depth; ++depth) {
-        trace_closest(s, ray, c, tape);
+        { CatScope scope(c, depth == 0 ? 0 : 1); trace_closest(s, ray, c, tape); }
         if (ray.t >= sentinel<R>()) break;
         HitShading<R> h = shade_hit(s, ray);
         MaterialR<R> m = load_material<R>(s.materials[h.material_index]);

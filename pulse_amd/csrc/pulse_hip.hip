@@ -513,7 +513,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     const uint32_t shade_grid = (cap + kShadeBlock - 1) / kShadeBlock;
 
     ctx->stats = pulse_stats{};
-    double ms_primary = 0, ms_incoherent = 0, ms_total = 0;
+    double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0;
     uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
     std::vector<uint32_t> counts(2 * (kMaxTraces + 1));
     for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
@@ -561,8 +561,8 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
                 int shade0 = ext0 + 1, shadow0 = ext0 + 2, next0 = ext0 + 3;
                 double t_ext = el(ext0, shade0), t_sh = el(shadow0, next0);
                 if (k == 0) ms_primary += t_ext;
-                else ms_incoherent += t_ext;
-                ms_incoherent += t_sh;
+                else ms_ext_bounce += t_ext;
+                ms_shadow += t_sh;
             }
         }
     }
@@ -573,8 +573,12 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.stack_entries = ctx->stack_entries;
     ctx->stats.ms_total = ms_total;
     ctx->stats.ms_primary_extend = ms_primary;
-    ctx->stats.ms_incoherent = ms_incoherent;
-    ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
+    ctx->stats.ms_incoherent = ms_ext_bounce + ms_shadow;
+    ctx->stats.ms_extend_bounce = ms_ext_bounce;
+    ctx->stats.ms_shadow = ms_shadow;
+    ctx->stats.ms_shade = ms_total - ms_primary - ms_ext_bounce - ms_shadow;
+    ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
+    ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
     return PULSE_OK;
 }
 
