@@ -112,7 +112,20 @@ def main() -> int:
     ap.add_argument("--max-depth", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard pixel rows or views across GPUs, see the design notes).", file=sys.stderr)

@@ -52,7 +52,7 @@ def test_trace_rays_bit_exact(dev, cornell, field, which):
     o, _ = OracleScene(scene).trace_rays(rays)
     ok = _hits_equal(g, o)
     assert ok.all(), f"{(~ok).sum()} of {len(ok)} hits differ; first {np.flatnonzero(~ok)[:5]}"
-    assert (g["t"] < 1e30).mean() > 0.5
+    assert (g["t"] < 1e30).mean() > 0.3
 
 
 def test_trace_rays_axis_parallel_and_tmax(dev, cornell):
