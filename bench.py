@@ -112,6 +112,7 @@ def main() -> int:
     ap.add_argument("--max-depth", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--loop", type=int, default=1, help="traversal loop variant (0 nested, 1 single loop)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -143,6 +144,7 @@ def main() -> int:
           f"{scene.info['instanced_triangles']} instanced triangles", file=sys.stderr, flush=True)
 
     dev = Device(0)
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, args.loop)
     t0 = time.perf_counter()
     dev.upload_scene(scene)
     dev.sync()
@@ -206,6 +208,9 @@ def main() -> int:
         "all_rays_mrays_per_s": round(all_rays / wall / 1e6, 2),
         "host_build_s": round(host_build_s, 2),
         "upload_s": round(upload_s, 2),
+        "traversal_loop": args.loop,
+        "stack_entries": dev.get_option(N.OPT_STACK_ENTRIES),
+        "fast_division": dev.get_option(N.OPT_FAST_DIVISION),
         "image_mean": float(np.mean(img[..., :3])),
     }
 

@@ -280,6 +280,15 @@ int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uin
  * threshold |pl - origin| - 0.005 passed in t_max). */
 int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occluded, uint32_t n);
 
+/* Tuning options of a context (defaults are the measured best on MI355X). */
+#define PULSE_OPT_TRAVERSAL_LOOP 1 /* 0: nested TLAS/BLAS loops as in the reference, */
+                                   /* 1: one loop over both levels (default)         */
+#define PULSE_OPT_STACK_ENTRIES 2  /* read-only: LDS traversal stack entries in use  */
+#define PULSE_OPT_FAST_DIVISION 3  /* read-only: 1 if the scene's coordinates allow   */
+                                   /* the exact reciprocal-multiply slab division     */
+int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
+int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
+
 /* Counters of the last pulse_trace call (synchronizes the stream). */
 int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out);
 

@@ -306,6 +306,10 @@ inline float cull_bound(V3<float> o, float world_bound, float dlen) {
     return ob < kTFar ? ob : kTFar;
 }
 
+// Object t below which a hit certainly converts to a world t below
+// `threshold` (occlusion early exit of the kernels); -1 disables it.
+inline thread_local float blas_safe_t_world = -1.0f;
+
 template <class R>
 void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c, R world_bound = sentinel<R>()) {
     c.instances_entered++;
@@ -316,10 +320,13 @@ void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c, R wo
     ro.dir = normalize(draw);
     ro.t = sentinel<R>();
     bool culling = false;
+    float safe_t = -1.0f;
     if constexpr (std::is_same<R, float>::value) {
         if (s.cull_world) {
-            ro.t = cull_bound(ray.origin, world_bound, length(draw));
+            float dlen = length(draw);
+            ro.t = cull_bound(ray.origin, world_bound, dlen);
             culling = true;
+            if (blas_safe_t_world > 0.0f) safe_t = blas_safe_t_world * dlen * (1.0f - 0x1p-10f);
         }
     }
     const R start_t = ro.t;
@@ -334,6 +341,10 @@ void traverse_blas(const Scene &s, Ray<R> &ray, uint32_t inst, Counters &c, R wo
         if (node.tri_count > 0u) {
             c.blas_leaf++;
             for (uint32_t i = 0; i < node.tri_count; ++i) ray_triangle_intersect(s, ro, node.a_or_first_tri + i, inst, c);
+            if (culling && safe_t > 0.0f && ro.t < safe_t) {
+                ray_world_t_last = -1.0;  // certainly below the threshold
+                return;
+            }
             if (sp == 0) break;
             sp -= 1;
             node_index = stack[sp];
@@ -437,8 +448,13 @@ inline bool traverse_tlas_anyhit(const Scene &s, Ray<float> &ray, float threshol
     if (s.tlas_nodes[0].a_or_first_instance == 0u && s.tlas_nodes[0].instance_count == 0u) return false;
     float m = std::fmax(std::fmax(std::fabs(ray.origin.x), std::fabs(ray.origin.y)),
                         std::fmax(std::fabs(ray.origin.z), threshold));
-    const float cull_t = threshold + (m * 0x1p-12f + 0x1p-20f);
+    const float slack = m * 0x1p-12f + 0x1p-20f;
+    const float cull_t = threshold + slack;
     ray.t = cull_t;
+    struct SafeScope {
+        explicit SafeScope(float v) { blas_safe_t_world = v; }
+        ~SafeScope() { blas_safe_t_world = -1.0f; }
+    } safe_scope(threshold - slack > 0.0f ? threshold - slack : -1.0f);
     uint32_t node_index = 0;
     uint32_t stack[kOracleStack];
     int sp = 0;

@@ -33,6 +33,9 @@ NEE_LAMBERT = 0
 NEE_GGX = 1
 FLAG_COLLECT_STATS = 1
 FLAG_STAGE_TIMING = 2
+OPT_TRAVERSAL_LOOP = 1
+OPT_STACK_ENTRIES = 2
+OPT_FAST_DIVISION = 3
 
 
 class PulseError(RuntimeError):
@@ -201,6 +204,8 @@ SIGNATURES = [
     ("pulse_trace_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_trace_shadow_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_get_stats", _I, [_P, C.POINTER(Stats)]),
+    ("pulse_set_option", _I, [_P, _I, C.c_int64]),
+    ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),
     ("pulse_build_blas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),
     ("pulse_build_tlas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),
     ("pulse_sb_create", _I, [C.POINTER(_P)]),

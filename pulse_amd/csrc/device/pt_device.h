@@ -52,6 +52,9 @@ struct DevScene {
     uint32_t instance_count;
     uint32_t light_count;
     uint32_t *error_flags;
+    // every node coordinate is 0 or has magnitude in [2^-56, 2^40] (the
+    // range in which the slab test may divide by reciprocal multiplication)
+    bool fast_div;
 };
 
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "pt_device.h"
+#include "traverse.h"
 
 namespace pt {
 
@@ -41,6 +42,15 @@ struct WfBuffers {
     uint32_t *sh_count;   // [kMaxTraces + 1]
 };
 constexpr int kMaxTraces = 64;
+
+// Traversal variants: 0 = the reference's nested TLAS/BLAS loops,
+// 1 = single loop over both levels (traverse.h).  Same results.
+template <int LOOP, bool ANY_HIT>
+__device__ __forceinline__ bool trace_any(const DevScene &sc, V3 o, V3 d, float &t, uint32_t &inst, uint32_t &tri,
+                                          float &u, float &v, float threshold, LdsStack &stk) {
+    if (LOOP == 1) return trace_unified<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
+    return traverse_tlas<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
+}
 
 struct FrameArgs {
     pulse_camera cam;
@@ -89,7 +99,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_generate(WfBuffers wb, FrameArg
 }
 
 // extend: closest hit for queue `q` at trace index k.
-template <int BLOCK, int STACK>
+template <int BLOCK, int STACK, int LOOP>
 __global__ void __launch_bounds__(BLOCK) k_extend(DevScene sc, WfBuffers wb, int q, int k) {
     __shared__ uint32_t smem[STACK * BLOCK];
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -99,7 +109,7 @@ __global__ void __launch_bounds__(BLOCK) k_extend(DevScene sc, WfBuffers wb, int
     V3 o = mk(oa.x, oa.y, oa.z), d = mk(da.x, da.y, da.z);
     float t = kTFar, u = 0.0f, v = 0.0f;
     uint32_t inst = 0, tri = 0;
-    traverse_tlas<false>(sc, o, d, t, inst, tri, u, v, 0.0f, stk);
+    trace_any<LOOP, false>(sc, o, d, t, inst, tri, u, v, 0.0f, stk);
     wb.hit[i] = make_float4(t, __uint_as_float(inst), __uint_as_float(tri), u);
     wb.hit_v[i] = v;
 }
@@ -232,7 +242,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, WfBuffers wb
 
 // shadow: occlusion test of the NEE rays of trace index k; a visible light
 // replaces the pixel colour by the one that includes its contribution.
-template <int BLOCK, int STACK>
+template <int BLOCK, int STACK, int LOOP>
 __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int k) {
     __shared__ uint32_t smem[STACK * BLOCK];
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -242,7 +252,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
     V3 o = mk(oa.x, oa.y, oa.z), d = mk(da.x, da.y, da.z);
     float t = kTFar, u = 0.0f, v = 0.0f;
     uint32_t inst = 0, tri = 0;
-    bool occluded = traverse_tlas<true>(sc, o, d, t, inst, tri, u, v, da.w, stk);
+    bool occluded = trace_any<LOOP, true>(sc, o, d, t, inst, tri, u, v, da.w, stk);
     if (!occluded) {
         float4 c = wb.sh_c[i];
         wb.state_c[__float_as_uint(oa.w)] = c;
@@ -267,7 +277,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_accumulate(WfBuffers wb, float4
 }
 
 // Debug/parity entry: closest hit per ray.
-template <int BLOCK, int STACK>
+template <int BLOCK, int STACK, int LOOP>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays(DevScene sc, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
     __shared__ uint32_t smem[STACK * BLOCK];
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -277,7 +287,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DevScene sc, const pulse_r
     V3 o = mk(r.origin[0], r.origin[1], r.origin[2]), d = mk(r.dir[0], r.dir[1], r.dir[2]);
     float t = r.t_max, u = 0.0f, v = 0.0f;
     uint32_t inst = 0, tri = 0;
-    traverse_tlas<false>(sc, o, d, t, inst, tri, u, v, 0.0f, stk);
+    trace_any<LOOP, false>(sc, o, d, t, inst, tri, u, v, 0.0f, stk);
     pulse_hit h;
     h.t = t;
     h.instance_index = inst;
@@ -287,7 +297,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DevScene sc, const pulse_r
     hits[i] = h;
 }
 
-template <int BLOCK, int STACK>
+template <int BLOCK, int STACK, int LOOP>
 __global__ void __launch_bounds__(BLOCK) k_shadow_rays(DevScene sc, const pulse_ray *rays, uint32_t *occ, uint32_t n) {
     __shared__ uint32_t smem[STACK * BLOCK];
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -297,7 +307,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_rays(DevScene sc, const pulse_
     V3 o = mk(r.origin[0], r.origin[1], r.origin[2]), d = mk(r.dir[0], r.dir[1], r.dir[2]);
     float t = kTFar, u = 0.0f, v = 0.0f;
     uint32_t inst = 0, tri = 0;
-    occ[i] = traverse_tlas<true>(sc, o, d, t, inst, tri, u, v, r.t_max, stk) ? 1u : 0u;
+    occ[i] = trace_any<LOOP, true>(sc, o, d, t, inst, tri, u, v, r.t_max, stk) ? 1u : 0u;
 }
 
 }  // namespace pt

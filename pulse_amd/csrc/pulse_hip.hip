@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -69,6 +70,9 @@ struct pulse_ctx {
     uint64_t generation = 0;
     pulse_scene_uniform uniform{};
     uint32_t stack_entries = 0;  // traversal stack capacity in use
+    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
+    uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
+    bool fast_div = false;       // node coordinates allow the exact fast division
     uint32_t tlas_depth = 0, blas_depth = 0;
     pulse_stats stats{};
     uint32_t last_traces = 0;
@@ -86,10 +90,16 @@ int set_device(pulse_ctx *ctx) {
 // Smallest compiled stack size that holds tlas_depth + blas_depth entries:
 // a depth-first traversal that pushes at most one far child per level never
 // holds more entries than the depth of the node it is at.
-uint32_t pick_stack(uint32_t need) {
-    const uint32_t sizes[] = {16, 24, 32, 48, 64, 96, 128};
-    for (uint32_t s : sizes)
-        if (s >= need) return s;
+uint32_t pick_stack(uint32_t need, int loop_variant) {
+    static const uint32_t nested[] = {32, 48, 64, 128};
+    static const uint32_t unified[] = {24, 32, 40, 48, 64, 128};
+    if (loop_variant == 0) {
+        for (uint32_t s : nested)
+            if (s >= need) return s;
+    } else {
+        for (uint32_t s : unified)
+            if (s >= need) return s;
+    }
     return 0;
 }
 
@@ -159,46 +169,58 @@ DevScene dev_scene(pulse_ctx *ctx) {
     s.instance_count = ctx->uniform.instance_count;
     s.light_count = ctx->uniform.light_count;
     s.error_flags = ctx->errors.as<uint32_t>();
+    s.fast_div = ctx->fast_div;
     return s;
 }
 
 constexpr int kTraceBlock = 64;  // one wave per block: LDS stack granularity
 
-template <int STACK>
+template <int STACK, int LOOP>
 void launch_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
     uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL((k_trace_rays<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_trace_rays<kTraceBlock, STACK, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
                        dev_scene(ctx), rays, hits, n);
 }
-template <int STACK>
+template <int STACK, int LOOP>
 void launch_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ, uint32_t n) {
     uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL((k_shadow_rays<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_shadow_rays<kTraceBlock, STACK, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
                        dev_scene(ctx), rays, occ, n);
 }
-template <int STACK>
+template <int STACK, int LOOP>
 void launch_extend(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int q, int k) {
     uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL((k_extend<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_extend<kTraceBlock, STACK, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
                        dev_scene(ctx), wb, q, k);
 }
-template <int STACK>
+template <int STACK, int LOOP>
 void launch_shadow(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int k) {
     uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL((k_shadow<kTraceBlock, STACK>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_shadow<kTraceBlock, STACK, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
                        dev_scene(ctx), wb, k);
 }
 
-#define PULSE_STACK_DISPATCH(fn, ctx, ...)                  \
-    switch ((ctx)->stack_entries) {                         \
-        case 16: fn<16>(ctx, __VA_ARGS__); break;           \
-        case 24: fn<24>(ctx, __VA_ARGS__); break;           \
-        case 32: fn<32>(ctx, __VA_ARGS__); break;           \
-        case 48: fn<48>(ctx, __VA_ARGS__); break;           \
-        case 64: fn<64>(ctx, __VA_ARGS__); break;           \
-        case 96: fn<96>(ctx, __VA_ARGS__); break;           \
-        default: fn<128>(ctx, __VA_ARGS__); break;          \
-    }
+// Instantiates a launcher for the context's stack size and traversal loop.
+#define PULSE_STACK_DISPATCH(fn, ctx, ...)                                           \
+    do {                                                                             \
+        if ((ctx)->loop_variant == 0) {                                              \
+            switch ((ctx)->stack_entries) {                                          \
+                case 32: fn<32, 0>(ctx, __VA_ARGS__); break;                         \
+                case 48: fn<48, 0>(ctx, __VA_ARGS__); break;                         \
+                case 64: fn<64, 0>(ctx, __VA_ARGS__); break;                         \
+                default: fn<128, 0>(ctx, __VA_ARGS__); break;                        \
+            }                                                                        \
+        } else {                                                                     \
+            switch ((ctx)->stack_entries) {                                          \
+                case 24: fn<24, 1>(ctx, __VA_ARGS__); break;                         \
+                case 32: fn<32, 1>(ctx, __VA_ARGS__); break;                         \
+                case 40: fn<40, 1>(ctx, __VA_ARGS__); break;                         \
+                case 48: fn<48, 1>(ctx, __VA_ARGS__); break;                         \
+                case 64: fn<64, 1>(ctx, __VA_ARGS__); break;                         \
+                default: fn<128, 1>(ctx, __VA_ARGS__); break;                        \
+            }                                                                        \
+        }                                                                            \
+    } while (0)
 
 int ensure_wavefront(pulse_ctx *ctx, size_t npix) {
     if (npix <= ctx->wf_pixels) return PULSE_OK;
@@ -423,7 +445,17 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
             PULSE_FAIL(PULSE_E_LAYOUT, "light %zu: cdf range outside light_triangle_area_cdfs", l);
     }
     uint32_t need = tlas_depth + blas_depth + 1;
-    uint32_t stack = pick_stack(need);
+    uint32_t stack = pick_stack(need, ctx->loop_variant);
+    // node coordinates: 0 or magnitude in [2^-56, 2^40] for the fast division
+    auto coord_ok = [](float c) {
+        float a = std::fabs(c);
+        return c == 0.0f || (a >= 0x1p-56f && a <= 0x1p40f);
+    };
+    bool fast_div = true;
+    for (size_t k = 0; k < n_blas && fast_div; ++k)
+        for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(blas[k].aabb_min[c]) && coord_ok(blas[k].aabb_max[c]);
+    for (size_t k = 0; k < n_tlas && fast_div; ++k)
+        for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(tlas[k].aabb_min[c]) && coord_ok(tlas[k].aabb_max[c]);
     if (stack == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
                    tlas_depth, blas_depth);
@@ -445,6 +477,8 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
         return rc;
     ctx->uniform = d->uniform;
     ctx->stack_entries = stack;
+    ctx->stack_need = need;
+    ctx->fast_div = fast_div;
     ctx->tlas_depth = tlas_depth;
     ctx->blas_depth = blas_depth;
     ctx->has_scene = true;
@@ -580,6 +614,32 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
     return PULSE_OK;
+}
+
+int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: null ctx");
+    switch (option) {
+        case PULSE_OPT_TRAVERSAL_LOOP: {
+            if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "traversal loop must be 0 or 1");
+            uint32_t st = ctx->has_scene ? pick_stack(ctx->stack_need, static_cast<int>(value)) : 1;
+            if (st == 0) PULSE_FAIL(PULSE_E_UNSUPPORTED, "no stack size for this variant and scene");
+            ctx->loop_variant = static_cast<int>(value);
+            if (ctx->has_scene) ctx->stack_entries = st;
+            return PULSE_OK;
+        }
+        default:
+            PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
+    }
+}
+
+int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
+    if (!ctx || !value) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: null argument");
+    switch (option) {
+        case PULSE_OPT_TRAVERSAL_LOOP: *value = ctx->loop_variant; return PULSE_OK;
+        case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
+        case PULSE_OPT_FAST_DIVISION: *value = ctx->fast_div ? 1 : 0; return PULSE_OK;
+        default: PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: unknown option %d", option);
+    }
 }
 
 int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out) {

@@ -104,6 +104,14 @@ class Device:
         do.copy_to(occ)
         return occ
 
+    def set_option(self, option: int, value: int) -> None:
+        N.check(N.lib().pulse_set_option(self._ctx, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int64()
+        N.check(N.lib().pulse_get_option(self._ctx, option, C.byref(v)))
+        return v.value
+
     def stats(self) -> N.Stats:
         s = N.Stats()
         N.check(N.lib().pulse_get_stats(self._ctx, C.byref(s)))

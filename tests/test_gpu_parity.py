@@ -174,3 +174,26 @@ def test_full_size_scene_rows(dev):
         orc.render(cam, prm, accum=o, rows=(r0, r0 + 3))
         ok = same_bits(g[r0:r0 + 3], o[r0:r0 + 3])
         assert ok.all(), f"rows {r0}: {(~ok).sum()} channels differ"
+
+
+@pytest.mark.parametrize("loop", [0, 1])
+def test_traversal_variants_bit_exact(dev, field, loop):
+    """Both traversal loops (nested as in the reference, single loop) give
+    the oracle's hits, occlusion and image."""
+    scene, camf = field
+    dev.upload_scene(scene)
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        rays = random_rays(8000, np.random.default_rng(9), (0, 0, 0), 20.0)
+        g = dev.trace_rays(rays)
+        orc = OracleScene(scene)
+        o, _ = orc.trace_rays(rays)
+        assert _hits_equal(g, o).all()
+        cam = camf(80, 48)
+        prm = make_params(spp=1, max_depth=4, mode=N.MODE_GGX_NEE)
+        node = PulsePathTracerNode(dev, 80, 48)
+        node.run(cam, prm)
+        oi, _, _ = orc.render(cam, prm)
+        assert same_bits(node.readback(), oi).all()
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
