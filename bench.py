@@ -101,6 +101,46 @@ def cpu_baseline(scene, cam, prm, seconds: float):
     return base, cw["by_category"]
 
 
+def bench_adjoint(scene, camf, iters: int = 5) -> dict:
+    """Differentiable form at 512x512, 16 spp, 3 bounces (GGX + Lambert NEE,
+    no Russian roulette) on the same geometry with 64 materials: the target
+    is rendered at 64 spp from hidden parameters (albedo +-0.3, roughness
+    +-0.3 from the start values); timed: plain forward vs forward + loss +
+    adjoint."""
+    from pulse_amd import _native as N
+    from pulse_amd.path_tracer import Device, PulseAdjoint, apply_material_params, make_params, material_params
+
+    dev = Device(0)
+    dev.upload_scene(scene)
+    cam = camf(512, 512)
+    mats = scene.buffers["materials"]
+    nm = len(mats)
+    adj = PulseAdjoint(dev, 512, 512, nm)
+    rng = np.random.default_rng(4)
+    start = material_params(mats)
+    hidden = start.copy()
+    hidden[:, 0:3] = np.clip(start[:, 0:3] + rng.uniform(-0.3, 0.3, (nm, 3)), 0.02, 0.98)
+    hidden[:, 3] = np.clip(start[:, 3] + rng.uniform(-0.3, 0.3, nm), 0.05, 1.0)
+    adj.set_materials(apply_material_params(mats, hidden))
+    adj.set_target(adj.forward(cam, make_params(spp=64, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3)))
+    adj.set_materials(mats)
+    prm = make_params(spp=16, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3)
+    adj.forward(cam, prm)
+    adj.forward_backward(cam, prm)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        adj.forward(cam, prm)
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / iters
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        loss, g = adj.forward_backward(cam, prm)
+    fa_ms = (time.perf_counter() - t0) * 1e3 / iters
+    dev.close()
+    return {"forward_ms": round(fwd_ms, 3), "forward_adjoint_ms": round(fa_ms, 3),
+            "ratio": round(fa_ms / fwd_ms, 3), "loss": loss, "grad_norm": float(np.linalg.norm(g)),
+            "config": "512x512, 16 spp, max_depth 3, GGX + Lambert NEE, no RR, 65 materials x 7 parameters"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +153,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--loop", type=int, default=1, help="traversal loop variant (0 nested, 1 single loop)")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -213,6 +254,10 @@ def main() -> int:
         "fast_division": dev.get_option(N.OPT_FAST_DIVISION),
         "image_mean": float(np.mean(img[..., :3])),
     }
+
+    if not args.no_adjoint:
+        result["fwd_adjoint"] = bench_adjoint(scene, camf)
+        print(f"bench: forward+adjoint {result['fwd_adjoint']['ratio']}x forward", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         prm = make_params(accumulation_count=0, max_depth=args.max_depth, mode=N.MODE_GGX_NEE)

@@ -257,6 +257,11 @@ int pulse_memset_d(pulse_ctx *ctx, void *dst_device, int value, size_t bytes);
 int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *desc);
 int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out);
 
+/* Replaces the material table only (binding 8) -- the per-iteration update
+ * of an optimisation loop over material parameters.  Same count as the
+ * uploaded scene. */
+int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials, uint32_t count);
+
 /* ------------------------------------------------------------------------ */
 /* Hot path                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -288,6 +293,32 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                    /* the exact reciprocal-multiply slab division     */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable form: seed-replay adjoint of an L2 image loss              */
+/* ------------------------------------------------------------------------ */
+/* Gradient layout: 7 floats per material m at grad[7*m]:
+ *   d/d base_color.r, .g, .b, d/d perceptual_roughness, d/d emissive.r, .g, .b
+ * Loss: L = mean over pixels and rgb of (image - target)^2. */
+#define PULSE_GRAD_PER_MATERIAL 7
+
+/* L of `image` against `target` (device, npix*4 floats each); synchronous. */
+int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, uint32_t npix, double *loss);
+
+/* dL/dparameters of the image rendered by pulse_trace with the same camera
+ * and params (accumulation_count 0, params->spp samples, mode
+ * PULSE_MODE_GGX_NEE, max_depth <= 4, no Russian roulette:
+ * rr_start_depth >= max_depth).  Replays every path from its seed; `grad`
+ * (device, n_materials*7 floats) is overwritten.  `replay_rgba` (device,
+ * optional) receives the replayed image, which equals the forward image bit
+ * for bit.  Requires metallic == 0 for every material. */
+int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params, const float *image,
+                  const float *target, float *grad, float *replay_rgba);
+
+/* Forward + loss + adjoint in one call: renders into image_out (device,
+ * width*height*4), writes the loss (host) and the gradient (device). */
+int pulse_forward_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params,
+                          const float *target, float *image_out, float *grad, double *loss);
 
 /* Counters of the last pulse_trace call (synchronizes the stream). */
 int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out);

@@ -260,6 +260,68 @@ int oracle_render_pair(oracle_scene *sc, const pulse_camera *cam, const pulse_pa
     return 0;
 }
 
+// Gradient of L = mean over pixels and rgb of (I - target)^2 by central
+// differences in f64 along the f32 decisions (decision replay, no probes):
+// with every decision fixed the f64 image is a smooth function of the
+// material parameters.  params: n_materials*7 f64 values {base.rgb,
+// roughness, emissive.rgb}; ids: parameter indices (7*m + j); h: step.
+// Returns 0; grad_out[i] = dL/dparam[ids[i]].  loss_out (optional) = L at
+// the unperturbed parameters.
+int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                   const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
+                   double *grad_out, double *loss_out, int threads) {
+    OwnedScene *os = reinterpret_cast<OwnedScene *>(sc);
+    Scene &s = os->s;
+    const uint32_t W = cam->width, H = cam->height, npix = W * H, spp = std::max(prm->spp, 1u);
+    // record the f32 decisions of every path once
+    std::vector<std::vector<uint64_t>> logs(static_cast<size_t>(npix) * spp);
+    std::vector<Counters> pc(std::max(1, threads));
+    parallel_for(H, threads, [&](uint32_t y, int tid) {
+        for (uint32_t x = 0; x < W; ++x)
+            for (uint32_t k = 0; k < spp; ++k) {
+                DecisionTape rec;
+                rec.mode = DecisionTape::RECORD;
+                rec.log = &logs[(static_cast<size_t>(y) * W + x) * spp + k];
+                trace_path<float>(s, *cam, *prm, x, y, prm->accumulation_count + k, pc[tid], rec);
+            }
+    });
+    std::vector<double> p(params, params + 7 * static_cast<size_t>(n_materials));
+    auto loss = [&](const std::vector<double> &pp) {
+        s.mat_override = pp.data();
+        std::vector<double> part(H, 0.0);
+        parallel_for(H, threads, [&](uint32_t y, int tid) {
+            double acc = 0.0;
+            for (uint32_t x = 0; x < W; ++x) {
+                V3<double> I{0, 0, 0};
+                for (uint32_t k = 0; k < spp; ++k) {
+                    DecisionTape rep;
+                    rep.mode = DecisionTape::REPLAY;
+                    rep.probe = false;
+                    rep.log = &logs[(static_cast<size_t>(y) * W + x) * spp + k];
+                    V3<double> c = trace_path<double>(s, *cam, *prm, x, y, prm->accumulation_count + k, pc[tid], rep);
+                    I = accumulate(I, c, k);
+                }
+                const float *t = target + 4 * (static_cast<size_t>(y) * W + x);
+                double dx = I.x - t[0], dy = I.y - t[1], dz = I.z - t[2];
+                acc += dx * dx + dy * dy + dz * dz;
+            }
+            part[y] = acc;
+        });
+        s.mat_override = nullptr;
+        double tot = 0.0;
+        for (double v : part) tot += v;
+        return tot / (3.0 * npix);
+    };
+    if (loss_out) *loss_out = loss(p);
+    for (uint32_t i = 0; i < n_ids; ++i) {
+        std::vector<double> pp = p, pm = p;
+        pp[ids[i]] += h;
+        pm[ids[i]] -= h;
+        grad_out[i] = (loss(pp) - loss(pm)) / (2.0 * h);
+    }
+    return 0;
+}
+
 // ----- unit hooks for the sampler / RNG tests -----
 uint32_t oracle_rand_u(uint32_t *state) { return rand_u(*state); }
 float oracle_rand_f(uint32_t *state) { return rand_f(*state); }

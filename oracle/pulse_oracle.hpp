@@ -170,6 +170,9 @@ struct Scene {
     // occluding instance.  Results must equal the reference traversal; the
     // work counters then describe what the kernels execute.
     bool cull_world = false;
+    // f64 only: per-material parameters {base.rgb, roughness, emissive.rgb}
+    // replacing the buffer's values (finite differences of the adjoint).
+    const double *mat_override = nullptr;
 };
 
 // Per-ray traversal counters (the algorithmic-bytes model counts these).
@@ -527,6 +530,10 @@ struct DecisionTape {
     std::vector<uint64_t> *log = nullptr;
     size_t pos = 0;
     uint64_t flips = 0;  // replayed decisions that differ from the computed one
+    bool probe = true;   // replay: still run the traversals to count flips
+    // Replay without probing: the next recorded decision.
+    bool skip_probe() const { return mode == REPLAY && !probe; }
+    uint64_t next() { return pos < log->size() ? (*log)[pos++] : 0; }
     uint64_t take(uint64_t computed) {
         if (mode == RECORD) {
             log->push_back(computed);
@@ -556,9 +563,14 @@ void trace_closest(const Scene &s, Ray<R> &ray, Counters &c, DecisionTape &tape)
         tape.take(ray.t >= sentinel<R>() ? kMissCode : hit_code(ray.instance_index, ray.triangle_index));
         return;
     }
-    Ray<R> probe = ray;
-    traverse_tlas(s, probe, c);
-    uint64_t code = tape.take(probe.t >= sentinel<R>() ? kMissCode : hit_code(probe.instance_index, probe.triangle_index));
+    uint64_t code;
+    if (tape.skip_probe()) {
+        code = tape.next();
+    } else {
+        Ray<R> probe = ray;
+        traverse_tlas(s, probe, c);
+        code = tape.take(probe.t >= sentinel<R>() ? kMissCode : hit_code(probe.instance_index, probe.triangle_index));
+    }
     if (code == kMissCode) {
         ray.t = sentinel<R>();
         return;
@@ -669,6 +681,19 @@ inline MaterialR<R> load_material(const pulse_material &m) {
     return {load3<R>(m.base_color), load3<R>(m.emissive), static_cast<R>(m.perceptual_roughness),
             static_cast<R>(m.reflectance), static_cast<R>(m.metallic)};
 }
+// Material / emission of material `idx`, with the f64 parameter override.
+template <class R>
+inline MaterialR<R> scene_material(const Scene &s, uint32_t idx) {
+    MaterialR<R> m = load_material<R>(s.materials[idx]);
+    if (std::is_same<R, double>::value && s.mat_override) {
+        const double *p = s.mat_override + 7 * static_cast<size_t>(idx);
+        m.base = {static_cast<R>(p[0]), static_cast<R>(p[1]), static_cast<R>(p[2])};
+        m.roughness = static_cast<R>(p[3]);
+        m.emissive = {static_cast<R>(p[4]), static_cast<R>(p[5]), static_cast<R>(p[6])};
+    }
+    return m;
+}
+
 template <class R>
 inline V3<R> material_f0(const MaterialR<R> &m) {
     R f = R(0.16f) * m.reflectance * m.reflectance;
@@ -809,6 +834,7 @@ ShadowSetup<R> nee_setup(const Scene &s, V3<R> p0, V3<R> n0, uint32_t &rng, Deci
 template <class R>
 bool nee_occluded(const Scene &s, ShadowSetup<R> &st, Counters &c, DecisionTape &tape) {
     c.shadow_rays++;
+    if (tape.skip_probe()) return tape.next() != 0;
     CatScope scope(c, 2);
     Ray<R> r = st.ray;
     R threshold = length(st.pl - st.ray.origin) - R(0.005f);
@@ -844,7 +870,7 @@ V3<R> sample_direct_light(const Scene &s, V3<R> p0, V3<R> n0, V3<R> base_color, 
     R cos_r = dot(st.ray.dir, n0);
     R cos_e = dot(-st.ray.dir, nl);
     V3<R> brdf = base_color * R(kINV_PI);
-    V3<R> le = load3<R>(s.materials[st.material_index].emissive);
+    V3<R> le = scene_material<R>(s, st.material_index).emissive;
     R dsq = distance_sq(n0, nl);
     V3<R> d = ((((brdf * le) * cos_r) * cos_e) / pdf) / dsq;
     return {M<R>::fmax(d.x, R(0)), M<R>::fmax(d.y, R(0)), M<R>::fmax(d.z, R(0))};
@@ -874,7 +900,7 @@ V3<R> sample_direct_light_ggx(const Scene &s, V3<R> p0, V3<R> n0, const Material
     R G = G_smith(NdotO, NdotI, mat.roughness);
     R denom = R(4) * NdotO * NdotI;
     V3<R> brdf = V3<R>{F.x * D * G, F.y * D * G, F.z * D * G} / denom;
-    V3<R> le = load3<R>(s.materials[st.material_index].emissive);
+    V3<R> le = scene_material<R>(s, st.material_index).emissive;
     R dsq = distance_sq(n0, nl);
     V3<R> d = ((((brdf * le) * cos_r) * cos_e) / pdf) / dsq;
     return {M<R>::fmax(d.x, R(0)), M<R>::fmax(d.y, R(0)), M<R>::fmax(d.z, R(0))};
@@ -948,7 +974,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
         { CatScope scope(c, 0); trace_closest(s, ray, c, tape); }
         if (ray.t >= sentinel<R>()) return color;
         HitShading<R> h0 = shade_hit(s, ray);
-        MaterialR<R> m0 = load_material<R>(s.materials[h0.material_index]);
+        MaterialR<R> m0 = scene_material<R>(s, h0.material_index);
         R a0 = static_cast<R>(rand_f(rng));
         R a1 = static_cast<R>(rand_f(rng));
         V3<R> scatter = sample_cosine_hemisphere(h0.normal, a0, a1);
@@ -965,7 +991,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
             { CatScope scope(c, 1); trace_closest(s, ray, c, tape); }
             if (ray.t >= sentinel<R>()) break;
             HitShading<R> h = shade_hit(s, ray);
-            MaterialR<R> m = load_material<R>(s.materials[h.material_index]);
+            MaterialR<R> m = scene_material<R>(s, h.material_index);
             V3<R> dl{R(0), R(0), R(0)};
             if (have_lights) dl = sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape);
             color = color + throughput * dl;
@@ -985,7 +1011,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
         { CatScope scope(c, depth == 0 ? 0 : 1); trace_closest(s, ray, c, tape); }
         if (ray.t >= sentinel<R>()) break;
         HitShading<R> h = shade_hit(s, ray);
-        MaterialR<R> m = load_material<R>(s.materials[h.material_index]);
+        MaterialR<R> m = scene_material<R>(s, h.material_index);
         V3<R> wo = -ray.dir;
         if (prm.mode == PULSE_MODE_REF_GGX) {
             color = color + throughput * m.emissive;

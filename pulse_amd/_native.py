@@ -33,6 +33,7 @@ NEE_LAMBERT = 0
 NEE_GGX = 1
 FLAG_COLLECT_STATS = 1
 FLAG_STAGE_TIMING = 2
+GRAD_PER_MATERIAL = 7
 OPT_TRAVERSAL_LOOP = 1
 OPT_STACK_ENTRIES = 2
 OPT_FAST_DIVISION = 3
@@ -200,11 +201,15 @@ SIGNATURES = [
     ("pulse_memset_d", _I, [_P, _P, _I, C.c_size_t]),
     ("pulse_scene_upload", _I, [_P, C.POINTER(SceneDesc)]),
     ("pulse_scene_generation", _I, [_P, C.POINTER(C.c_uint64)]),
+    ("pulse_scene_update_materials", _I, [_P, _P, _U32]),
     ("pulse_trace", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P]),
     ("pulse_trace_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_trace_shadow_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_get_stats", _I, [_P, C.POINTER(Stats)]),
     ("pulse_set_option", _I, [_P, _I, C.c_int64]),
+    ("pulse_image_loss", _I, [_P, _P, _P, _U32, C.POINTER(C.c_double)]),
+    ("pulse_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, _P]),
+    ("pulse_forward_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, C.POINTER(C.c_double)]),
     ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),
     ("pulse_build_blas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),
     ("pulse_build_tlas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),

@@ -1,0 +1,261 @@
+// adjoint.h -- replay adjoint of the L2 image loss with respect to per-
+// material albedo (base_color.rgb), GGX roughness and emissive.rgb.
+//
+// Nothing of the forward paths is stored: each path is replayed from its
+// seed (the pixel index and accumulation count, path_tracer.wgsl:60-61) with
+// the same f32 traversal, so it takes exactly the forward decisions.  Along
+// the replay the vertex computations run in Dual<kAdjMaxVertices>
+// arithmetic: the tangent k is the derivative with respect to the roughness
+// used at vertex k, which moves the sampled half vector and therefore every
+// later ray, hit point, normal, NEE term and BSDF weight.  Albedo and emission
+// enter linearly; they are handled by a reverse sweep over the stored vertex
+// weights: with C = sum_k beta_k c_k, beta_{k+1} = beta_k w_k and
+// S_k = c_k + w_k S_{k+1}, dC/dc_j = beta_j and dC/dw_j = beta_j S_{j+1}.
+//
+// The derivative is that of the estimator with every decision held fixed
+// (closest-hit ids, lobe, occlusion, fold of the light sample), i.e. the
+// derivative a finite difference with the same random numbers measures away
+// from those discontinuities.  Russian roulette is not differentiated (its
+// probability depends on the albedo): the adjoint requires it off
+// (rr_start_depth >= max_depth); metallic materials are rejected on upload
+// for the adjoint (their lobe probability would depend on the albedo).
+//
+// Per block the gradient is accumulated in LDS (float atomics), then added to
+// the global gradient with one atomic per entry.
+#pragma once
+
+#include "traverse.h"
+
+namespace pt {
+
+constexpr int kAdjMaxVertices = 4;
+constexpr int kGradPerMaterial = 7;  // base.rgb, roughness, emissive.rgb
+using DA = Dual<kAdjMaxVertices>;
+
+struct AdjArgs {
+    pulse_camera cam;
+    uint32_t max_depth;
+    uint32_t nee_ggx;
+    uint32_t spp;
+    uint32_t npix;
+    const float4 *image;   // forward image (running mean over spp samples)
+    const float4 *target;  // target image
+    float *grad;           // n_materials * 7, accumulated into
+    float4 *replay;        // optional: the replayed running mean (verification)
+    float grad_scale;      // dL/dI = grad_scale * (I - T)
+    uint32_t n_materials;
+    uint32_t use_lds;      // gradient buffer fits in LDS
+};
+
+template <class S>
+__device__ __forceinline__ V3 val3(const V3T<S> &a) { return mk(val(a.x), val(a.y), val(a.z)); }
+
+// The intersection of the traversal (traverse.h / pt_device.h: object-space
+// ray, Moeller-Trumbore, world t through the hit point) recomputed for a
+// known triangle in scalar type S.  Values equal the traversal's bit for bit.
+template <class S>
+__device__ __forceinline__ void intersect_known(const DevScene &sc, const V3T<S> &o, const V3T<S> &d, uint32_t inst,
+                                                uint32_t tri, S &tw, S &u, S &v) {
+    const pulse_mesh_instance &mi = sc.instances[inst];
+    float wm[16], ow[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        wm[i] = mi.world_object[i];
+        ow[i] = mi.object_world[i];
+    }
+    V3T<S> oo = transform_position(wm, o);
+    V3T<S> dd = transform_direction(wm, d);
+    S dlen = length(dd);
+    V3T<S> od = divs(dd, dlen);
+    uint32_t slot = mi.index_offset + tri;
+    float4 A = sc.tri_bvh[3 * slot], B = sc.tri_bvh[3 * slot + 1], C = sc.tri_bvh[3 * slot + 2];
+    V3 p0 = mk(A.x, A.y, A.z), e1 = mk(A.w, B.x, B.y), e2 = mk(B.z, B.w, C.x);
+    V3T<S> hv = cross(od, e2);
+    S a = dot(e1, hv);
+    S f = 1.0f / a;
+    V3T<S> s = sub(oo, p0);
+    u = f * dot(s, hv);
+    V3T<S> q = cross(s, e1);
+    v = f * dot(od, q);
+    S t = f * dot(e2, q);
+    V3T<S> hit_world = transform_position(ow, add(oo, scale(t, od)));
+    tw = length(sub(hit_world, o));
+}
+
+__device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
+    if (v != 0.0f) atomicAdd(buf + idx, v);
+}
+
+template <int BLOCK, int STACK>
+__global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
+    __shared__ uint32_t smem[STACK * BLOCK];
+    extern __shared__ float sgrad[];
+    const uint32_t nent = a.n_materials * kGradPerMaterial;
+    if (a.use_lds) {
+        for (uint32_t i = threadIdx.x; i < nent; i += BLOCK) sgrad[i] = 0.0f;
+        __syncthreads();
+    }
+    float *gbuf = a.use_lds ? sgrad : a.grad;
+    const uint32_t pixel = blockIdx.x * BLOCK + threadIdx.x;
+    if (pixel < a.npix) {
+        LdsStack stk{smem + threadIdx.x, BLOCK, STACK};
+        const uint32_t W = a.cam.width;
+        const uint32_t x = pixel % W, y = pixel / W;
+        float4 im = a.image[pixel], tg = a.target[pixel];
+        // dL/dC of one sample: dL/dI * dI/dC, I the mean of spp samples
+        const float inv_spp = 1.0f / static_cast<float>(a.spp);
+        const float gx = a.grad_scale * (im.x - tg.x) * inv_spp;
+        const float gy = a.grad_scale * (im.y - tg.y) * inv_spp;
+        const float gz = a.grad_scale * (im.z - tg.z) * inv_spp;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        for (uint32_t s = 0; s < a.spp; ++s) {
+            uint32_t rng = pixel_seed(pixel, s);
+            V3 o0, d0;
+            primary_ray(a.cam, x, y, o0, d0);
+            V3T<DA> o = lift<DA>(o0), d = lift<DA>(d0);
+            V3T<DA> T = lift<DA>(mk(1.0f, 1.0f, 1.0f));
+            V3T<DA> C = lift<DA>(mk(0.0f, 0.0f, 0.0f));
+            // stored vertex values for the reverse sweep
+            V3 beta[kAdjMaxVertices], cvert[kAdjMaxVertices], wvert[kAdjMaxVertices];
+            V3 dDdb[kAdjMaxVertices], dDdLe[kAdjMaxVertices];
+            float dwdb[kAdjMaxVertices];
+            uint32_t mat[kAdjMaxVertices], lmat[kAdjMaxVertices];
+            bool vis[kAdjMaxVertices];
+            int nv = 0;
+            for (uint32_t depth = 0; depth < a.max_depth; ++depth) {
+                float t = kTFar, u = 0.0f, v = 0.0f;
+                uint32_t inst = 0, tri = 0;
+                trace_unified<false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
+                if (!(t < kTFar)) break;
+                DA tw, uu, vv;
+                intersect_known(sc, o, d, inst, tri, tw, uu, vv);
+                HitInfoT<DA> h = shade_hit(sc, o, d, tw, inst, tri, uu, vv);
+                const pulse_material &pm = sc.materials[h.material_index];
+                MaterialT<DA> m;
+                m.base = ld3(pm.base_color);
+                m.emissive = ld3(pm.emissive);
+                m.roughness = dvar<kAdjMaxVertices>(pm.perceptual_roughness, static_cast<int>(depth));
+                m.reflectance = pm.reflectance;
+                m.metallic = pm.metallic;
+                V3T<DA> wo = neg(d);
+                V3 ck = mk(0.0f, 0.0f, 0.0f);
+                beta[nv] = val3(T);
+                mat[nv] = h.material_index;
+                if (depth == 0) {
+                    C = add(C, mul(T, m.emissive));
+                    ck = m.emissive;
+                }
+                vis[nv] = false;
+                dDdb[nv] = mk(0.0f, 0.0f, 0.0f);
+                dDdLe[nv] = mk(0.0f, 0.0f, 0.0f);
+                lmat[nv] = 0;
+                if (sc.light_count > 0u) {
+                    NeeSampleT<DA> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng);
+                    float tt = kTFar, su = 0.0f, sv = 0.0f;
+                    uint32_t si = 0, st = 0;
+                    bool occ = trace_unified<true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,
+                                                   val(ns.threshold), stk);
+                    if (!occ) {
+                        C = add(C, mul(T, ns.contrib));
+                        ck = add(ck, val3(ns.contrib));
+                        vis[nv] = true;
+                        dDdb[nv] = ns.d_base;
+                        dDdLe[nv] = ns.d_emissive;
+                        lmat[nv] = ns.light_material;
+                    } else {
+                        C = add(C, mul(T, mk(0.0f, 0.0f, 0.0f)));
+                    }
+                }
+                cvert[nv] = ck;
+                wvert[nv] = mk(0.0f, 0.0f, 0.0f);
+                dwdb[nv] = 0.0f;
+                const bool last = depth + 1u >= a.max_depth;
+                if (!last) {
+                    SampleT<DA> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
+                    T = mul(T, smp.reflectance);
+                    wvert[nv] = val3(smp.reflectance);
+                    dwdb[nv] = smp.diffuse ? smp.dbase : 0.0f;
+                    d = normalize(smp.wi);
+                    o = add(h.position, scale(0.001f, h.normal));
+                }
+                nv += 1;
+            }
+            // roughness: tangents of the replayed colour
+            for (int k = 0; k < nv; ++k) {
+                float dr = gx * C.x.d[k] + gy * C.y.d[k] + gz * C.z.d[k];
+                if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
+            }
+            // albedo and emission: reverse sweep over the vertices
+            float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
+            for (int k = nv - 1; k >= 0; --k) {
+                const V3 b = beta[k];
+                float ax = gx * b.x * (dDdb[k].x + Sx * dwdb[k]);
+                float ay = gy * b.y * (dDdb[k].y + Sy * dwdb[k]);
+                float az = gz * b.z * (dDdb[k].z + Sz * dwdb[k]);
+                if (ax == ax && ay == ay && az == az) {
+                    grad_add(gbuf, mat[k] * kGradPerMaterial + 0, ax);
+                    grad_add(gbuf, mat[k] * kGradPerMaterial + 1, ay);
+                    grad_add(gbuf, mat[k] * kGradPerMaterial + 2, az);
+                }
+                if (vis[k]) {
+                    float ex = gx * b.x * dDdLe[k].x, ey = gy * b.y * dDdLe[k].y, ez = gz * b.z * dDdLe[k].z;
+                    if (ex == ex && ey == ey && ez == ez) {
+                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 4, ex);
+                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 5, ey);
+                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 6, ez);
+                    }
+                }
+                if (k == 0) {
+                    grad_add(gbuf, mat[0] * kGradPerMaterial + 4, gx * b.x);
+                    grad_add(gbuf, mat[0] * kGradPerMaterial + 5, gy * b.y);
+                    grad_add(gbuf, mat[0] * kGradPerMaterial + 6, gz * b.z);
+                }
+                Sx = cvert[k].x + wvert[k].x * Sx;
+                Sy = cvert[k].y + wvert[k].y * Sy;
+                Sz = cvert[k].z + wvert[k].z * Sz;
+            }
+            if (a.replay) {
+                // running mean of path_tracer.wgsl:116-123 over the replayed samples
+                float w = 1.0f / (static_cast<float>(s) + 1.0f);
+                float omw = 1.0f - w;
+                float ox = s > 0 ? acc.x : 0.0f, oy = s > 0 ? acc.y : 0.0f, oz = s > 0 ? acc.z : 0.0f;
+                acc = make_float4(ox * omw + C.x.v * w, oy * omw + C.y.v * w, oz * omw + C.z.v * w, 1.0f);
+            }
+        }
+        if (a.replay) a.replay[pixel] = acc;
+    }
+    if (a.use_lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nent; i += BLOCK)
+            if (sgrad[i] != 0.0f) atomicAdd(a.grad + i, sgrad[i]);
+    }
+}
+
+// Loss L = mean over pixels and rgb of (I - T)^2, accumulated in f64.
+__global__ void __launch_bounds__(256) k_image_loss(const float4 *image, const float4 *target, uint32_t npix,
+                                                    double *out) {
+    __shared__ double part[256];
+    double s = 0.0;
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < npix; p += gridDim.x * 256) {
+        float4 i = image[p], t = target[p];
+        double dx = static_cast<double>(i.x) - t.x, dy = static_cast<double>(i.y) - t.y,
+               dz = static_cast<double>(i.z) - t.z;
+        s += dx * dx + dy * dy + dz * dz;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0] / (3.0 * static_cast<double>(npix)));
+}
+
+// Zero-fill (used instead of a memset so the sequence can be captured in a
+// graph).
+__global__ void k_fill_zero(float *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+}  // namespace pt

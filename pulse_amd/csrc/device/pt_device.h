@@ -15,9 +15,7 @@
 
 #include "pulse_hip.h"
 
-#pragma clang force_cuda_host_device begin
-#include "../pt_math.h"
-#pragma clang force_cuda_host_device end
+#include "dual.h"
 
 namespace pt {
 
@@ -58,42 +56,76 @@ struct DevScene {
 };
 
 // ---------------------------------------------------------------------------
-// f32 vectors with the reference's evaluation order
+// 3-vectors with the reference's evaluation order.  Templated on the scalar:
+// float in the forward kernels, Dual<N> in the adjoint replay (dual.h), whose
+// values follow exactly the same f32 operations.
 // ---------------------------------------------------------------------------
-struct V3 {
-    float x, y, z;
+template <class S>
+struct V3T {
+    S x, y, z;
 };
+using V3 = V3T<float>;
 __device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
-__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ V3 mul(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ V3 scale(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ V3 divs(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
-    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+template <class S>
+__device__ __forceinline__ V3T<S> mk3(S x, S y, S z) { return V3T<S>{x, y, z}; }
+template <class A, class B>
+__device__ __forceinline__ auto add(const V3T<A> &a, const V3T<B> &b) -> V3T<decltype(a.x + b.x)> {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
 }
-__device__ __forceinline__ float length(V3 a) { return __builtin_sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 normalize(V3 a) { return divs(a, length(a)); }
-__device__ __forceinline__ float max3(V3 v) { return fmaxf(fmaxf(v.x, v.y), v.z); }
+template <class A, class B>
+__device__ __forceinline__ auto sub(const V3T<A> &a, const V3T<B> &b) -> V3T<decltype(a.x - b.x)> {
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class S>
+__device__ __forceinline__ V3T<S> neg(const V3T<S> &a) { return {-a.x, -a.y, -a.z}; }
+template <class A, class B>
+__device__ __forceinline__ auto mul(const V3T<A> &a, const V3T<B> &b) -> V3T<decltype(a.x * b.x)> {
+    return {a.x * b.x, a.y * b.y, a.z * b.z};
+}
+template <class A, class B>
+__device__ __forceinline__ auto scale(const A &s, const V3T<B> &a) -> V3T<decltype(s * a.x)> {
+    return {s * a.x, s * a.y, s * a.z};
+}
+template <class A, class B>
+__device__ __forceinline__ auto divs(const V3T<A> &a, const B &s) -> V3T<decltype(a.x / s)> {
+    return {a.x / s, a.y / s, a.z / s};
+}
+template <class A, class B>
+__device__ __forceinline__ auto dot(const V3T<A> &a, const V3T<B> &b) -> decltype(a.x * b.x) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class A, class B>
+__device__ __forceinline__ auto cross(const V3T<A> &a, const V3T<B> &b) -> V3T<decltype(a.x * b.x)> {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class S>
+__device__ __forceinline__ S length(const V3T<S> &a) { return sqrt_(dot(a, a)); }
+template <class S>
+__device__ __forceinline__ V3T<S> normalize(const V3T<S> &a) { return divs(a, length(a)); }
+template <class S>
+__device__ __forceinline__ S max3(const V3T<S> &v) { return fmax_(fmax_(v.x, v.y), v.z); }
 __device__ __forceinline__ V3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+template <class S>
+__device__ __forceinline__ V3T<S> lift(const V3 &a) { return {sconst<S>(a.x), sconst<S>(a.y), sconst<S>(a.z)}; }
 
 // Column-major mat4 * (p, w) with ((c0*x + c1*y) + c2*z) + c3*w per row.
-__device__ __forceinline__ void mat_vec(const float *m, V3 p, float w, float h[4]) {
+template <class S>
+__device__ __forceinline__ void mat_vec(const float *m, const V3T<S> &p, float w, S h[4]) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) h[r] = m[r] * p.x + m[4 + r] * p.y + m[8 + r] * p.z + m[12 + r] * w;
 }
 // transform_position / transform_direction, utilities.wgsl:751-759
-__device__ __forceinline__ V3 transform_position(const float *m, V3 p) {
-    float h[4];
+template <class S>
+__device__ __forceinline__ V3T<S> transform_position(const float *m, const V3T<S> &p) {
+    S h[4];
     mat_vec(m, p, 1.0f, h);
-    return mk(h[0] / h[3], h[1] / h[3], h[2] / h[3]);
+    return {h[0] / h[3], h[1] / h[3], h[2] / h[3]};
 }
-__device__ __forceinline__ V3 transform_direction(const float *m, V3 p) {
-    float h[4];
+template <class S>
+__device__ __forceinline__ V3T<S> transform_direction(const float *m, const V3T<S> &p) {
+    S h[4];
     mat_vec(m, p, 0.0f, h);
-    return mk(h[0], h[1], h[2]);
+    return {h[0], h[1], h[2]};
 }
 
 // ---------------------------------------------------------------------------
@@ -363,31 +395,41 @@ __device__ __forceinline__ bool traverse_tlas(const DevScene &sc, V3 o, V3 d, fl
 
 // ---------------------------------------------------------------------------
 // R8: GGX terms and sampling, utilities.wgsl:468-476, 719-769, 787-872
+// (templated on the scalar S: float, or Dual<N> in the adjoint replay)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ V3 fresnel_schlick(float cos_theta, V3 f0) {
-    float p = pt_pow5f(1.0f - cos_theta);
-    return mk(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+template <class S>
+__device__ __forceinline__ V3T<S> fresnel_schlick(const S &cos_theta, const V3 &f0) {
+    S p = pow5_(1.0f - cos_theta);
+    return {f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
 }
-__device__ __forceinline__ float D_GGX(float NdotM, float roughness) {
-    float a = roughness * roughness;
-    float a2 = a * a;
-    float NdotM2 = NdotM * NdotM;
-    float b = (NdotM2 * (a2 - 1.0f) + 1.0f);
+template <class S>
+__device__ __forceinline__ S D_GGX(const S &NdotM, const S &roughness) {
+    S a = roughness * roughness;
+    S a2 = a * a;
+    S NdotM2 = NdotM * NdotM;
+    S b = (NdotM2 * (a2 - 1.0f) + 1.0f);
     return a2 * kINV_PI / (b * b);
 }
-__device__ __forceinline__ float G1(float NdotO, float roughness) {
-    float a = roughness * roughness;
-    float k = a / 2.0f;
+template <class S>
+__device__ __forceinline__ S G1(const S &NdotO, const S &roughness) {
+    S a = roughness * roughness;
+    S k = a / 2.0f;
     return NdotO / (NdotO * (1.0f - k) + k);
 }
-__device__ __forceinline__ float G_smith(float NdotO, float NdotI, float roughness) {
+template <class S>
+__device__ __forceinline__ S G_smith(const S &NdotO, const S &NdotI, const S &roughness) {
     return G1(NdotI, roughness) * G1(NdotO, roughness);
 }
 
-struct Material {
+// Material; the roughness has the scalar type of the replay (its tangent is
+// the derivative direction of the adjoint), the rest stays f32.
+template <class S>
+struct MaterialT {
     V3 base, emissive;
-    float roughness, reflectance, metallic;
+    S roughness;
+    float reflectance, metallic;
 };
+using Material = MaterialT<float>;
 __device__ __forceinline__ Material load_material(const pulse_material &m) {
     Material r;
     r.base = ld3(m.base_color);
@@ -397,14 +439,16 @@ __device__ __forceinline__ Material load_material(const pulse_material &m) {
     r.metallic = m.metallic;
     return r;
 }
-__device__ __forceinline__ V3 material_f0(const Material &m) {
+template <class S>
+__device__ __forceinline__ V3 material_f0(const MaterialT<S> &m) {
     float f = 0.16f * m.reflectance * m.reflectance;
     float om = 1.0f - m.metallic;
     return mk(f * om + m.base.x * m.metallic, f * om + m.base.y * m.metallic, f * om + m.base.z * m.metallic);
 }
 
 // sample_cosine_hemisphere, utilities.wgsl:468-476
-__device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float e0, float e1) {
+template <class S>
+__device__ __forceinline__ V3T<S> sample_cosine_hemisphere(const V3T<S> &n, float e0, float e1) {
     float cos_theta = 2.0f * e0 - 1.0f;
     float phi = 2.0f * kPI * e1;
     float sin_theta = __builtin_sqrtf(fmaxf(1.0f - cos_theta * cos_theta, 0.0f));
@@ -414,66 +458,76 @@ __device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float e0, float e1)
     return normalize(add(n, dir));
 }
 
-struct Sample {
-    V3 wi, reflectance;
+template <class S>
+struct SampleT {
+    V3T<S> wi, reflectance;
+    bool diffuse;     // lobe taken
+    float dbase;      // d reflectance_c / d base_c (diffuse lobe; 0 for specular)
 };
 
 // importance_sample_ggx_d, utilities.wgsl:816-872
-__device__ __forceinline__ Sample importance_sample_ggx_d(V3 n, V3 wo, const Material &m, uint32_t &rng) {
+template <class S>
+__device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, const V3T<S> &wo,
+                                                              const MaterialT<S> &m, uint32_t &rng) {
     V3 f0 = material_f0(m);
-    float NdotO = dot(n, wo);
-    V3 F0 = fresnel_schlick(NdotO, f0);
-    float F0_max = max3(F0);
+    S NdotO = dot(n, wo);
+    V3T<S> F0 = fresnel_schlick(NdotO, f0);
+    S F0_max = max3(F0);
     float e0 = rand_f(rng);
     float e1 = rand_f(rng);
     float e2 = rand_f(rng);
-    V3 brdf, wi;
-    float pdf_s = 0.0f, pdf_d = 0.0f;
-    if (e0 <= F0_max) {
-        float a = m.roughness * m.roughness;
-        float a2 = a * a;
-        float theta = pt_acosf(__builtin_sqrtf((1.0f - e1) / (e1 * (a2 - 1.0f) + 1.0f)));
+    V3T<S> brdf, wi;
+    S pdf_s = sconst<S>(0.0f), pdf_d = sconst<S>(0.0f);
+    SampleT<S> out;
+    out.diffuse = !(e0 <= val(F0_max));
+    float dbase_num = 0.0f;
+    if (!out.diffuse) {
+        S a = m.roughness * m.roughness;
+        S a2 = a * a;
+        S theta = acos_(sqrt_((1.0f - e1) / (e1 * (a2 - 1.0f) + 1.0f)));
         float phi = e2 * kTWO_PI;
         // orthonormal_from_normal, utilities.wgsl:719-729
-        V3 e_two = n;
-        V3 e_three;
-        if (!(length(sub(n, mk(1.0f, 0.0f, 0.0f))) < 0.001f)) {
+        V3T<S> e_two = n;
+        V3T<S> e_three;
+        if (!(val(length(sub(n, mk(1.0f, 0.0f, 0.0f)))) < 0.001f)) {
             e_three = normalize(cross(e_two, mk(1.0f, 0.0f, 0.0f)));
         } else {
             e_three = normalize(cross(e_two, mk(0.0f, 1.0f, 0.0f)));
         }
-        V3 e_one = normalize(cross(e_two, e_three));
+        V3T<S> e_one = normalize(cross(e_two, e_three));
         // spherical_to_cartesian_in_on, utilities.wgsl:732-739: A + (B + C)
-        float sin_theta = pt_sinf(theta);
-        float sp = pt_sinf(phi), cp = pt_cosf(phi), ct = pt_cosf(theta);
-        V3 A = mk((e_one.x * sin_theta) * sp, (e_one.y * sin_theta) * sp, (e_one.z * sin_theta) * sp);
-        V3 B = mk(e_two.x * ct, e_two.y * ct, e_two.z * ct);
-        V3 C = mk((e_three.x * sin_theta) * cp, (e_three.y * sin_theta) * cp, (e_three.z * sin_theta) * cp);
-        V3 wm = add(A, add(B, C));
+        S sin_theta = sin_(theta);
+        float sp = pt_sinf(phi), cp = pt_cosf(phi);
+        S ct = cos_(theta);
+        V3T<S> A = {(e_one.x * sin_theta) * sp, (e_one.y * sin_theta) * sp, (e_one.z * sin_theta) * sp};
+        V3T<S> B = {e_two.x * ct, e_two.y * ct, e_two.z * ct};
+        V3T<S> C = {(e_three.x * sin_theta) * cp, (e_three.y * sin_theta) * cp, (e_three.z * sin_theta) * cp};
+        V3T<S> wm = add(A, add(B, C));
         // reflect, utilities.wgsl:767-769
-        float k = 2.0f * dot(wo, wm);
+        S k = 2.0f * dot(wo, wm);
         wi = sub(scale(k, wm), wo);
-        float NdotM = dot(n, wm);
-        float NdotI = dot(n, wi);
-        float OdotM = dot(wo, wm);
-        V3 F = fresnel_schlick(OdotM, f0);
-        float D = D_GGX(NdotM, m.roughness);
-        float G = G_smith(NdotO, NdotI, m.roughness);
-        float denom = 4.0f * NdotO * NdotI;
-        brdf = mk(F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom);
+        S NdotM = dot(n, wm);
+        S NdotI = dot(n, wi);
+        S OdotM = dot(wo, wm);
+        V3T<S> F = fresnel_schlick(OdotM, f0);
+        S D = D_GGX(NdotM, m.roughness);
+        S G = G_smith(NdotO, NdotI, m.roughness);
+        S denom = 4.0f * NdotO * NdotI;
+        brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
         pdf_s = D * NdotM / (4.0f * OdotM);
     } else {
         wi = sample_cosine_hemisphere(n, e1, e2);
-        float NdotI = dot(n, wi);
-        brdf = mk(m.base.x * kPI * NdotI, m.base.y * kPI * NdotI, m.base.z * kPI * NdotI);
+        S NdotI = dot(n, wi);
+        brdf = {m.base.x * kPI * NdotI, m.base.y * kPI * NdotI, m.base.z * kPI * NdotI};
         pdf_d = kPI * NdotI;
+        dbase_num = val(kPI * NdotI);
     }
-    float pdf = ((1.0f - F0_max) * pdf_d) + (F0_max * pdf_s);
-    if (pdf < 0.0001f) pdf = 1.0f;
-    Sample s;
-    s.wi = wi;
-    s.reflectance = divs(brdf, pdf);
-    return s;
+    S pdf = ((1.0f - F0_max) * pdf_d) + (F0_max * pdf_s);
+    if (val(pdf) < 0.0001f) pdf = sconst<S>(1.0f);
+    out.wi = wi;
+    out.reflectance = divs(brdf, pdf);
+    out.dbase = dbase_num / val(pdf);
+    return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -498,18 +552,23 @@ __device__ __forceinline__ uint32_t cdf_pick(const float *cdf, float e, uint32_t
     return min(lo, count - 1u);
 }
 
-struct NeeSample {
-    V3 origin, dir;     // shadow ray
-    float threshold;    // occluded iff closest t < threshold
-    V3 contrib;         // direct light if unoccluded (before * throughput)
+template <class S>
+struct NeeSampleT {
+    V3T<S> origin, dir;  // shadow ray
+    S threshold;         // occluded iff closest t < threshold
+    V3T<S> contrib;      // direct light if unoccluded (before * throughput)
+    V3 d_base;           // d contrib_c / d base_c (Lambert evaluator)
+    V3 d_emissive;       // d contrib_c / d emissive_c of the light's material
+    uint32_t light_material;
 };
 
 // sample_direct_light / sample_direct_light_ggx (utilities.wgsl:500-558,
 // 562-655) up to the occlusion test, plus the contribution they return when
 // the light is visible.  Consumes 4 random numbers.
-__device__ __forceinline__ NeeSample nee_sample(const DevScene &sc, V3 p0, V3 n0, const Material &m, V3 wo, bool ggx,
-                                                uint32_t &rng) {
-    NeeSample ns;
+template <class S>
+__device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3T<S> &p0, const V3T<S> &n0,
+                                                    const MaterialT<S> &m, const V3T<S> &wo, bool ggx, uint32_t &rng) {
+    NeeSampleT<S> ns;
     uint32_t light_index = rand_u(rng) % sc.light_count;
     pulse_light_data_index ldi = sc.lights[light_index];
     const pulse_mesh_instance &mi = sc.instances[ldi.mesh_instance_index];
@@ -531,45 +590,56 @@ __device__ __forceinline__ NeeSample nee_sample(const DevScene &sc, V3 p0, V3 n0
 #pragma unroll
     for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
     V3 pl = transform_position(ow, pl_obj);
-    V3 to_light = sub(pl, p0);
+    V3T<S> to_light = sub(pl, p0);
     ns.origin = add(p0, scale(0.001f, n0));
     ns.dir = normalize(to_light);
     ns.threshold = length(sub(pl, ns.origin)) - 0.005f;
 
     // light normal from the object-space triangle (utilities.wgsl:533-538)
     V3 nl = normalize(cross(sa, sb));
-    if (dot(nl, neg(ns.dir)) < 0.0f) nl = neg(nl);
+    if (val(dot(nl, neg(ns.dir))) < 0.0f) nl = neg(nl);
     float light_pdf = 1.0f / static_cast<float>(sc.light_count);
     float triangle_pdf = 1.0f / sc.light_areas[light_index];
     float pdf = light_pdf * triangle_pdf;
-    float cos_r = dot(ns.dir, n0);
-    float cos_e = dot(neg(ns.dir), nl);
-    V3 brdf;
+    S cos_r = dot(ns.dir, n0);
+    S cos_e = dot(neg(ns.dir), nl);
+    V3T<S> brdf;
     if (!ggx) {
-        brdf = mk(m.base.x * kINV_PI, m.base.y * kINV_PI, m.base.z * kINV_PI);
+        brdf = lift<S>(mk(m.base.x * kINV_PI, m.base.y * kINV_PI, m.base.z * kINV_PI));
     } else {
         V3 f0 = material_f0(m);
-        float NdotO = dot(n0, wo);
-        V3 wi = to_light;  // unnormalized, as in utilities.wgsl:639
-        V3 wm = normalize(add(scale(0.5f, wi), scale(0.5f, wo)));
-        float NdotM = dot(n0, wm);
-        float NdotI = dot(n0, wi);
-        float OdotM = dot(wo, wm);
-        V3 F = fresnel_schlick(OdotM, f0);
-        float D = D_GGX(NdotM, m.roughness);
-        float G = G_smith(NdotO, NdotI, m.roughness);
-        float denom = 4.0f * NdotO * NdotI;
-        brdf = mk(F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom);
+        S NdotO = dot(n0, wo);
+        V3T<S> wi = to_light;  // unnormalized, as in utilities.wgsl:639
+        V3T<S> wm = normalize(add(scale(0.5f, wi), scale(0.5f, wo)));
+        S NdotM = dot(n0, wm);
+        S NdotI = dot(n0, wi);
+        S OdotM = dot(wo, wm);
+        V3T<S> F = fresnel_schlick(OdotM, f0);
+        S D = D_GGX(NdotM, m.roughness);
+        S G = G_smith(NdotO, NdotI, m.roughness);
+        S denom = 4.0f * NdotO * NdotI;
+        brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
     }
+    ns.light_material = mi.material_index;
     V3 le = ld3(sc.materials[mi.material_index].emissive);
-    float dsq = dot(sub(n0, nl), sub(n0, nl));
-    V3 dl;
+    V3T<S> dn = sub(n0, nl);
+    S dsq = dot(dn, dn);
+    V3T<S> dl;
     dl.x = brdf.x * le.x * cos_r * cos_e / pdf / dsq;
     dl.y = brdf.y * le.y * cos_r * cos_e / pdf / dsq;
     dl.z = brdf.z * le.z * cos_r * cos_e / pdf / dsq;
-    ns.contrib = mk(fmaxf(dl.x, 0.0f), fmaxf(dl.y, 0.0f), fmaxf(dl.z, 0.0f));
+    ns.contrib = {fmax_(dl.x, 0.0f), fmax_(dl.y, 0.0f), fmax_(dl.z, 0.0f)};
+    // partial derivatives of the clamped contribution (values)
+    float g = val(cos_r) * val(cos_e) / pdf / val(dsq);
+    ns.d_emissive = mk(val(dl.x) > 0.0f ? val(brdf.x) * g : 0.0f, val(dl.y) > 0.0f ? val(brdf.y) * g : 0.0f,
+                       val(dl.z) > 0.0f ? val(brdf.z) * g : 0.0f);
+    ns.d_base = ggx ? mk(0.0f, 0.0f, 0.0f)
+                    : mk(val(dl.x) > 0.0f ? kINV_PI * le.x * g : 0.0f, val(dl.y) > 0.0f ? kINV_PI * le.y * g : 0.0f,
+                         val(dl.z) > 0.0f ? kINV_PI * le.z * g : 0.0f);
     return ns;
 }
+using NeeSample = NeeSampleT<float>;
+using Sample = SampleT<float>;
 
 // ---------------------------------------------------------------------------
 // R2 / R7: primary ray and hit shading
@@ -585,22 +655,25 @@ __device__ __forceinline__ void primary_ray(const pulse_camera &cam, uint32_t x,
     d = normalize(sub(mk(h[0] / h[3], h[1] / h[3], h[2] / h[3]), o));
 }
 
-struct HitInfo {
-    V3 normal, position;
+template <class S>
+struct HitInfoT {
+    V3T<S> normal, position;
     uint32_t material_index;
 };
+using HitInfo = HitInfoT<float>;
 // path_tracer.wgsl:89-98
-__device__ __forceinline__ HitInfo shade_hit(const DevScene &sc, V3 o, V3 d, float t, uint32_t inst, uint32_t tri,
-                                             float u, float v) {
+template <class S>
+__device__ __forceinline__ HitInfoT<S> shade_hit(const DevScene &sc, const V3T<S> &o, const V3T<S> &d, const S &t,
+                                                 uint32_t inst, uint32_t tri, const S &u, const S &v) {
     const pulse_mesh_instance &mi = sc.instances[inst];
     uint32_t slot = mi.index_offset + tri;
     float4 a = sc.nrm_bvh[3 * slot], b = sc.nrm_bvh[3 * slot + 1], c = sc.nrm_bvh[3 * slot + 2];
-    float w = 1.0f - (u + v);
-    V3 nrm = add(add(scale(w, mk(a.x, a.y, a.z)), scale(u, mk(b.x, b.y, b.z))), scale(v, mk(c.x, c.y, c.z)));
+    S w = 1.0f - (u + v);
+    V3T<S> nrm = add(add(scale(w, mk(a.x, a.y, a.z)), scale(u, mk(b.x, b.y, b.z))), scale(v, mk(c.x, c.y, c.z)));
     float ow[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
-    HitInfo h;
+    HitInfoT<S> h;
     h.normal = normalize(transform_direction(ow, nrm));
     h.position = add(o, scale(t, d));
     h.material_index = mi.material_index;

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <vector>
 
+#include "device/adjoint.h"
 #include "device/wavefront.h"
 #include "host/common.h"
 
@@ -73,6 +74,9 @@ struct pulse_ctx {
     int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
     uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
     bool fast_div = false;       // node coordinates allow the exact fast division
+    bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
+    uint32_t n_materials = 0;
+    DevBuf loss_acc;             // f64 accumulator of pulse_image_loss
     uint32_t tlas_depth = 0, blas_depth = 0;
     pulse_stats stats{};
     uint32_t last_traces = 0;
@@ -254,6 +258,15 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
 
 }  // namespace
 
+namespace {
+constexpr int kAdjBlock = 128;
+template <int STACK>
+void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
+    uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
+    hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream, dev_scene(ctx), a);
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -290,6 +303,7 @@ int pulse_destroy(pulse_ctx *ctx) {
     for (auto &b : ctx->scene) b.release();
     for (auto &b : ctx->wf) b.release();
     ctx->errors.release();
+    ctx->loss_acc.release();
     if (ctx->events_created)
         for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -478,6 +492,10 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
     ctx->uniform = d->uniform;
     ctx->stack_entries = stack;
     ctx->stack_need = need;
+    ctx->n_materials = static_cast<uint32_t>(n_mat);
+    ctx->any_metallic = false;
+    for (size_t k = 0; k < n_mat; ++k)
+        if (static_cast<const pulse_material *>(d->materials.data)[k].metallic != 0.0f) ctx->any_metallic = true;
     ctx->fast_div = fast_div;
     ctx->tlas_depth = tlas_depth;
     ctx->blas_depth = blas_depth;
@@ -489,6 +507,21 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
 int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out) {
     if (!ctx || !out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_generation: null argument");
     *out = ctx->generation;
+    return PULSE_OK;
+}
+
+int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials, uint32_t count) {
+    if (!ctx || (count && !materials)) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_update_materials: bad argument");
+    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_update_materials: no scene uploaded");
+    if (count != ctx->n_materials)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_materials: %u materials, scene has %u", count, ctx->n_materials);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (count) HIP_TRY(hipMemcpy(ctx->scene[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
+    ctx->any_metallic = false;
+    for (uint32_t k = 0; k < count; ++k)
+        if (materials[k].metallic != 0.0f) ctx->any_metallic = true;
     return PULSE_OK;
 }
 
@@ -614,6 +647,77 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
     return PULSE_OK;
+}
+
+int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, uint32_t npix, double *loss) {
+    if (!ctx || !image || !target || !loss || npix == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_image_loss: bad argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ctx->loss_acc.ensure(sizeof(double))) != PULSE_OK) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->loss_acc.ptr, 0, sizeof(double), ctx->stream));
+    uint32_t grid = std::min<uint32_t>((npix + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_image_loss, dim3(grid), dim3(256), 0, ctx->stream, reinterpret_cast<const float4 *>(image),
+                       reinterpret_cast<const float4 *>(target), npix, ctx->loss_acc.as<double>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(loss, ctx->loss_acc.ptr, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PULSE_OK;
+}
+
+int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *image,
+                  const float *target, float *grad, float *replay_rgba) {
+    if (!ctx || !cam || !prm || !image || !target || !grad) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: null argument");
+    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
+    if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs PULSE_MODE_GGX_NEE");
+    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVertices);
+    if (prm->rr_start_depth < prm->max_depth)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: Russian roulette must be off (rr_start_depth >= max_depth)");
+    if (prm->accumulation_count != 0 || prm->spp == 0)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs accumulation_count 0 and spp >= 1");
+    if (ctx->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: materials with metallic != 0");
+    if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: BVH too deep for the adjoint stack");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const uint32_t npix = cam->width * cam->height;
+    const uint32_t nent = ctx->n_materials * kGradPerMaterial;
+    if (nent) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
+    }
+    AdjArgs a;
+    a.cam = *cam;
+    a.max_depth = prm->max_depth;
+    a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
+    a.spp = prm->spp;
+    a.npix = npix;
+    a.image = reinterpret_cast<const float4 *>(image);
+    a.target = reinterpret_cast<const float4 *>(target);
+    a.grad = grad;
+    a.replay = reinterpret_cast<float4 *>(replay_rgba);
+    a.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
+    a.n_materials = ctx->n_materials;
+    const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
+    a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
+    const size_t lds = a.use_lds ? lds_bytes : 0;
+    switch (ctx->stack_entries) {
+        case 24: launch_adjoint<24>(ctx, a, lds); break;
+        case 32: launch_adjoint<32>(ctx, a, lds); break;
+        case 40: launch_adjoint<40>(ctx, a, lds); break;
+        case 48: launch_adjoint<48>(ctx, a, lds); break;
+        default: launch_adjoint<64>(ctx, a, lds); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return PULSE_OK;
+}
+
+int pulse_forward_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                          float *image_out, float *grad, double *loss) {
+    int rc = pulse_trace(ctx, cam, prm, image_out);
+    if (rc) return rc;
+    if (loss) {
+        if ((rc = pulse_image_loss(ctx, image_out, target, cam->width * cam->height, loss)) != PULSE_OK) return rc;
+    }
+    return pulse_adjoint(ctx, cam, prm, image_out, target, grad, nullptr);
 }
 
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {

@@ -207,3 +207,74 @@ class PulsePathTracerPlugin:
         p.accumulation_count = cam.accumulation_count
         node.run(camera, p)
         return node
+
+
+# ---------------------------------------------------------------------------
+# Differentiable form
+# ---------------------------------------------------------------------------
+def material_params(materials: np.ndarray) -> np.ndarray:
+    """(n, 7) f64 parameter table {base.rgb, roughness, emissive.rgb}."""
+    p = np.zeros((len(materials), N.GRAD_PER_MATERIAL), np.float64)
+    p[:, 0:3] = materials["base_color"][:, :3]
+    p[:, 3] = materials["perceptual_roughness"]
+    p[:, 4:7] = materials["emissive"][:, :3]
+    return p
+
+
+def apply_material_params(materials: np.ndarray, params: np.ndarray) -> np.ndarray:
+    m = materials.copy()
+    m["base_color"][:, :3] = params[:, 0:3]
+    m["perceptual_roughness"] = params[:, 3]
+    m["emissive"][:, :3] = params[:, 4:7]
+    return m
+
+
+class PulseAdjoint:
+    """Forward/backward pair of the differentiable form: renders an image
+    with pulse_trace and returns dL/d(material parameters) of
+    L = mean((image - target)^2) from the seed-replay adjoint."""
+
+    def __init__(self, device: Device, width: int, height: int, n_materials: int):
+        self.device, self.width, self.height, self.n_materials = device, width, height, n_materials
+        self.image = device.alloc(width * height * 16)
+        self.target = device.alloc(width * height * 16)
+        self.grad = device.alloc(max(n_materials * N.GRAD_PER_MATERIAL * 4, 4))
+        self.replay = None
+
+    def set_target(self, target: np.ndarray) -> None:
+        self.target.copy_from(np.ascontiguousarray(target, np.float32))
+
+    def set_materials(self, materials: np.ndarray) -> None:
+        m = np.ascontiguousarray(materials, N.MATERIAL_DTYPE)
+        N.check(N.lib().pulse_scene_update_materials(self.device.handle, m.ctypes.data, len(m)))
+
+    def forward(self, camera: N.Camera, params: N.Params) -> np.ndarray:
+        N.check(N.lib().pulse_trace(self.device.handle, C.byref(camera), C.byref(params), self.image.ptr))
+        self.device.sync()
+        return self.image.copy_to(np.zeros((self.height, self.width, 4), np.float32))
+
+    def loss(self) -> float:
+        out = C.c_double()
+        N.check(N.lib().pulse_image_loss(self.device.handle, self.image.ptr, self.target.ptr,
+                                         self.width * self.height, C.byref(out)))
+        return out.value
+
+    def backward(self, camera: N.Camera, params: N.Params, want_replay: bool = False):
+        if want_replay and self.replay is None:
+            self.replay = self.device.alloc(self.width * self.height * 16)
+        N.check(N.lib().pulse_adjoint(self.device.handle, C.byref(camera), C.byref(params), self.image.ptr,
+                                      self.target.ptr, self.grad.ptr, self.replay.ptr if want_replay else None))
+        self.device.sync()
+        g = self.grad.copy_to(np.zeros(self.n_materials * N.GRAD_PER_MATERIAL, np.float32))
+        g = g.reshape(self.n_materials, N.GRAD_PER_MATERIAL)
+        if want_replay:
+            return g, self.replay.copy_to(np.zeros((self.height, self.width, 4), np.float32))
+        return g
+
+    def forward_backward(self, camera: N.Camera, params: N.Params):
+        loss = C.c_double()
+        N.check(N.lib().pulse_forward_adjoint(self.device.handle, C.byref(camera), C.byref(params), self.target.ptr,
+                                              self.image.ptr, self.grad.ptr, C.byref(loss)))
+        self.device.sync()
+        g = self.grad.copy_to(np.zeros(self.n_materials * N.GRAD_PER_MATERIAL, np.float32))
+        return loss.value, g.reshape(self.n_materials, N.GRAD_PER_MATERIAL)
