@@ -122,9 +122,11 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
     hidden[:, 0:3] = np.clip(start[:, 0:3] + rng.uniform(-0.3, 0.3, (nm, 3)), 0.02, 0.98)
     hidden[:, 3] = np.clip(start[:, 3] + rng.uniform(-0.3, 0.3, nm), 0.05, 1.0)
     adj.set_materials(apply_material_params(mats, hidden))
-    adj.set_target(adj.forward(cam, make_params(spp=64, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3)))
+    clamp = N.FLAG_HORIZON_CLAMP
+    adj.set_target(adj.forward(cam, make_params(spp=64, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3,
+                                                flags=clamp)))
     adj.set_materials(mats)
-    prm = make_params(spp=16, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3)
+    prm = make_params(spp=16, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3, flags=clamp)
     adj.forward(cam, prm)
     adj.forward_backward(cam, prm)  # warm-up
     t0 = time.perf_counter()
@@ -138,7 +140,8 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
     dev.close()
     return {"forward_ms": round(fwd_ms, 3), "forward_adjoint_ms": round(fa_ms, 3),
             "ratio": round(fa_ms / fwd_ms, 3), "loss": loss, "grad_norm": float(np.linalg.norm(g)),
-            "config": "512x512, 16 spp, max_depth 3, GGX + Lambert NEE, no RR, 65 materials x 7 parameters"}
+            "config": "512x512, 16 spp, max_depth 3, GGX + Lambert NEE, no RR, horizon clamp, "
+                      "65 materials x 7 parameters"}
 
 
 def main() -> int:

@@ -190,6 +190,11 @@ typedef struct pulse_params {
 
 #define PULSE_FLAG_COLLECT_STATS 1u /* count rays per stage into pulse_stats */
 #define PULSE_FLAG_STAGE_TIMING 2u  /* hipEvent timing per wavefront stage   */
+#define PULSE_FLAG_HORIZON_CLAMP 4u /* not in the reference: a path ends when */
+                                    /* n.wo <= 0 or the sampled n.wi <= 0 (the */
+                                    /* reference keeps such samples, Q6, which */
+                                    /* gives its GGX estimator unbounded       */
+                                    /* weights); for the differentiable form   */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */

@@ -707,6 +707,7 @@ inline R max3(V3<R> v) { return M<R>::fmax(M<R>::fmax(v.x, v.y), v.z); }
 template <class R>
 struct SampleResult {
     V3<R> wi, reflectance;
+    R NdotO, NdotI;  // for the optional horizon clamp
 };
 
 // importance_sample_ggx_d, utilities.wgsl:816-872
@@ -722,6 +723,7 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
     R e2 = static_cast<R>(rand_f(rng));
     V3<R> brdf, wi;
     R pdf_s = R(0), pdf_d = R(0);
+    R NdotI_out = R(0);
     if (tape.take_bool(e0 <= F0_max)) {
         R a = mat.roughness * mat.roughness;
         R a2 = a * a;
@@ -739,15 +741,17 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
         R denom = R(4) * NdotO * NdotI;
         brdf = V3<R>{F.x * D * G, F.y * D * G, F.z * D * G} / denom;
         pdf_s = D * NdotM / (R(4) * OdotM);
+        NdotI_out = NdotI;
     } else {
         wi = sample_cosine_hemisphere(n, e1, e2);
         R NdotI = dot(n, wi);
+        NdotI_out = NdotI;
         brdf = V3<R>{mat.base.x * R(kPI) * NdotI, mat.base.y * R(kPI) * NdotI, mat.base.z * R(kPI) * NdotI};
         pdf_d = R(kPI) * NdotI;
     }
     R pdf = ((R(1) - F0_max) * pdf_d) + (F0_max * pdf_s);
     if (tape.take_bool(pdf < R(0.0001f))) pdf = R(1);
-    return {wi, brdf / pdf};
+    return {wi, brdf / pdf, NdotO, NdotI_out};
 }
 
 // ---------------------------------------------------------------------------
@@ -1025,6 +1029,11 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
             }
         }
         SampleResult<R> smp = importance_sample_ggx_d(h.normal, wo, m, rng, tape);
+        // optional horizon clamp (not in the reference): a path whose view or
+        // sampled direction lies below the shading horizon ends here
+        if ((prm.flags & PULSE_FLAG_HORIZON_CLAMP) &&
+            tape.take_bool(!(smp.NdotO > R(0)) || !(smp.NdotI > R(0))))
+            break;
         throughput = throughput * smp.reflectance;
         if (depth >= prm.rr_start_depth && !russian_roulette(throughput, rng, tape)) break;
         ray.dir = normalize(smp.wi);

@@ -45,6 +45,7 @@ struct AdjArgs {
     float grad_scale;      // dL/dI = grad_scale * (I - T)
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
+    uint32_t horizon_clamp;
 };
 
 template <class S>
@@ -172,6 +173,10 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 const bool last = depth + 1u >= a.max_depth;
                 if (!last) {
                     SampleT<DA> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
+                    if (a.horizon_clamp && below_horizon(smp)) {
+                        nv += 1;
+                        break;
+                    }
                     T = mul(T, smp.reflectance);
                     wvert[nv] = val3(smp.reflectance);
                     dwdb[nv] = smp.diffuse ? smp.dbase : 0.0f;

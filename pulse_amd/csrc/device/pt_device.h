@@ -463,7 +463,14 @@ struct SampleT {
     V3T<S> wi, reflectance;
     bool diffuse;     // lobe taken
     float dbase;      // d reflectance_c / d base_c (diffuse lobe; 0 for specular)
+    float NdotO, NdotI;  // values, for the optional horizon clamp
 };
+
+// Optional horizon clamp (PULSE_FLAG_HORIZON_CLAMP, not in the reference):
+// the path ends when the view or the sampled direction is below the shading
+// horizon.
+template <class S>
+__device__ __forceinline__ bool below_horizon(const SampleT<S> &smp) { return !(smp.NdotO > 0.0f) || !(smp.NdotI > 0.0f); }
 
 // importance_sample_ggx_d, utilities.wgsl:816-872
 template <class S>
@@ -481,6 +488,7 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
     SampleT<S> out;
     out.diffuse = !(e0 <= val(F0_max));
     float dbase_num = 0.0f;
+    out.NdotO = val(NdotO);
     if (!out.diffuse) {
         S a = m.roughness * m.roughness;
         S a2 = a * a;
@@ -515,9 +523,11 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         S denom = 4.0f * NdotO * NdotI;
         brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
         pdf_s = D * NdotM / (4.0f * OdotM);
+        out.NdotI = val(NdotI);
     } else {
         wi = sample_cosine_hemisphere(n, e1, e2);
         S NdotI = dot(n, wi);
+        out.NdotI = val(NdotI);
         brdf = {m.base.x * kPI * NdotI, m.base.y * kPI * NdotI, m.base.z * kPI * NdotI};
         pdf_d = kPI * NdotI;
         dbase_num = val(kPI * NdotI);

@@ -59,6 +59,7 @@ struct FrameArgs {
     uint32_t max_traces;      // extension traces per path
     uint32_t rr_start;        // first loop depth with Russian roulette
     uint32_t accumulation_count;
+    uint32_t horizon_clamp;   // PULSE_FLAG_HORIZON_CLAMP
 };
 
 // Appends `valid` items of a wave to a queue: returns this lane's slot.
@@ -209,9 +210,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, WfBuffers wb
                 }
                 if (!last) {
                     Sample smp = importance_sample_ggx_d(h.normal, wo, m, rng);
-                    T = mul(T, smp.reflectance);
-                    bool alive = true;
-                    if (static_cast<uint32_t>(k) >= fa.rr_start) {
+                    bool alive = !(fa.horizon_clamp && below_horizon(smp));
+                    if (alive) T = mul(T, smp.reflectance);
+                    if (alive && static_cast<uint32_t>(k) >= fa.rr_start) {
                         float p = max3(T);
                         if (rand_f(rng) > p) alive = false;
                         else T = scale(1.0f / p, T);

@@ -574,6 +574,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     fa.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
     fa.max_traces = traces;
     fa.rr_start = prm->rr_start_depth;
+    fa.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     const uint32_t cap = static_cast<uint32_t>(npix);
     const uint32_t tiles = ((cam->width + 7) / 8) * ((cam->height + 7) / 8);
     const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
@@ -696,6 +697,7 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     a.replay = reinterpret_cast<float4 *>(replay_rgba);
     a.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
     a.n_materials = ctx->n_materials;
+    a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
     a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
     const size_t lds = a.use_lds ? lds_bytes : 0;

@@ -197,3 +197,13 @@ def test_traversal_variants_bit_exact(dev, field, loop):
         assert same_bits(node.readback(), oi).all()
     finally:
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
+@pytest.mark.parametrize("mode", [N.MODE_REF_GGX, N.MODE_GGX_NEE])
+def test_horizon_clamp_bit_exact(dev, field, mode):
+    scene, camf = field
+    cam = camf(96, 64)
+    prm = make_params(spp=2, max_depth=4, mode=mode, rr_start_depth=2, flags=N.FLAG_HORIZON_CLAMP)
+    g = _render_gpu(dev, scene, cam, prm)
+    o, _, _ = OracleScene(scene).render(cam, prm)
+    assert same_bits(g, o).all()

@@ -11,7 +11,8 @@ from pulse_amd.path_tracer import make_params, material_params
 def _setup(spp=2, depth=3):
     scene, camf = P.cornell()
     cam = camf(24, 24)
-    prm = make_params(spp=spp, max_depth=depth, mode=N.MODE_GGX_NEE, rr_start_depth=depth)
+    prm = make_params(spp=spp, max_depth=depth, mode=N.MODE_GGX_NEE, rr_start_depth=depth,
+                      flags=N.FLAG_HORIZON_CLAMP)
     return scene, cam, prm
 
 
@@ -57,3 +58,19 @@ def test_albedo_gradient_sign_and_zero_for_unseen():
     ids = np.array([7 * m + c for m in range(len(mp)) for c in range(3)], np.uint32)
     g, _ = o.grad_fd(cam, prm, target, mp.reshape(-1), ids, h=1e-6)
     assert (g >= -1e-12).all() and g.max() > 0
+
+
+def test_horizon_clamp_bounds_the_estimator():
+    """Without the clamp the reference's GGX estimator keeps below-horizon
+    samples (Q6) and produces unbounded (and negative) pixel values; with it
+    the image is non-negative and its mean is stable across sample counts."""
+    scene, camf = P.icosphere_field(n_meshes=6, subdiv=3, n_instances=40, n_emissive=5, box=24.0, radius=(1.0, 3.0))
+    o = OracleScene(scene)
+    cam = camf(64, 48)
+    means = []
+    for spp in (2, 16):
+        img, _, _ = o.render(cam, make_params(spp=spp, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3,
+                                              flags=N.FLAG_HORIZON_CLAMP))
+        assert (img[..., :3] >= 0).all() and np.isfinite(img).all()
+        means.append(img[..., :3].mean())
+    assert abs(means[0] - means[1]) < 0.1 * means[1]
