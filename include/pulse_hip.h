@@ -195,6 +195,9 @@ typedef struct pulse_params {
                                     /* reference keeps such samples, Q6, which */
                                     /* gives its GGX estimator unbounded       */
                                     /* weights); for the differentiable form   */
+#define PULSE_FLAG_ADJOINT_REPARAM 8u /* adjoint: differentiate roughness through */
+                                      /* the moving sampled directions instead of */
+                                      /* at fixed directions (the default)        */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */

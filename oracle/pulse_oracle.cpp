@@ -265,11 +265,17 @@ int oracle_render_pair(oracle_scene *sc, const pulse_camera *cam, const pulse_pa
 // with every decision fixed the f64 image is a smooth function of the
 // material parameters.  params: n_materials*7 f64 values {base.rgb,
 // roughness, emissive.rgb}; ids: parameter indices (7*m + j); h: step.
+// frozen_dirs = 0: the sampled directions follow the roughness (the
+// reparameterised derivative); 1: the specular half vectors sampled at the
+// unperturbed parameters (f64) are held fixed (the detached derivative);
+// 2: no replay at all -- f64 renders whose decisions follow the perturbed
+// parameters (the derivative of the estimator itself, visibility changes
+// included; use a larger h).
 // Returns 0; grad_out[i] = dL/dparam[ids[i]].  loss_out (optional) = L at
 // the unperturbed parameters.
-int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
-                   const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
-                   double *grad_out, double *loss_out, int threads) {
+int oracle_grad_fd_ex(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                      const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
+                      double *grad_out, double *loss_out, int threads, int frozen_dirs) {
     OwnedScene *os = reinterpret_cast<OwnedScene *>(sc);
     Scene &s = os->s;
     const uint32_t W = cam->width, H = cam->height, npix = W * H, spp = std::max(prm->spp, 1u);
@@ -286,6 +292,8 @@ int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params
             }
     });
     std::vector<double> p(params, params + 7 * static_cast<size_t>(n_materials));
+    std::vector<std::vector<double>> dirs(frozen_dirs == 1 ? logs.size() : 0);
+    DecisionTape::DirMode dmode = frozen_dirs == 1 ? DecisionTape::RECORD_DIRS : DecisionTape::DIRS_OFF;
     auto loss = [&](const std::vector<double> &pp) {
         s.mat_override = pp.data();
         std::vector<double> part(H, 0.0);
@@ -295,9 +303,13 @@ int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params
                 V3<double> I{0, 0, 0};
                 for (uint32_t k = 0; k < spp; ++k) {
                     DecisionTape rep;
-                    rep.mode = DecisionTape::REPLAY;
+                    rep.mode = frozen_dirs == 2 ? DecisionTape::OFF : DecisionTape::REPLAY;
                     rep.probe = false;
                     rep.log = &logs[(static_cast<size_t>(y) * W + x) * spp + k];
+                    if (frozen_dirs == 1) {
+                        rep.dir_mode = dmode;
+                        rep.dirs = &dirs[(static_cast<size_t>(y) * W + x) * spp + k];
+                    }
                     V3<double> c = trace_path<double>(s, *cam, *prm, x, y, prm->accumulation_count + k, pc[tid], rep);
                     I = accumulate(I, c, k);
                 }
@@ -312,7 +324,9 @@ int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params
         for (double v : part) tot += v;
         return tot / (3.0 * npix);
     };
-    if (loss_out) *loss_out = loss(p);
+    double l0 = loss(p);  // also records the frozen directions
+    if (loss_out) *loss_out = l0;
+    if (frozen_dirs == 1) dmode = DecisionTape::FREEZE_DIRS;
     for (uint32_t i = 0; i < n_ids; ++i) {
         std::vector<double> pp = p, pm = p;
         pp[ids[i]] += h;
@@ -320,6 +334,12 @@ int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params
         grad_out[i] = (loss(pp) - loss(pm)) / (2.0 * h);
     }
     return 0;
+}
+
+int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                   const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
+                   double *grad_out, double *loss_out, int threads) {
+    return oracle_grad_fd_ex(sc, cam, prm, target, params, n_materials, ids, n_ids, h, grad_out, loss_out, threads, 0);
 }
 
 // ----- unit hooks for the sampler / RNG tests -----

@@ -548,6 +548,18 @@ struct DecisionTape {
         return computed;
     }
     bool take_bool(bool computed) { return take(computed ? 1u : 0u) != 0; }
+    // Frozen half vectors (the specular lobe's sampled microfacet normal):
+    // RECORD_DIRS stores them with the sampling density q0 they were drawn
+    // from; FREEZE_DIRS replays them instead of mapping the random numbers
+    // through the current roughness, and reweights the sample by the
+    // likelihood ratio q(r)/q0 (importance reweighting keeps the estimator's
+    // expectation that of the perturbed parameters).  The roughness
+    // derivative of a path so replayed is the detached (score-function)
+    // estimator of the derivative of the expected image.
+    enum DirMode { DIRS_OFF, RECORD_DIRS, FREEZE_DIRS };
+    DirMode dir_mode = DIRS_OFF;
+    std::vector<double> *dirs = nullptr;
+    size_t dir_pos = 0;
 };
 
 constexpr uint64_t kMissCode = ~0ull;
@@ -724,13 +736,20 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
     V3<R> brdf, wi;
     R pdf_s = R(0), pdf_d = R(0);
     R NdotI_out = R(0);
-    if (tape.take_bool(e0 <= F0_max)) {
+    V3<R> wm_out{R(0), R(0), R(0)};
+    const bool specular = tape.take_bool(e0 <= F0_max);
+    if (specular) {
         R a = mat.roughness * mat.roughness;
         R a2 = a * a;
         R theta = M<R>::acos(M<R>::sqrt((R(1) - e1) / (e1 * (a2 - R(1)) + R(1))));
         R phi = e2 * R(kTWO_PI);
         ON<R> on = orthonormal_from_normal(n, tape);
         V3<R> wm = spherical_to_cartesian_in_on(theta, phi, on);
+        if (tape.dir_mode == DecisionTape::FREEZE_DIRS && tape.dir_pos + 4 <= tape.dirs->size()) {
+            const double *w = tape.dirs->data() + tape.dir_pos;
+            wm = V3<R>{static_cast<R>(w[0]), static_cast<R>(w[1]), static_cast<R>(w[2])};
+        }
+        wm_out = wm;
         wi = reflect(wo, wm);
         R NdotM = dot(n, wm);
         R NdotI = dot(n, wi);
@@ -750,8 +769,19 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
         pdf_d = R(kPI) * NdotI;
     }
     R pdf = ((R(1) - F0_max) * pdf_d) + (F0_max * pdf_s);
+    R ratio = R(1);  // likelihood ratio of a frozen specular sample
+    if (specular) {
+        if (tape.dir_mode == DecisionTape::RECORD_DIRS) {
+            // wm was just sampled; pdf is the density it was drawn from
+            tape.dirs->insert(tape.dirs->end(), {static_cast<double>(wm_out.x), static_cast<double>(wm_out.y),
+                                                 static_cast<double>(wm_out.z), static_cast<double>(pdf)});
+        } else if (tape.dir_mode == DecisionTape::FREEZE_DIRS && tape.dir_pos + 4 <= tape.dirs->size()) {
+            ratio = pdf / static_cast<R>((*tape.dirs)[tape.dir_pos + 3]);
+            tape.dir_pos += 4;
+        }
+    }
     if (tape.take_bool(pdf < R(0.0001f))) pdf = R(1);
-    return {wi, brdf / pdf, NdotO, NdotI_out};
+    return {wi, brdf / pdf * ratio, NdotO, NdotI_out};
 }
 
 // ---------------------------------------------------------------------------

@@ -24,6 +24,8 @@
 // the global gradient with one atomic per entry.
 #pragma once
 
+#include <type_traits>
+
 #include "traverse.h"
 
 namespace pt {
@@ -46,6 +48,7 @@ struct AdjArgs {
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
     uint32_t horizon_clamp;
+    uint32_t detached;     // roughness derivative at fixed sampled directions
 };
 
 template <class S>
@@ -87,8 +90,15 @@ __device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
     if (v != 0.0f) atomicAdd(buf + idx, v);
 }
 
-template <int BLOCK, int STACK>
+// DETACHED = true: the roughness derivative holds every sampled direction
+// fixed and differentiates the sampling weight brdf(wi; r) / pdf with the pdf
+// detached, i.e. w * d ln(D G)/dr at each specular vertex (and at GGX-evaluated
+// NEE samples).  Unbiased for the derivative of the expected image including
+// the effect of visibility changes, which the reparameterised form (DETACHED =
+// false, tangents through the moving directions) does not capture.
+template <int BLOCK, int STACK, bool DETACHED>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
+    using S = typename std::conditional<DETACHED, float, DA>::type;
     __shared__ uint32_t smem[STACK * BLOCK];
     extern __shared__ float sgrad[];
     const uint32_t nent = a.n_materials * kGradPerMaterial;
@@ -113,13 +123,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             uint32_t rng = pixel_seed(pixel, s);
             V3 o0, d0;
             primary_ray(a.cam, x, y, o0, d0);
-            V3T<DA> o = lift<DA>(o0), d = lift<DA>(d0);
-            V3T<DA> T = lift<DA>(mk(1.0f, 1.0f, 1.0f));
-            V3T<DA> C = lift<DA>(mk(0.0f, 0.0f, 0.0f));
+            V3T<S> o = lift<S>(o0), d = lift<S>(d0);
+            V3T<S> T = lift<S>(mk(1.0f, 1.0f, 1.0f));
+            V3T<S> C = lift<S>(mk(0.0f, 0.0f, 0.0f));
             // stored vertex values for the reverse sweep
             V3 beta[kAdjMaxVertices], cvert[kAdjMaxVertices], wvert[kAdjMaxVertices];
-            V3 dDdb[kAdjMaxVertices], dDdLe[kAdjMaxVertices];
-            float dwdb[kAdjMaxVertices];
+            V3 dDdb[kAdjMaxVertices], dDdLe[kAdjMaxVertices], dDdr[kAdjMaxVertices];
+            float dwdb[kAdjMaxVertices], dlogw[kAdjMaxVertices];
             uint32_t mat[kAdjMaxVertices], lmat[kAdjMaxVertices];
             bool vis[kAdjMaxVertices];
             int nv = 0;
@@ -128,17 +138,27 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 uint32_t inst = 0, tri = 0;
                 trace_unified<false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
                 if (!(t < kTFar)) break;
-                DA tw, uu, vv;
-                intersect_known(sc, o, d, inst, tri, tw, uu, vv);
-                HitInfoT<DA> h = shade_hit(sc, o, d, tw, inst, tri, uu, vv);
+                S tw, uu, vv;
+                if constexpr (DETACHED) {
+                    tw = t;
+                    uu = u;
+                    vv = v;
+                } else {
+                    intersect_known(sc, o, d, inst, tri, tw, uu, vv);
+                }
+                HitInfoT<S> h = shade_hit(sc, o, d, tw, inst, tri, uu, vv);
                 const pulse_material &pm = sc.materials[h.material_index];
-                MaterialT<DA> m;
+                MaterialT<S> m;
                 m.base = ld3(pm.base_color);
                 m.emissive = ld3(pm.emissive);
-                m.roughness = dvar<kAdjMaxVertices>(pm.perceptual_roughness, static_cast<int>(depth));
+                if constexpr (DETACHED) {
+                    m.roughness = pm.perceptual_roughness;
+                } else {
+                    m.roughness = dvar<kAdjMaxVertices>(pm.perceptual_roughness, static_cast<int>(depth));
+                }
                 m.reflectance = pm.reflectance;
                 m.metallic = pm.metallic;
-                V3T<DA> wo = neg(d);
+                V3T<S> wo = neg(d);
                 V3 ck = mk(0.0f, 0.0f, 0.0f);
                 beta[nv] = val3(T);
                 mat[nv] = h.material_index;
@@ -149,9 +169,10 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 vis[nv] = false;
                 dDdb[nv] = mk(0.0f, 0.0f, 0.0f);
                 dDdLe[nv] = mk(0.0f, 0.0f, 0.0f);
+                dDdr[nv] = mk(0.0f, 0.0f, 0.0f);
                 lmat[nv] = 0;
                 if (sc.light_count > 0u) {
-                    NeeSampleT<DA> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng);
+                    NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
                     bool occ = trace_unified<true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,
@@ -162,6 +183,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                         vis[nv] = true;
                         dDdb[nv] = ns.d_base;
                         dDdLe[nv] = ns.d_emissive;
+                        dDdr[nv] = ns.d_rough;
                         lmat[nv] = ns.light_material;
                     } else {
                         C = add(C, mul(T, mk(0.0f, 0.0f, 0.0f)));
@@ -170,9 +192,10 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 cvert[nv] = ck;
                 wvert[nv] = mk(0.0f, 0.0f, 0.0f);
                 dwdb[nv] = 0.0f;
+                dlogw[nv] = 0.0f;
                 const bool last = depth + 1u >= a.max_depth;
                 if (!last) {
-                    SampleT<DA> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
+                    SampleT<S> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
                     if (a.horizon_clamp && below_horizon(smp)) {
                         nv += 1;
                         break;
@@ -180,15 +203,18 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     T = mul(T, smp.reflectance);
                     wvert[nv] = val3(smp.reflectance);
                     dwdb[nv] = smp.diffuse ? smp.dbase : 0.0f;
+                    dlogw[nv] = smp.dlogf_dr;
                     d = normalize(smp.wi);
                     o = add(h.position, scale(0.001f, h.normal));
                 }
                 nv += 1;
             }
-            // roughness: tangents of the replayed colour
-            for (int k = 0; k < nv; ++k) {
-                float dr = gx * C.x.d[k] + gy * C.y.d[k] + gz * C.z.d[k];
-                if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
+            if constexpr (!DETACHED) {
+                // roughness: tangents of the replayed colour
+                for (int k = 0; k < nv; ++k) {
+                    float dr = gx * C.x.d[k] + gy * C.y.d[k] + gz * C.z.d[k];
+                    if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
+                }
             }
             // albedo and emission: reverse sweep over the vertices
             float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
@@ -210,6 +236,14 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                         grad_add(gbuf, lmat[k] * kGradPerMaterial + 6, ez);
                     }
                 }
+                if constexpr (DETACHED) {
+                    // roughness at fixed directions: NEE (GGX evaluator) and
+                    // the sampling weight w_k (d w = w d ln f)
+                    float dr = gx * b.x * (dDdr[k].x + Sx * wvert[k].x * dlogw[k]) +
+                               gy * b.y * (dDdr[k].y + Sy * wvert[k].y * dlogw[k]) +
+                               gz * b.z * (dDdr[k].z + Sz * wvert[k].z * dlogw[k]);
+                    if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
+                }
                 if (k == 0) {
                     grad_add(gbuf, mat[0] * kGradPerMaterial + 4, gx * b.x);
                     grad_add(gbuf, mat[0] * kGradPerMaterial + 5, gy * b.y);
@@ -224,7 +258,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 float w = 1.0f / (static_cast<float>(s) + 1.0f);
                 float omw = 1.0f - w;
                 float ox = s > 0 ? acc.x : 0.0f, oy = s > 0 ? acc.y : 0.0f, oz = s > 0 ? acc.z : 0.0f;
-                acc = make_float4(ox * omw + C.x.v * w, oy * omw + C.y.v * w, oz * omw + C.z.v * w, 1.0f);
+                acc = make_float4(ox * omw + val(C.x) * w, oy * omw + val(C.y) * w, oz * omw + val(C.z) * w, 1.0f);
             }
         }
         if (a.replay) a.replay[pixel] = acc;

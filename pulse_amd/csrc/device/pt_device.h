@@ -464,7 +464,29 @@ struct SampleT {
     bool diffuse;     // lobe taken
     float dbase;      // d reflectance_c / d base_c (diffuse lobe; 0 for specular)
     float NdotO, NdotI;  // values, for the optional horizon clamp
+    // Detached roughness derivative of the sampling weight w = brdf / pdf
+    // with the sample held fixed and the density it was drawn from, q,
+    // differentiated too (score function): d ln(w q)/d roughness.  Specular
+    // sample: w q = brdf (F and the lobe pick do not depend on roughness), so
+    // d ln(D G); when the pdf floor replaced pdf by 1, w q = brdf q, one more
+    // d ln D.  Diffuse sample: 0.
+    float dlogf_dr;
 };
+
+// d ln D_GGX / d roughness at a fixed microfacet normal (D = a2 / (pi b^2),
+// a2 = r^4, b = NdotM^2 (a2 - 1) + 1).
+__device__ __forceinline__ float dlog_d_dr(float r, float NdotM) {
+    float a2 = (r * r) * (r * r);
+    float m2 = NdotM * NdotM;
+    float b = m2 * (a2 - 1.0f) + 1.0f;
+    return 4.0f / r - 8.0f * r * r * r * m2 / b;
+}
+// d ln G_smith / d roughness at fixed directions (G1(x) = x / (x (1 - k) + k),
+// k = r^2 / 2).
+__device__ __forceinline__ float dlog_g_dr(float r, float NdotO, float NdotI) {
+    float k = r * r / 2.0f;
+    return -r * (1.0f - NdotI) / (NdotI * (1.0f - k) + k) - r * (1.0f - NdotO) / (NdotO * (1.0f - k) + k);
+}
 
 // Optional horizon clamp (PULSE_FLAG_HORIZON_CLAMP, not in the reference):
 // the path ends when the view or the sampled direction is below the shading
@@ -487,8 +509,9 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
     S pdf_s = sconst<S>(0.0f), pdf_d = sconst<S>(0.0f);
     SampleT<S> out;
     out.diffuse = !(e0 <= val(F0_max));
-    float dbase_num = 0.0f;
+    float dbase_num = 0.0f, dlog_d = 0.0f;
     out.NdotO = val(NdotO);
+    out.dlogf_dr = 0.0f;
     if (!out.diffuse) {
         S a = m.roughness * m.roughness;
         S a2 = a * a;
@@ -524,6 +547,8 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
         pdf_s = D * NdotM / (4.0f * OdotM);
         out.NdotI = val(NdotI);
+        dlog_d = dlog_d_dr(val(m.roughness), val(NdotM));
+        out.dlogf_dr = dlog_d + dlog_g_dr(val(m.roughness), val(NdotO), val(NdotI));
     } else {
         wi = sample_cosine_hemisphere(n, e1, e2);
         S NdotI = dot(n, wi);
@@ -533,7 +558,10 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         dbase_num = val(kPI * NdotI);
     }
     S pdf = ((1.0f - F0_max) * pdf_d) + (F0_max * pdf_s);
-    if (val(pdf) < 0.0001f) pdf = sconst<S>(1.0f);
+    if (val(pdf) < 0.0001f) {
+        pdf = sconst<S>(1.0f);
+        out.dlogf_dr += dlog_d;
+    }
     out.wi = wi;
     out.reflectance = divs(brdf, pdf);
     out.dbase = dbase_num / val(pdf);
@@ -569,6 +597,7 @@ struct NeeSampleT {
     V3T<S> contrib;      // direct light if unoccluded (before * throughput)
     V3 d_base;           // d contrib_c / d base_c (Lambert evaluator)
     V3 d_emissive;       // d contrib_c / d emissive_c of the light's material
+    V3 d_rough;          // d contrib_c / d roughness at fixed light sample (GGX evaluator)
     uint32_t light_material;
 };
 
@@ -614,6 +643,7 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
     S cos_r = dot(ns.dir, n0);
     S cos_e = dot(neg(ns.dir), nl);
     V3T<S> brdf;
+    float dlog_r = 0.0f;
     if (!ggx) {
         brdf = lift<S>(mk(m.base.x * kINV_PI, m.base.y * kINV_PI, m.base.z * kINV_PI));
     } else {
@@ -629,6 +659,7 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
         S G = G_smith(NdotO, NdotI, m.roughness);
         S denom = 4.0f * NdotO * NdotI;
         brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
+        dlog_r = dlog_d_dr(val(m.roughness), val(NdotM)) + dlog_g_dr(val(m.roughness), val(NdotO), val(NdotI));
     }
     ns.light_material = mi.material_index;
     V3 le = ld3(sc.materials[mi.material_index].emissive);
@@ -643,6 +674,8 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
     float g = val(cos_r) * val(cos_e) / pdf / val(dsq);
     ns.d_emissive = mk(val(dl.x) > 0.0f ? val(brdf.x) * g : 0.0f, val(dl.y) > 0.0f ? val(brdf.y) * g : 0.0f,
                        val(dl.z) > 0.0f ? val(brdf.z) * g : 0.0f);
+    ns.d_rough = mk(val(dl.x) > 0.0f ? val(dl.x) * dlog_r : 0.0f, val(dl.y) > 0.0f ? val(dl.y) * dlog_r : 0.0f,
+                    val(dl.z) > 0.0f ? val(dl.z) * dlog_r : 0.0f);
     ns.d_base = ggx ? mk(0.0f, 0.0f, 0.0f)
                     : mk(val(dl.x) > 0.0f ? kINV_PI * le.x * g : 0.0f, val(dl.y) > 0.0f ? kINV_PI * le.y * g : 0.0f,
                          val(dl.z) > 0.0f ? kINV_PI * le.z * g : 0.0f);

@@ -263,7 +263,12 @@ constexpr int kAdjBlock = 128;
 template <int STACK>
 void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
     uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
-    hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream, dev_scene(ctx), a);
+    if (a.detached)
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+                           dev_scene(ctx), a);
+    else
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, false>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+                           dev_scene(ctx), a);
 }
 }  // namespace
 
@@ -698,6 +703,7 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     a.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
     a.n_materials = ctx->n_materials;
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
+    a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
     a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
     const size_t lds = a.use_lds ? lds_bytes : 0;

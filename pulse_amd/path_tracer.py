@@ -240,6 +240,7 @@ class PulseAdjoint:
         self.target = device.alloc(width * height * 16)
         self.grad = device.alloc(max(n_materials * N.GRAD_PER_MATERIAL * 4, 4))
         self.replay = None
+        self.image_in = None
 
     def set_target(self, target: np.ndarray) -> None:
         self.target.copy_from(np.ascontiguousarray(target, np.float32))
@@ -259,10 +260,20 @@ class PulseAdjoint:
                                          self.width * self.height, C.byref(out)))
         return out.value
 
-    def backward(self, camera: N.Camera, params: N.Params, want_replay: bool = False):
+    def backward(self, camera: N.Camera, params: N.Params, want_replay: bool = False, image: np.ndarray = None):
+        """dL/d(material parameters).  dL/dI = 2 (image - target) / (3 N) is
+        taken from the last forward image, or from `image` when given (an
+        independently sampled image decorrelates dL/dI from the replayed
+        paths; any array turns L into a fixed linear functional of I)."""
         if want_replay and self.replay is None:
             self.replay = self.device.alloc(self.width * self.height * 16)
-        N.check(N.lib().pulse_adjoint(self.device.handle, C.byref(camera), C.byref(params), self.image.ptr,
+        src = self.image
+        if image is not None:
+            if self.image_in is None:
+                self.image_in = self.device.alloc(self.width * self.height * 16)
+            self.image_in.copy_from(np.ascontiguousarray(image, np.float32))
+            src = self.image_in
+        N.check(N.lib().pulse_adjoint(self.device.handle, C.byref(camera), C.byref(params), src.ptr,
                                       self.target.ptr, self.grad.ptr, self.replay.ptr if want_replay else None))
         self.device.sync()
         g = self.grad.copy_to(np.zeros(self.n_materials * N.GRAD_PER_MATERIAL, np.float32))

@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
             ("oracle_render_pair", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, C.c_int, C.c_int, _P]),
             ("oracle_grad_fd", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
                                          C.c_double, _P, _P, C.c_int]),
+            ("oracle_grad_fd_ex", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
+                                            C.c_double, _P, _P, C.c_int, C.c_int]),
             ("oracle_rand_u", _U32, [C.POINTER(_U32)]),
             ("oracle_rand_f", C.c_float, [C.POINTER(_U32)]),
             ("oracle_pixel_seed", _U32, [_U32, _U32]),
@@ -125,15 +127,19 @@ class OracleScene:
                                      cnt.ctypes.data, rows[0], rows[1])
         return accum, parse_counters(cnt), ms
 
-    def grad_fd(self, camera, params, target, mat_params, ids, h=1e-6, threads=8):
-        """dL/dparam[ids] by f64 central differences along the f32 decisions."""
+    def grad_fd(self, camera, params, target, mat_params, ids, h=1e-6, threads=8, frozen=False):
+        """dL/dparam[ids] by f64 central differences along the f32 decisions;
+        frozen=True also holds the sampled specular half vectors fixed (the
+        detached roughness derivative); frozen="free" replays nothing (f64
+        renders with free decisions: the derivative of the estimator itself)."""
         target = np.ascontiguousarray(target, np.float32)
         mp = np.ascontiguousarray(mat_params, np.float64)
         ids = np.ascontiguousarray(ids, np.uint32)
         out = np.zeros(len(ids), np.float64)
         loss = C.c_double()
-        lib().oracle_grad_fd(self.h, C.byref(camera), C.byref(params), target.ctypes.data, mp.ctypes.data,
-                             len(mp) // 7, ids.ctypes.data, len(ids), h, out.ctypes.data, C.byref(loss), threads)
+        lib().oracle_grad_fd_ex(self.h, C.byref(camera), C.byref(params), target.ctypes.data, mp.ctypes.data,
+                                len(mp) // 7, ids.ctypes.data, len(ids), h, out.ctypes.data, C.byref(loss), threads,
+                                2 if frozen == "free" else int(bool(frozen)))
         return out, loss.value
 
     def render_pair(self, camera, params, replay=True, threads=8):
