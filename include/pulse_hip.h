@@ -270,6 +270,18 @@ int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out);
  * uploaded scene. */
 int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials, uint32_t count);
 
+/* Device-resident scenes beside the uploaded one, for rendering several
+ * scenes from one context (the batched form; the reference keeps one scene
+ * per render world, src/scene/mod.rs:911).  pulse_scene_create validates and
+ * uploads like pulse_scene_upload; pulse_scene_bind makes a scene the one
+ * that pulse_trace, pulse_trace_rays, pulse_adjoint and
+ * pulse_scene_update_materials use (NULL: the uploaded scene again).  A
+ * scene belongs to its context; destroy scenes before the context. */
+typedef struct pulse_scene pulse_scene;
+int pulse_scene_create(pulse_ctx *ctx, const pulse_scene_desc *desc, pulse_scene **out);
+int pulse_scene_destroy(pulse_scene *scene);
+int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
+
 /* ------------------------------------------------------------------------ */
 /* Hot path                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -301,6 +313,60 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                    /* the exact reciprocal-multiply slab division     */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form: many views rendered by one captured launch sequence         */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference, which renders one camera per render-graph
+ * node (src/path_tracer/node.rs:40-119); each view here is what one such
+ * node renders: a scene, a camera (its own resolution) and an rgba32f
+ * accumulation buffer (device, width*height*4 floats) updated with the
+ * reference's running mean.  Every view's image is bit-identical to
+ * pulse_trace of that view with the same params. */
+typedef struct {
+    pulse_scene *scene;
+    pulse_camera camera;
+    float *accum_rgba;
+} pulse_view;
+
+/* Launch shape of a batch; none of it changes the images. */
+typedef struct {
+    uint32_t persistent_waves_per_cu; /* 0: one wave per chunk of rays; n: a fixed */
+                                      /* grid of n waves per CU drains the queues   */
+    uint32_t chunk_rays;              /* rays per work item: 64, 128, 256 or 512    */
+    uint32_t stack_entries;           /* LDS traversal stack (0: smallest that fits) */
+    uint32_t traversal_loop;          /* as PULSE_OPT_TRAVERSAL_LOOP                */
+    uint32_t regroup_every;           /* compact the extension queue every n bounces */
+                                      /* (in between, rays stay in place)           */
+    uint32_t tile_width;              /* primary-ray tile: width x (64 / width)      */
+    uint32_t use_graph;               /* 1: replay one captured hipGraph per frame   */
+    uint32_t _reserved;
+} pulse_batch_config;
+
+typedef struct pulse_batch pulse_batch;
+int pulse_batch_default_config(pulse_batch_config *out);
+/* Allocates the per-view queues and path state (about 164 bytes per pixel
+ * of all views).  Resolutions, scenes, accumulation buffers and params are
+ * fixed for the batch's life. */
+int pulse_batch_create(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views, const pulse_params *params,
+                       pulse_batch **out);
+int pulse_batch_destroy(pulse_batch *batch);
+/* Renders params->spp samples of every view, sample s with seed count
+ * accumulation_count + s (asynchronous on the context stream). */
+int pulse_batch_render(pulse_batch *batch, uint32_t accumulation_count);
+/* New camera matrices for a view (same width and height). */
+int pulse_batch_set_camera(pulse_batch *batch, uint32_t view, const pulse_camera *camera);
+int pulse_batch_set_config(pulse_batch *batch, const pulse_batch_config *config);
+int pulse_batch_get_config(pulse_batch *batch, pulse_batch_config *out);
+/* Times frames under alternative launch shapes (coordinate descent over the
+ * fields of pulse_batch_config, best of `reps` frames each) and keeps the
+ * fastest.  It renders into the views' accumulation buffers with
+ * accumulation_count 0: tune before the frames that count. */
+int pulse_batch_autotune(pulse_batch *batch, uint32_t reps, pulse_batch_config *best, double *best_ms);
+/* Extension and shadow queue entries since the last call (synchronous). */
+int pulse_batch_get_stats(pulse_batch *batch, pulse_stats *out);
+/* One-shot: create, render params->accumulation_count, destroy. */
+int pulse_trace_batch(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views, const pulse_params *params);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiable form: seed-replay adjoint of an L2 image loss              */

@@ -189,6 +189,28 @@ _U32 = C.c_uint32
 _I = C.c_int
 
 # (name, restype, argtypes) of every exported symbol declared in include/*.h
+class View(C.Structure):
+    """pulse_view: one view of a batch."""
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("accum_rgba", C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    """pulse_batch_config: launch shape of a batch (does not change images)."""
+    _fields_ = [
+        ("persistent_waves_per_cu", C.c_uint32),
+        ("chunk_rays", C.c_uint32),
+        ("stack_entries", C.c_uint32),
+        ("traversal_loop", C.c_uint32),
+        ("regroup_every", C.c_uint32),
+        ("tile_width", C.c_uint32),
+        ("use_graph", C.c_uint32),
+        ("_reserved", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if not f.startswith("_")}
+
+
 SIGNATURES = [
     ("pulse_last_error", C.c_char_p, []),
     ("pulse_version", C.c_char_p, []),
@@ -213,6 +235,19 @@ SIGNATURES = [
     ("pulse_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, _P]),
     ("pulse_forward_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, C.POINTER(C.c_double)]),
     ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),
+    ("pulse_scene_create", _I, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
+    ("pulse_scene_destroy", _I, [_P]),
+    ("pulse_scene_bind", _I, [_P, _P]),
+    ("pulse_batch_default_config", _I, [C.POINTER(BatchConfig)]),
+    ("pulse_batch_create", _I, [_P, C.POINTER(View), _U32, C.POINTER(Params), C.POINTER(_P)]),
+    ("pulse_batch_destroy", _I, [_P]),
+    ("pulse_batch_render", _I, [_P, _U32]),
+    ("pulse_batch_set_camera", _I, [_P, _U32, C.POINTER(Camera)]),
+    ("pulse_batch_set_config", _I, [_P, C.POINTER(BatchConfig)]),
+    ("pulse_batch_get_config", _I, [_P, C.POINTER(BatchConfig)]),
+    ("pulse_batch_autotune", _I, [_P, _U32, C.POINTER(BatchConfig), C.POINTER(C.c_double)]),
+    ("pulse_batch_get_stats", _I, [_P, C.POINTER(Stats)]),
+    ("pulse_trace_batch", _I, [_P, C.POINTER(View), _U32, C.POINTER(Params)]),
     ("pulse_build_blas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),
     ("pulse_build_tlas", _I, [_P, _U32, _P, _U32, C.POINTER(_U32), _P]),
     ("pulse_sb_create", _I, [C.POINTER(_P)]),

@@ -1,0 +1,48 @@
+// queue.h -- pieces shared by the single-view and the batched stage kernels:
+// wave-level queue append, the traversal variant switch and the per-frame
+// arguments of the path loop.
+#pragma once
+
+#include "pt_device.h"
+#include "traverse.h"
+
+namespace pt {
+
+constexpr int kMaxTraces = 64;
+
+// Traversal variants: 0 = the reference's nested TLAS/BLAS loops,
+// 1 = single loop over both levels (traverse.h).  Same results.
+template <int LOOP, bool ANY_HIT>
+__device__ __forceinline__ bool trace_any(const DevScene &sc, V3 o, V3 d, float &t, uint32_t &inst, uint32_t &tri,
+                                          float &u, float &v, float threshold, LdsStack &stk) {
+    if (LOOP == 1) return trace_unified<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
+    return traverse_tlas<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
+}
+
+struct FrameArgs {
+    pulse_camera cam;
+    uint32_t mode;
+    uint32_t nee_ggx;
+    uint32_t max_traces;      // extension traces per path
+    uint32_t rr_start;        // first loop depth with Russian roulette
+    uint32_t accumulation_count;
+    uint32_t horizon_clamp;   // PULSE_FLAG_HORIZON_CLAMP
+};
+
+// Appends `valid` items of a wave to a queue: returns this lane's slot.
+// All 64 lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(bool valid, uint32_t *counter) {
+    uint64_t mask = __ballot(valid);
+    uint32_t total = __popcll(mask);
+    uint32_t lane = __lane_id();
+    uint32_t before = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (total > 0) {
+        int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        if (static_cast<int>(lane) == leader) base = atomicAdd(counter, total);
+        base = __shfl(base, leader);
+    }
+    return base + before;
+}
+
+}  // namespace pt

@@ -1,0 +1,104 @@
+// context.h -- host-side state of libpulse_hip shared by its translation
+// units: device buffers, device-resident scenes and the context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../device/pt_device.h"
+#include "../device/queue.h"
+#include "common.h"
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            PULSE_FAIL(PULSE_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                              \
+    } while (0)
+
+namespace pulse_host {
+
+// One device allocation that grows on demand.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && ptr) return PULSE_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        size_t b = std::max<size_t>(bytes, 256);
+        HIP_TRY(hipMalloc(&ptr, b));
+        cap = b;
+        return PULSE_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+enum SceneBuf {
+    B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
+    B_CDFS, B_AREAS, B_LIGHTS, B_COUNT
+};
+
+enum WfBuf {
+    W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_HIT_V, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C,
+    W_COUNTERS, W_COUNT
+};
+
+// A validated scene in device memory (the 13 bindings, with the triangle
+// records regathered in BVH order) and what the launches need to know of it.
+struct SceneData {
+    DevBuf buf[B_COUNT];
+    pulse_scene_uniform uniform{};
+    uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
+    uint32_t tlas_depth = 0, blas_depth = 0;
+    uint32_t n_materials = 0;
+    bool fast_div = false;       // node coordinates allow the exact fast division
+    bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
+    bool valid = false;
+    void release() {
+        for (auto &b : buf) b.release();
+        valid = false;
+    }
+};
+
+// Validates `d` and uploads it into `s` (left unchanged on failure).
+int scene_upload(SceneData &s, const pulse_scene_desc *d);
+// Device view of a scene; `errors` is the context's device error word.
+pt::DevScene dev_scene(const SceneData &s, uint32_t *errors);
+// Smallest compiled stack size that holds `need` entries (0: none).
+uint32_t pick_stack(uint32_t need, int loop_variant);
+
+}  // namespace pulse_host
+
+struct pulse_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    pulse_host::SceneData own;            // the scene of pulse_scene_upload
+    pulse_host::SceneData *cur = nullptr; // the scene the launches use
+    pulse_host::DevBuf wf[pulse_host::W_COUNT];
+    pulse_host::DevBuf errors;  // u32 device error word
+    size_t wf_pixels = 0;
+    uint64_t generation = 0;
+    uint32_t stack_entries = 0;  // traversal stack capacity in use
+    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
+    pulse_host::DevBuf loss_acc; // f64 accumulator of pulse_image_loss
+    pulse_stats stats{};
+    hipEvent_t ev[3 * pt::kMaxTraces + 8] = {};
+    bool events_created = false;
+    bool has_scene() const { return cur && cur->valid; }
+};
+
+struct pulse_scene {
+    pulse_ctx *ctx = nullptr;
+    pulse_host::SceneData data;
+};

@@ -11,100 +11,17 @@
 #include "device/adjoint.h"
 #include "device/wavefront.h"
 #include "host/common.h"
+#include "host/context.h"
 
 using namespace pt;
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            PULSE_FAIL(PULSE_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                       __LINE__);                                                              \
-    } while (0)
-
-namespace {
-
-// One device allocation that grows on demand.
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes) {
-        if (bytes <= cap && ptr) return PULSE_OK;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-        size_t b = std::max<size_t>(bytes, 256);
-        HIP_TRY(hipMalloc(&ptr, b));
-        cap = b;
-        return PULSE_OK;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(ptr); }
-};
-
-enum SceneBuf {
-    B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
-    B_CDFS, B_AREAS, B_LIGHTS, B_COUNT
-};
-
-enum WfBuf {
-    W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_HIT_V, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C,
-    W_COUNTERS, W_COUNT
-};
-
-}  // namespace
-
-struct pulse_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int num_cus = 0;
-    DevBuf scene[B_COUNT];
-    DevBuf wf[W_COUNT];
-    DevBuf errors;  // u32 device error word
-    size_t wf_pixels = 0;
-    bool has_scene = false;
-    uint64_t generation = 0;
-    pulse_scene_uniform uniform{};
-    uint32_t stack_entries = 0;  // traversal stack capacity in use
-    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
-    uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
-    bool fast_div = false;       // node coordinates allow the exact fast division
-    bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
-    uint32_t n_materials = 0;
-    DevBuf loss_acc;             // f64 accumulator of pulse_image_loss
-    uint32_t tlas_depth = 0, blas_depth = 0;
-    pulse_stats stats{};
-    uint32_t last_traces = 0;
-    hipEvent_t ev[3 * kMaxTraces + 8] = {};
-    bool events_created = false;
-};
+using namespace pulse_host;
 
 namespace {
 
 int set_device(pulse_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     return PULSE_OK;
-}
-
-// Smallest compiled stack size that holds tlas_depth + blas_depth entries:
-// a depth-first traversal that pushes at most one far child per level never
-// holds more entries than the depth of the node it is at.
-uint32_t pick_stack(uint32_t need, int loop_variant) {
-    static const uint32_t nested[] = {32, 48, 64, 128};
-    static const uint32_t unified[] = {24, 32, 40, 48, 64, 128};
-    if (loop_variant == 0) {
-        for (uint32_t s : nested)
-            if (s >= need) return s;
-    } else {
-        for (uint32_t s : unified)
-            if (s >= need) return s;
-    }
-    return 0;
 }
 
 // Depth of the BLAS rooted at node_offset `root` (child indices are local to
@@ -157,24 +74,18 @@ bool walk_tlas(const pulse_tlas_node *nodes, size_t n, size_t n_iidx, uint32_t &
     return true;
 }
 
-DevScene dev_scene(pulse_ctx *ctx) {
-    DevScene s;
-    s.tri_bvh = ctx->scene[B_TRI_BVH].as<float4>();
-    s.nrm_bvh = ctx->scene[B_NRM_BVH].as<float4>();
-    s.primitives = ctx->scene[B_PRIMS].as<pulse_primitive>();
-    s.blas_nodes = ctx->scene[B_BLAS].as<float4>();
-    s.tlas_nodes = ctx->scene[B_TLAS].as<float4>();
-    s.instance_indices = ctx->scene[B_INST_IDX].as<uint32_t>();
-    s.instances = ctx->scene[B_INSTANCES].as<pulse_mesh_instance>();
-    s.materials = ctx->scene[B_MATERIALS].as<pulse_material>();
-    s.light_cdfs = ctx->scene[B_CDFS].as<float>();
-    s.light_areas = ctx->scene[B_AREAS].as<float>();
-    s.lights = ctx->scene[B_LIGHTS].as<pulse_light_data_index>();
-    s.instance_count = ctx->uniform.instance_count;
-    s.light_count = ctx->uniform.light_count;
-    s.error_flags = ctx->errors.as<uint32_t>();
-    s.fast_div = ctx->fast_div;
-    return s;
+DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx->errors.as<uint32_t>()); }
+
+// Makes `sd` the scene of the context's launches.
+int bind_scene(pulse_ctx *ctx, SceneData *sd) {
+    uint32_t st = pick_stack(sd->stack_need, ctx->loop_variant);
+    if (st == 0)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the %s stack",
+                   sd->tlas_depth, sd->blas_depth, ctx->loop_variant == 0 ? "nested loop's" : "single loop's");
+    ctx->cur = sd;
+    ctx->stack_entries = st;
+    ctx->generation += 1;
+    return PULSE_OK;
 }
 
 constexpr int kTraceBlock = 64;  // one wave per block: LDS stack granularity
@@ -258,114 +169,48 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
 
 }  // namespace
 
-namespace {
-constexpr int kAdjBlock = 128;
-template <int STACK>
-void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
-    uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
-    if (a.detached)
-        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
-                           dev_scene(ctx), a);
-    else
-        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, false>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
-                           dev_scene(ctx), a);
-}
-}  // namespace
+namespace pulse_host {
 
-// ---------------------------------------------------------------------------
-// C ABI
-// ---------------------------------------------------------------------------
-extern "C" {
-
-const char *pulse_version(void) { return "pulse_hip 0.1 (gfx950)"; }
-
-int pulse_create(int device, pulse_ctx **out) {
-    if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: null out");
-    int n = 0;
-    HIP_TRY(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: device %d of %d", device, n);
-    auto *ctx = new pulse_ctx();
-    ctx->device = device;
-    hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
-    if (e != hipSuccess) {
-        delete ctx;
-        PULSE_FAIL(PULSE_E_HIP, "pulse_create: %s", hipGetErrorString(e));
+// A depth-first traversal that pushes at most one far child per level never
+// holds more entries than the depth of the node it is at.
+uint32_t pick_stack(uint32_t need, int loop_variant) {
+    static const uint32_t nested[] = {32, 48, 64, 128};
+    static const uint32_t unified[] = {24, 32, 40, 48, 64, 128};
+    if (loop_variant == 0) {
+        for (uint32_t s : nested)
+            if (s >= need) return s;
+    } else {
+        for (uint32_t s : unified)
+            if (s >= need) return s;
     }
-    if (ctx->errors.ensure(sizeof(uint32_t)) != PULSE_OK || hipMemset(ctx->errors.ptr, 0, sizeof(uint32_t)) != hipSuccess) {
-        pulse_destroy(ctx);
-        PULSE_FAIL(PULSE_E_HIP, "pulse_create: cannot allocate the error word");
-    }
-    *out = ctx;
-    return PULSE_OK;
+    return 0;
 }
 
-int pulse_destroy(pulse_ctx *ctx) {
-    if (!ctx) return PULSE_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &b : ctx->scene) b.release();
-    for (auto &b : ctx->wf) b.release();
-    ctx->errors.release();
-    ctx->loss_acc.release();
-    if (ctx->events_created)
-        for (auto &e : ctx->ev) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-    return PULSE_OK;
+
+DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
+    DevScene s;
+    s.tri_bvh = sd.buf[B_TRI_BVH].as<float4>();
+    s.nrm_bvh = sd.buf[B_NRM_BVH].as<float4>();
+    s.primitives = sd.buf[B_PRIMS].as<pulse_primitive>();
+    s.blas_nodes = sd.buf[B_BLAS].as<float4>();
+    s.tlas_nodes = sd.buf[B_TLAS].as<float4>();
+    s.instance_indices = sd.buf[B_INST_IDX].as<uint32_t>();
+    s.instances = sd.buf[B_INSTANCES].as<pulse_mesh_instance>();
+    s.materials = sd.buf[B_MATERIALS].as<pulse_material>();
+    s.light_cdfs = sd.buf[B_CDFS].as<float>();
+    s.light_areas = sd.buf[B_AREAS].as<float>();
+    s.lights = sd.buf[B_LIGHTS].as<pulse_light_data_index>();
+    s.instance_count = sd.uniform.instance_count;
+    s.light_count = sd.uniform.light_count;
+    s.error_flags = errors;
+    s.fast_div = sd.fast_div;
+    return s;
 }
 
-void *pulse_stream(pulse_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
-
-int pulse_sync(pulse_ctx *ctx) {
-    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sync: null ctx");
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    uint32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
-    if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
-    if (err & kErrIterationCap) PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached the 100000-iteration cap");
-    return PULSE_OK;
-}
-
-int pulse_device_alloc(pulse_ctx *ctx, size_t bytes, void **out) {
-    if (!ctx || !out || bytes == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_alloc: bad argument");
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMalloc(out, bytes));
-    return PULSE_OK;
-}
-int pulse_device_free(pulse_ctx *ctx, void *p) {
-    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_free: null ctx");
-    HIP_TRY(hipSetDevice(ctx->device));
-    if (p) HIP_TRY(hipFree(p));
-    return PULSE_OK;
-}
-int pulse_copy_h2d(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_h2d: bad argument");
-    HIP_TRY(hipSetDevice(ctx->device));
-    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return PULSE_OK;
-}
-int pulse_copy_d2h(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_d2h: bad argument");
-    HIP_TRY(hipSetDevice(ctx->device));
-    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return PULSE_OK;
-}
-int pulse_memset_d(pulse_ctx *ctx, void *dst, int value, size_t bytes) {
-    if (!ctx || (bytes && !dst)) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_memset_d: bad argument");
-    HIP_TRY(hipSetDevice(ctx->device));
-    if (bytes) HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
-    return PULSE_OK;
-}
-
-int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
-    if (!ctx || !d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
-    int rc = set_device(ctx);
-    if (rc) return rc;
+// Validation failures leave `sd` unchanged.
+int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
+    if (!d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
+    int rc = PULSE_OK;
     auto count_of = [](const pulse_buffer_view &v, size_t elem, const char *name, size_t &n) -> int {
         if (v.bytes % elem != 0)
             PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: %s has %zu bytes, not a multiple of %zu", name, v.bytes, elem);
@@ -464,7 +309,6 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
             PULSE_FAIL(PULSE_E_LAYOUT, "light %zu: cdf range outside light_triangle_area_cdfs", l);
     }
     uint32_t need = tlas_depth + blas_depth + 1;
-    uint32_t stack = pick_stack(need, ctx->loop_variant);
     // node coordinates: 0 or magnitude in [2^-56, 2^40] for the fast division
     auto coord_ok = [](float c) {
         float a = std::fabs(c);
@@ -475,15 +319,16 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
         for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(blas[k].aabb_min[c]) && coord_ok(blas[k].aabb_max[c]);
     for (size_t k = 0; k < n_tlas && fast_div; ++k)
         for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(tlas[k].aabb_min[c]) && coord_ok(tlas[k].aabb_max[c]);
-    if (stack == 0)
+    if (pick_stack(need, 0) == 0 && pick_stack(need, 1) == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
                    tlas_depth, blas_depth);
 
     // --- copy to device ---
+    sd.valid = false;
     auto up = [&](int b, const void *src, size_t bytes) -> int {
-        int r = ctx->scene[b].ensure(bytes);
+        int r = sd.buf[b].ensure(bytes);
         if (r) return r;
-        if (bytes) HIP_TRY(hipMemcpy(ctx->scene[b].ptr, src, bytes, hipMemcpyHostToDevice));
+        if (bytes) HIP_TRY(hipMemcpy(sd.buf[b].ptr, src, bytes, hipMemcpyHostToDevice));
         return PULSE_OK;
     };
     if ((rc = up(B_PRIMS, prims, d->primitives.bytes)) || (rc = up(B_TRI_BVH, tri_bvh.data(), tri_bvh.size() * sizeof(float4))) ||
@@ -494,19 +339,132 @@ int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
         (rc = up(B_AREAS, d->light_mesh_areas.data, d->light_mesh_areas.bytes)) ||
         (rc = up(B_LIGHTS, lights, d->light_indices.bytes)))
         return rc;
-    ctx->uniform = d->uniform;
-    ctx->stack_entries = stack;
-    ctx->stack_need = need;
-    ctx->n_materials = static_cast<uint32_t>(n_mat);
-    ctx->any_metallic = false;
+    sd.uniform = d->uniform;
+    sd.stack_need = need;
+    sd.n_materials = static_cast<uint32_t>(n_mat);
+    sd.any_metallic = false;
     for (size_t k = 0; k < n_mat; ++k)
-        if (static_cast<const pulse_material *>(d->materials.data)[k].metallic != 0.0f) ctx->any_metallic = true;
-    ctx->fast_div = fast_div;
-    ctx->tlas_depth = tlas_depth;
-    ctx->blas_depth = blas_depth;
-    ctx->has_scene = true;
-    ctx->generation += 1;
+        if (static_cast<const pulse_material *>(d->materials.data)[k].metallic != 0.0f) sd.any_metallic = true;
+    sd.fast_div = fast_div;
+    sd.tlas_depth = tlas_depth;
+    sd.blas_depth = blas_depth;
+    sd.valid = true;
     return PULSE_OK;
+}
+
+}  // namespace pulse_host
+
+namespace {
+constexpr int kAdjBlock = 128;
+template <int STACK>
+void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
+    uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
+    if (a.detached)
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+                           dev_scene(ctx), a);
+    else
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, false>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+                           dev_scene(ctx), a);
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *pulse_version(void) { return "pulse_hip 0.1 (gfx950)"; }
+
+int pulse_create(int device, pulse_ctx **out) {
+    if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: null out");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_create: device %d of %d", device, n);
+    auto *ctx = new pulse_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        PULSE_FAIL(PULSE_E_HIP, "pulse_create: %s", hipGetErrorString(e));
+    }
+    if (ctx->errors.ensure(sizeof(uint32_t)) != PULSE_OK || hipMemset(ctx->errors.ptr, 0, sizeof(uint32_t)) != hipSuccess) {
+        pulse_destroy(ctx);
+        PULSE_FAIL(PULSE_E_HIP, "pulse_create: cannot allocate the error word");
+    }
+    *out = ctx;
+    return PULSE_OK;
+}
+
+int pulse_destroy(pulse_ctx *ctx) {
+    if (!ctx) return PULSE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->own.release();
+    for (auto &b : ctx->wf) b.release();
+    ctx->errors.release();
+    ctx->loss_acc.release();
+    if (ctx->events_created)
+        for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PULSE_OK;
+}
+
+void *pulse_stream(pulse_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+int pulse_sync(pulse_ctx *ctx) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sync: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
+    if (err & kErrIterationCap) PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached the 100000-iteration cap");
+    return PULSE_OK;
+}
+
+int pulse_device_alloc(pulse_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out || bytes == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_alloc: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(out, bytes));
+    return PULSE_OK;
+}
+int pulse_device_free(pulse_ctx *ctx, void *p) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_device_free: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (p) HIP_TRY(hipFree(p));
+    return PULSE_OK;
+}
+int pulse_copy_h2d(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_h2d: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PULSE_OK;
+}
+int pulse_copy_d2h(pulse_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_copy_d2h: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PULSE_OK;
+}
+int pulse_memset_d(pulse_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!ctx || (bytes && !dst)) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_memset_d: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return PULSE_OK;
+}
+
+int pulse_scene_upload(pulse_ctx *ctx, const pulse_scene_desc *d) {
+    if (!ctx || !d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = scene_upload(ctx->own, d)) != PULSE_OK) return rc;
+    return bind_scene(ctx, &ctx->own);
 }
 
 int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out) {
@@ -517,22 +475,57 @@ int pulse_scene_generation(pulse_ctx *ctx, uint64_t *out) {
 
 int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials, uint32_t count) {
     if (!ctx || (count && !materials)) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_update_materials: bad argument");
-    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_update_materials: no scene uploaded");
-    if (count != ctx->n_materials)
-        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_materials: %u materials, scene has %u", count, ctx->n_materials);
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_update_materials: no scene uploaded");
+    if (count != ctx->cur->n_materials)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_materials: %u materials, scene has %u", count, ctx->cur->n_materials);
     int rc = set_device(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (count) HIP_TRY(hipMemcpy(ctx->scene[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
-    ctx->any_metallic = false;
+    if (count) HIP_TRY(hipMemcpy(ctx->cur->buf[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
+    ctx->cur->any_metallic = false;
     for (uint32_t k = 0; k < count; ++k)
-        if (materials[k].metallic != 0.0f) ctx->any_metallic = true;
+        if (materials[k].metallic != 0.0f) ctx->cur->any_metallic = true;
     return PULSE_OK;
+}
+
+int pulse_scene_create(pulse_ctx *ctx, const pulse_scene_desc *desc, pulse_scene **out) {
+    if (!ctx || !desc || !out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_create: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    auto *sc = new pulse_scene();
+    sc->ctx = ctx;
+    if ((rc = scene_upload(sc->data, desc)) != PULSE_OK) {
+        sc->data.release();
+        delete sc;
+        return rc;
+    }
+    *out = sc;
+    return PULSE_OK;
+}
+
+int pulse_scene_destroy(pulse_scene *scene) {
+    if (!scene) return PULSE_OK;
+    pulse_ctx *ctx = scene->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->cur == &scene->data) ctx->cur = ctx->own.valid ? &ctx->own : nullptr;
+    scene->data.release();
+    delete scene;
+    return PULSE_OK;
+}
+
+int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_bind: null ctx");
+    if (scene && scene->ctx != ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_bind: scene of another context");
+    SceneData *sd = scene ? &scene->data : &ctx->own;
+    if (!sd->valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_bind: no scene uploaded");
+    if (ctx->cur == sd) return PULSE_OK;
+    return bind_scene(ctx, sd);
 }
 
 int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
     if (!ctx || (n && (!rays || !hits))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_rays: bad argument");
-    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_rays: no scene uploaded");
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_rays: no scene uploaded");
     int rc = set_device(ctx);
     if (rc) return rc;
     if (n == 0) return PULSE_OK;
@@ -543,7 +536,7 @@ int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uin
 
 int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occluded, uint32_t n) {
     if (!ctx || (n && (!rays || !occluded))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_shadow_rays: bad argument");
-    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_shadow_rays: no scene uploaded");
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_shadow_rays: no scene uploaded");
     int rc = set_device(ctx);
     if (rc) return rc;
     if (n == 0) return PULSE_OK;
@@ -554,7 +547,7 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 
 int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
     if (!ctx || !cam || !prm || !accum_rgba) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: null argument");
-    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace: no scene uploaded");
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace: no scene uploaded");
     if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: empty image");
     if (prm->mode > PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown mode %u", prm->mode);
     if (prm->nee_eval > PULSE_NEE_GGX) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown nee_eval %u", prm->nee_eval);
@@ -673,7 +666,7 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *image,
                   const float *target, float *grad, float *replay_rgba) {
     if (!ctx || !cam || !prm || !image || !target || !grad) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: null argument");
-    if (!ctx->has_scene) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
     if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs PULSE_MODE_GGX_NEE");
     if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVertices);
@@ -681,12 +674,12 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: Russian roulette must be off (rr_start_depth >= max_depth)");
     if (prm->accumulation_count != 0 || prm->spp == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs accumulation_count 0 and spp >= 1");
-    if (ctx->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: materials with metallic != 0");
+    if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: materials with metallic != 0");
     if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: BVH too deep for the adjoint stack");
     int rc = set_device(ctx);
     if (rc) return rc;
     const uint32_t npix = cam->width * cam->height;
-    const uint32_t nent = ctx->n_materials * kGradPerMaterial;
+    const uint32_t nent = ctx->cur->n_materials * kGradPerMaterial;
     if (nent) {
         hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
     }
@@ -701,7 +694,7 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     a.grad = grad;
     a.replay = reinterpret_cast<float4 *>(replay_rgba);
     a.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
-    a.n_materials = ctx->n_materials;
+    a.n_materials = ctx->cur->n_materials;
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
@@ -733,10 +726,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
     switch (option) {
         case PULSE_OPT_TRAVERSAL_LOOP: {
             if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "traversal loop must be 0 or 1");
-            uint32_t st = ctx->has_scene ? pick_stack(ctx->stack_need, static_cast<int>(value)) : 1;
+            uint32_t st = ctx->has_scene() ? pick_stack(ctx->cur->stack_need, static_cast<int>(value)) : 1;
             if (st == 0) PULSE_FAIL(PULSE_E_UNSUPPORTED, "no stack size for this variant and scene");
             ctx->loop_variant = static_cast<int>(value);
-            if (ctx->has_scene) ctx->stack_entries = st;
+            if (ctx->has_scene()) ctx->stack_entries = st;
             return PULSE_OK;
         }
         default:
@@ -749,7 +742,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
     switch (option) {
         case PULSE_OPT_TRAVERSAL_LOOP: *value = ctx->loop_variant; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
-        case PULSE_OPT_FAST_DIVISION: *value = ctx->fast_div ? 1 : 0; return PULSE_OK;
+        case PULSE_OPT_FAST_DIVISION: *value = ctx->cur->fast_div ? 1 : 0; return PULSE_OK;
         default: PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: unknown option %d", option);
     }
 }

@@ -14,6 +14,7 @@ is a device buffer owned by the node (read back with ``readback``).
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -44,6 +45,7 @@ class Device:
 
     def __init__(self, device: int = 0):
         self._ctx = C.c_void_p()
+        self._children = []  # weak references to scenes and batches of this context
         N.check(N.lib().pulse_create(device, C.byref(self._ctx)))
 
     @property
@@ -52,8 +54,26 @@ class Device:
 
     def close(self):
         if self._ctx and self._ctx.value:
+            for ref in self._children:
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             N.lib().pulse_destroy(self._ctx)
             self._ctx = C.c_void_p()
+
+    def _adopt(self, obj) -> None:
+        self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
+
+    # --- several scenes and batched views ---
+    def create_scene(self, scene: SceneBuffers) -> "DeviceScene":
+        """A device-resident scene beside the uploaded one (pulse_scene_create)."""
+        return DeviceScene(self, scene)
+
+    def bind_scene(self, scene: "DeviceScene" = None) -> None:
+        """Makes `scene` (None: the uploaded scene) the one pulse_trace uses."""
+        N.check(N.lib().pulse_scene_bind(self._ctx, scene.handle if scene is not None else None))
 
     def __del__(self):
         self.close()
@@ -115,6 +135,108 @@ class Device:
     def stats(self) -> N.Stats:
         s = N.Stats()
         N.check(N.lib().pulse_get_stats(self._ctx, C.byref(s)))
+        return s
+
+
+class DeviceScene:
+    """pulse_scene: a validated scene in device memory, owned by a Device."""
+
+    def __init__(self, dev: Device, scene: SceneBuffers):
+        self.dev = dev
+        self.buffers = scene
+        self._h = C.c_void_p()
+        d = scene.desc()
+        N.check(N.lib().pulse_scene_create(dev.handle, C.byref(d), C.byref(self._h)))
+        self.n_materials = len(scene.buffers["materials"])
+        dev._adopt(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h and self._h.value:
+            N.lib().pulse_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """pulse_batch: many views (scene, camera, accumulation buffer) rendered
+    by one captured launch sequence per frame."""
+
+    def __init__(self, dev: Device, views, params: N.Params):
+        """views: list of (DeviceScene, Camera); accumulation buffers are
+        allocated here (width*height rgba32f each)."""
+        self.dev = dev
+        self.params = params
+        self.scenes = [sc for sc, _ in views]
+        self.cameras = [cam for _, cam in views]
+        self.accum = [dev.alloc(cam.width * cam.height * 16) for cam in self.cameras]
+        arr = (N.View * len(views))()
+        for i, (sc, cam) in enumerate(views):
+            arr[i].scene = sc.handle.value
+            arr[i].camera = cam
+            arr[i].accum_rgba = self.accum[i].ptr.value
+        self._h = C.c_void_p()
+        N.check(N.lib().pulse_batch_create(dev.handle, arr, len(views), C.byref(params), C.byref(self._h)))
+        dev._adopt(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h and self._h.value:
+            N.lib().pulse_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, accumulation_count: int = 0) -> None:
+        N.check(N.lib().pulse_batch_render(self._h, accumulation_count))
+
+    def images(self):
+        self.dev.sync()
+        return [a.copy_to(np.zeros((c.height, c.width, 4), np.float32)) for a, c in zip(self.accum, self.cameras)]
+
+    def config(self) -> N.BatchConfig:
+        c = N.BatchConfig()
+        N.check(N.lib().pulse_batch_get_config(self._h, C.byref(c)))
+        return c
+
+    def set_config(self, **fields) -> None:
+        """Changes the named fields of the current launch shape (a new
+        traversal loop picks its own stack size unless one is given)."""
+        c = self.config()
+        if "traversal_loop" in fields and "stack_entries" not in fields:
+            c.stack_entries = 0
+        for k, v in fields.items():
+            setattr(c, k, v)
+        N.check(N.lib().pulse_batch_set_config(self._h, C.byref(c)))
+
+    def set_camera(self, view: int, cam: N.Camera) -> None:
+        N.check(N.lib().pulse_batch_set_camera(self._h, view, C.byref(cam)))
+        self.cameras[view] = cam
+
+    def autotune(self, reps: int = 1):
+        c = N.BatchConfig()
+        ms = C.c_double()
+        N.check(N.lib().pulse_batch_autotune(self._h, reps, C.byref(c), C.byref(ms)))
+        return c, ms.value
+
+    def stats(self) -> N.Stats:
+        s = N.Stats()
+        N.check(N.lib().pulse_batch_get_stats(self._h, C.byref(s)))
         return s
 
 
