@@ -144,6 +144,83 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
                       "65 materials x 7 parameters"}
 
 
+def bench_batch(frames: int = 2, check_views: int = 4) -> dict:
+    """Batched form: 32 views, resolutions from {512^2, 720p, 1080p}, icosphere
+    fields of 10k-2M unique triangles (log-uniform), 2 spp, 4 bounces, GGX +
+    Lambert NEE with Russian roulette.  Timed: 32 sequential plain launches
+    (bind scene, pulse_trace) per frame against one batched frame (autotuned
+    launch shape, hipGraph replay); both with scenes resident."""
+    from pulse_amd import _native as N
+    from pulse_amd import procedural as P
+    from pulse_amd.path_tracer import Batch, Device, make_params
+
+    t0 = time.perf_counter()
+    work = P.batch_workload(threads=min(16, os.cpu_count() or 1))
+    build_s = time.perf_counter() - t0
+    dev = Device(0)
+    scenes = [dev.create_scene(s) for s, _, _ in work]
+    cams = [c for _, c, _ in work]
+    prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE)
+    seq_acc = [dev.alloc(c.width * c.height * 16) for c in cams]
+
+    def sequential(acc):
+        p = N.Params.from_buffer_copy(prm)
+        p.accumulation_count = acc
+        for sc, cam, buf in zip(scenes, cams, seq_acc):
+            dev.bind_scene(sc)
+            N.check(N.lib().pulse_trace(dev.handle, cam, p, buf.ptr))
+
+    sequential(0)
+    dev.sync()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        sequential(2 * f + 2)
+    dev.sync()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / frames
+    print(f"bench: batch workload built in {build_s:.1f} s; sequential {seq_ms:.1f} ms/frame", file=sys.stderr,
+          flush=True)
+
+    batch = Batch(dev, list(zip(scenes, cams)), prm)
+    batch.render(0)
+    dev.sync()
+    t0 = time.perf_counter()
+    batch.render(0)
+    dev.sync()
+    default_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    cfg, tuned_ms = batch.autotune(reps=1)
+    tune_s = time.perf_counter() - t0
+    batch.render(0)
+    batch.stats()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        batch.render(2 * f + 2)
+    dev.sync()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / frames
+    st = batch.stats()
+    # bit-exact check of the smallest views against the sequential images (same frames)
+    order = sorted(range(len(cams)), key=lambda i: cams[i].width * cams[i].height)[:check_views]
+    imgs = batch.images()
+    exact = 0
+    for i in order:
+        ref = seq_acc[i].copy_to(np.zeros((cams[i].height, cams[i].width, 4), np.float32))
+        exact += int(np.array_equal(ref.view(np.uint32), imgs[i].view(np.uint32)))
+    rays = (st.extension_rays + st.shadow_rays) / frames
+    out = {"views": len(cams), "pixels": int(sum(c.width * c.height for c in cams)),
+           "unique_triangles": int(sum(i["unique_triangles"] for _, _, i in work)),
+           "spp": 2, "max_depth": 4, "host_build_s": round(build_s, 1),
+           "sequential_ms": round(seq_ms, 2), "batched_ms": round(batch_ms, 2),
+           "batched_default_shape_ms": round(default_ms, 2), "speedup": round(seq_ms / batch_ms, 3),
+           "rays_per_frame": rays, "batched_mrays_per_s": round(rays / batch_ms / 1e3, 1),
+           "autotune_s": round(tune_s, 1), "autotuned_shape": cfg.as_dict(),
+           "bit_exact_views": f"{exact}/{len(order)}"}
+    batch.close()
+    for s in scenes:
+        s.close()
+    dev.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +234,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--loop", type=int, default=1, help="traversal loop variant (0 nested, 1 single loop)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -261,6 +339,10 @@ def main() -> int:
     if not args.no_adjoint:
         result["fwd_adjoint"] = bench_adjoint(scene, camf)
         print(f"bench: forward+adjoint {result['fwd_adjoint']['ratio']}x forward", file=sys.stderr, flush=True)
+
+    if not args.no_batch:
+        result["batched"] = bench_batch()
+        print(f"bench: batched {result['batched']['speedup']}x sequential", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         prm = make_params(accumulation_count=0, max_depth=args.max_depth, mode=N.MODE_GGX_NEE)

@@ -359,8 +359,8 @@ int pulse_batch_set_camera(pulse_batch *batch, uint32_t view, const pulse_camera
 int pulse_batch_set_config(pulse_batch *batch, const pulse_batch_config *config);
 int pulse_batch_get_config(pulse_batch *batch, pulse_batch_config *out);
 /* Times frames under alternative launch shapes (coordinate descent over the
- * fields of pulse_batch_config, best of `reps` frames each) and keeps the
- * fastest.  It renders into the views' accumulation buffers with
+ * fields of pulse_batch_config, two passes, best of `reps` frames per shape,
+ * each shape timed once) and keeps the fastest.  It renders into the views' accumulation buffers with
  * accumulation_count 0: tune before the frames that count. */
 int pulse_batch_autotune(pulse_batch *batch, uint32_t reps, pulse_batch_config *best, double *best_ms);
 /* Extension and shadow queue entries since the last call (synchronous). */

@@ -234,3 +234,34 @@ def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800,
         return make_camera(width, height, eye=(0.0, 0.0, 1.3 * box), target=(0.0, 0.0, 0.0))
 
     return scene, camera
+
+
+def batch_workload(n_views: int = 32, seed: int = 3, tri_range=(10_000, 2_000_000),
+                   resolutions=((512, 512), (1280, 720), (1920, 1080)), threads: int = 8):
+    """Views of the batched form: per view a resolution drawn from
+    `resolutions` and an icosphere field whose unique triangle count is drawn
+    log-uniformly from `tri_range` (subdivision and mesh count chosen to
+    match; 12.5 instances per mesh, 4 % emissive instances, box scaled for
+    the instance density of the 800-instance field).  Returns a list of
+    (SceneBuffers, Camera, info dict)."""
+    rng = CounterRng(seed)
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    u_tris = rng.uniform(1, n_views, lo, hi)
+    u_res = rng.bits(2, n_views)
+    views = []
+    for v in range(n_views):
+        target = float(np.exp(u_tris[v]))
+        subdiv = 2
+        for s in range(2, 7):
+            if 20 * 4 ** s * 4 <= target:
+                subdiv = s
+        n_meshes = int(min(64, max(1, round(target / (20 * 4 ** subdiv)))))
+        n_inst = max(1, int(round(12.5 * n_meshes)))
+        n_emit = max(1, n_inst // 25)
+        box = 100.0 * (n_inst / 800.0) ** (1.0 / 3.0)
+        scene, camf = icosphere_field(n_meshes=n_meshes, subdiv=subdiv, n_instances=n_inst, n_emissive=n_emit,
+                                      box=box, seed=100 + v, threads=threads)
+        w, h = resolutions[int(u_res[v]) % len(resolutions)]
+        views.append((scene, camf(w, h), {"unique_triangles": n_meshes * 20 * 4 ** subdiv, "subdiv": subdiv,
+                                          "meshes": n_meshes, "instances": n_inst, "width": w, "height": h}))
+    return views
