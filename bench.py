@@ -233,6 +233,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--loop", type=int, default=1, help="traversal loop variant (0 nested, 1 single loop)")
+    ap.add_argument("--persistent", type=int, default=None, help="persistent traversal waves per CU (0: off)")
+    ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a persistent refill")
+    ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -267,6 +270,12 @@ def main() -> int:
 
     dev = Device(0)
     dev.set_option(N.OPT_TRAVERSAL_LOOP, args.loop)
+    if args.persistent is not None:
+        dev.set_option(N.OPT_PERSISTENT_WAVES, args.persistent)
+    if args.refill is not None:
+        dev.set_option(N.OPT_REFILL_LANES, args.refill)
+    if args.gather is not None:
+        dev.set_option(N.OPT_GATHER_LANES, args.gather)
     t0 = time.perf_counter()
     dev.upload_scene(scene)
     dev.sync()
@@ -331,6 +340,9 @@ def main() -> int:
         "host_build_s": round(host_build_s, 2),
         "upload_s": round(upload_s, 2),
         "traversal_loop": args.loop,
+        "persistent_waves_per_cu": dev.get_option(N.OPT_PERSISTENT_WAVES),
+        "refill_lanes": dev.get_option(N.OPT_REFILL_LANES),
+        "gather_lanes": dev.get_option(N.OPT_GATHER_LANES),
         "stack_entries": dev.get_option(N.OPT_STACK_ENTRIES),
         "fast_division": dev.get_option(N.OPT_FAST_DIVISION),
         "image_mean": float(np.mean(img[..., :3])),

@@ -311,6 +311,15 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 #define PULSE_OPT_STACK_ENTRIES 2  /* read-only: LDS traversal stack entries in use  */
 #define PULSE_OPT_FAST_DIVISION 3  /* read-only: 1 if the scene's coordinates allow   */
                                    /* the exact reciprocal-multiply slab division     */
+#define PULSE_OPT_PERSISTENT_WAVES 4 /* single loop: 0 = one ray per lane; n = a     */
+                                     /* persistent grid of n waves per CU whose lanes */
+                                     /* take new rays as theirs finish               */
+#define PULSE_OPT_REFILL_LANES 5   /* persistent: idle lanes that trigger a refill   */
+#define PULSE_OPT_GATHER_LANES 6   /* persistent: lanes waiting for an instance change */
+                                   /* before the wave runs it (1: no waiting)         */
+#define PULSE_OPT_LDS_STACK_ENTRIES 7 /* persistent: stack entries per lane kept in   */
+                                      /* LDS (0: all); deeper entries go to device    */
+                                      /* memory, so more waves fit per CU             */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

@@ -39,6 +39,10 @@ GRAD_PER_MATERIAL = 7
 OPT_TRAVERSAL_LOOP = 1
 OPT_STACK_ENTRIES = 2
 OPT_FAST_DIVISION = 3
+OPT_PERSISTENT_WAVES = 4
+OPT_REFILL_LANES = 5
+OPT_GATHER_LANES = 6
+OPT_LDS_STACK_ENTRIES = 7
 
 
 class PulseError(RuntimeError):

@@ -121,6 +121,14 @@ __device__ __forceinline__ V3T<S> transform_position(const float *m, const V3T<S
     mat_vec(m, p, 1.0f, h);
     return {h[0] / h[3], h[1] / h[3], h[2] / h[3]};
 }
+// The same for f32, skipping the divide when w == 1 (x / 1 == x exactly),
+// which is every affine transform applied to a finite point.
+__device__ __forceinline__ V3 transform_position(const float *m, const V3 &p) {
+    float h[4];
+    mat_vec(m, p, 1.0f, h);
+    if (h[3] == 1.0f) return {h[0], h[1], h[2]};
+    return {h[0] / h[3], h[1] / h[3], h[2] / h[3]};
+}
 template <class S>
 __device__ __forceinline__ V3T<S> transform_direction(const float *m, const V3T<S> &p) {
     S h[4];
@@ -199,14 +207,26 @@ __device__ __forceinline__ void mt(V3 o, V3 d, const float4 *tri, uint32_t slot,
     }
 }
 
-// Per-thread stack in LDS, entry i at base[i * stride] (consecutive lanes in
-// consecutive banks).
+// Per-thread traversal stack: entry i < lds_cap at base[i * stride] in LDS
+// (consecutive lanes in consecutive banks); deeper entries, when a kernel
+// keeps only the top of the stack in LDS to fit more waves per CU, at
+// spill[(i - lds_cap) * spill_stride] in global memory (one column per lane,
+// consecutive lanes adjacent).
 struct LdsStack {
     uint32_t *base;
     int stride;
-    int capacity;
-    __device__ __forceinline__ void put(int i, uint32_t v) { base[i * stride] = v; }
-    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+    int capacity;              // total entries
+    int lds_cap = 1 << 30;     // entries held in LDS
+    uint32_t *spill = nullptr;
+    uint32_t spill_stride = 0;
+    __device__ __forceinline__ void put(int i, uint32_t v) {
+        if (i < lds_cap) base[i * stride] = v;
+        else spill[static_cast<uint32_t>(i - lds_cap) * spill_stride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int i) const {
+        if (i < lds_cap) return base[i * stride];
+        return spill[static_cast<uint32_t>(i - lds_cap) * spill_stride];
+    }
 };
 
 // Slack of the object-space culling bound derived from a world-space bound:

@@ -70,55 +70,105 @@ __device__ __forceinline__ float slab_pre(const PreRay &p, float rec_t, float4 l
     return (tmax >= tmin && tmin < rec_t && tmax > 0.0f) ? tmin : kTFar;
 }
 
-// Closest hit (ANY_HIT = false): rec_* updated in place, returns false.
-// Occlusion (ANY_HIT = true): returns true iff the reference's closest-hit t
-// would be < threshold.  Two exact shortcuts are taken in occlusion mode: the
-// TLAS and each BLAS are culled at a padded threshold (nothing culled can
-// convert to a world t below the threshold), and a BLAS stops at the first
-// hit whose object t is below a bound that converts to a world t below the
-// threshold for certain (the instance's closest hit is then below it too).
+// Resumable form of the single-loop traversal: begin() sets a ray up, each
+// step() visits one node (or tests one triangle of a leaf, or enters/leaves
+// one instance), so a persistent kernel can hand a lane a new ray as soon as
+// its ray is done.
+//
+// Closest hit (ANY_HIT = false): rec_* hold the closest hit so far.
+// Occlusion (ANY_HIT = true): `occluded` becomes true iff the reference's
+// closest-hit t would be < threshold.  Two exact shortcuts are taken in
+// occlusion mode: the TLAS and each BLAS are culled at a padded threshold
+// (nothing culled can convert to a world t below the threshold), and a BLAS
+// stops at the first hit whose object t is below a bound that converts to a
+// world t below the threshold for certain (the instance's closest hit is
+// then below it too).
 template <bool ANY_HIT>
-__device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, float &rec_t, uint32_t &rec_inst,
-                                              uint32_t &rec_tri, float &rec_u, float &rec_v, float threshold,
-                                              LdsStack &stk) {
-    float4 n_lo = sc.tlas_nodes[0];
-    float4 n_hi = sc.tlas_nodes[1];
-    if (__float_as_uint(n_lo.w) == 0u && __float_as_uint(n_hi.w) == 0u) return false;
+struct Traversal {
+    V3 wo, wd;
+    PreRay world;
+    float threshold, slack_thr, world_cull;
+    PreRay cur;        // ray of the current level
+    float cur_t;       // culling bound of the current level
+    const float4 *nodes;
+    bool in_blas;
+    int sp, sp_base, iter_t, iter_b;
+    uint32_t leaf_next, leaf_end, inst, idx_off;
+    uint32_t tri_next, tri_end;  // triangles of the BLAS leaf in progress
+    bool want_advance;           // next: leave the BLAS and/or enter the next instance
+    float dlen, start_t, safe_t;
+    ObjHit h;
+    uint32_t node_first, node_count;  // current node: child pair / first primitive, primitive count
+    float rec_t, rec_u, rec_v;
+    uint32_t rec_inst, rec_tri;
+    bool occluded;
 
-    const PreRay world = prep_ray(wo, wd, sc.fast_div);
-    float world_cull = rec_t;
-    float slack_thr = 0.0f;
-    if (ANY_HIT) {
-        slack_thr = world_bound_slack(wo, threshold);
-        world_cull = threshold + slack_thr;
+    // false: nothing to traverse (empty TLAS); the result is already final.
+    __device__ __forceinline__ bool begin(const DevScene &sc, V3 o, V3 d, float t_max, float thr) {
+        rec_t = t_max;
+        rec_inst = 0;
+        rec_tri = 0;
+        rec_u = 0.0f;
+        rec_v = 0.0f;
+        occluded = false;
+        load_node(sc.tlas_nodes, 0u);
+        if (node_first == 0u && node_count == 0u) return false;
+        wo = o;
+        wd = d;
+        world = prep_ray(wo, wd, sc.fast_div);
+        threshold = thr;
+        world_cull = rec_t;
+        slack_thr = 0.0f;
+        if (ANY_HIT) {
+            slack_thr = world_bound_slack(wo, threshold);
+            world_cull = threshold + slack_thr;
+        }
+        cur = world;
+        cur_t = world_cull;
+        nodes = sc.tlas_nodes;
+        in_blas = false;
+        sp = 0;
+        sp_base = 0;
+        iter_t = 0;
+        iter_b = 0;
+        leaf_next = 0;
+        leaf_end = 0;
+        tri_next = 0;
+        tri_end = 0;
+        want_advance = false;
+        inst = 0;
+        idx_off = 0;
+        dlen = 1.0f;
+        start_t = kTFar;
+        safe_t = -1.0f;
+        h.t = kTFar;
+        h.found = false;
+        h.tri = 0;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        return true;
     }
 
-    PreRay cur = world;       // ray of the current level
-    float cur_t = world_cull; // culling bound of the current level
-    const float4 *nodes = sc.tlas_nodes;
-    bool in_blas = false;
-    int sp = 0, sp_base = 0;
-    int iter_t = 0, iter_b = 0;
-    uint32_t leaf_next = 0, leaf_end = 0;
-    // instance being traversed
-    uint32_t inst = 0, idx_off = 0;
-    float dlen = 1.0f, start_t = kTFar, safe_t = -1.0f;
-    ObjHit h;
-    h.t = kTFar;
-    h.found = false;
-    h.tri = 0;
-    h.u = 0.0f;
-    h.v = 0.0f;
-
-    for (;;) {
-        uint32_t first = __float_as_uint(n_lo.w);
-        uint32_t count = __float_as_uint(n_hi.w);
+    // One step; false when the traversal is complete.  A lane whose next
+    // operation is an instance change (leave a finished BLAS, enter the next
+    // instance of a TLAS leaf) waits while run_advance is false: the caller
+    // may gather such lanes so that the wave runs that long code path for
+    // many lanes at once.  Waiting does not change the ray's own sequence of
+    // operations.
+    __device__ __forceinline__ bool step(const DevScene &sc, LdsStack &stk, bool run_advance = true) {
+        if (want_advance) {
+            if (!run_advance) return true;
+            want_advance = false;
+            return advance(sc, stk);
+        }
+        const uint32_t first = node_first;
+        const uint32_t count = node_count;
         bool advance_instance = false;  // BLAS finished: convert and move on
         bool do_pop = false;
         if (!in_blas) {
             if (iter_t >= kMaxIterations) {
                 atomicOr(sc.error_flags, kErrIterationCap);
-                break;
+                return false;
             }
             iter_t += 1;
             if (count > 0u) {
@@ -128,17 +178,32 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
                 inst = 0xffffffffu;
             }
         } else {
-            if (iter_b >= kMaxIterations) {
-                atomicOr(sc.error_flags, kErrIterationCap);
-                advance_instance = true;
-            } else {
-                iter_b += 1;
-                if (count > 0u) {
-                    for (uint32_t i = 0; i < count; ++i) mt(cur.o, cur.d, sc.tri_bvh, idx_off + first + i, first + i, h);
-                    if (ANY_HIT && h.found && h.t < safe_t) return true;
-                    cur_t = h.t;
-                    do_pop = true;
+            // BLAS: a leaf's triangles are tested one per step, in order, so
+            // lanes at leaves and lanes at interior nodes share each step's cost
+            bool test_tri = tri_next < tri_end;
+            if (!test_tri) {
+                if (iter_b >= kMaxIterations) {
+                    atomicOr(sc.error_flags, kErrIterationCap);
+                    advance_instance = true;
+                } else {
+                    iter_b += 1;
+                    if (count > 0u) {
+                        tri_next = first;
+                        tri_end = first + count;
+                        test_tri = true;
+                    }
                 }
+            }
+            if (test_tri) {
+                mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
+                tri_next += 1;
+                if (tri_next < tri_end) return true;
+                if (ANY_HIT && h.found && h.t < safe_t) {
+                    occluded = true;
+                    return false;
+                }
+                cur_t = h.t;
+                do_pop = true;
             }
         }
 
@@ -158,8 +223,8 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
             if (da == kTFar) {
                 do_pop = true;
             } else {
-                n_lo = a_lo;
-                n_hi = a_hi;
+                node_first = __float_as_uint(a_lo.w);
+                node_count = __float_as_uint(a_hi.w);
                 if (db != kTFar) {
                     if (sp < stk.capacity) {
                         stk.put(sp, b);
@@ -168,24 +233,26 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
                         atomicOr(sc.error_flags, kErrStackOverflow);
                     }
                 }
-                continue;
+                return true;
             }
         }
 
         if (do_pop) {
             if (sp > sp_base) {
                 sp -= 1;
-                uint32_t nx = stk.get(sp);
-                n_lo = nodes[2 * nx];
-                n_hi = nodes[2 * nx + 1];
-                continue;
+                load_node(nodes, stk.get(sp));
+                return true;
             }
-            if (!in_blas) break;  // TLAS stack empty: done
+            if (!in_blas) return false;  // TLAS stack empty: done
             advance_instance = true;
         }
+        want_advance = true;
+        return true;
+    }
 
-        // advance_instance: finish the BLAS in progress (if any), then enter
-        // the next instance of the TLAS leaf or return to the TLAS.
+    // Finishes the BLAS in progress (if any), then enters the next instance
+    // of the TLAS leaf or returns to the TLAS.
+    __device__ __forceinline__ bool advance(const DevScene &sc, LdsStack &stk) {
         if (in_blas) {
             const pulse_mesh_instance &mi = sc.instances[inst];
             if (!h.found) h.t = kTFar;
@@ -195,7 +262,10 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
             V3 hit_world = transform_position(ow, add(cur.o, scale(h.t, cur.d)));
             float new_t = length(sub(hit_world, wo));
             if (ANY_HIT) {
-                if (new_t < threshold) return true;
+                if (new_t < threshold) {
+                    occluded = true;
+                    return false;
+                }
             } else if (new_t < rec_t) {
                 rec_t = new_t;
                 rec_inst = inst;
@@ -236,12 +306,13 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
             h.v = 0.0f;
             cur_t = h.t;
             nodes = sc.blas_nodes + 2 * node_off;
-            n_lo = nodes[0];
-            n_hi = nodes[1];
+            load_node(nodes, 0u);
             in_blas = true;
             sp_base = sp;
             iter_b = 0;
-            continue;
+            tri_next = 0;
+            tri_end = 0;
+            return true;
         }
         // back to the TLAS
         in_blas = false;
@@ -249,12 +320,39 @@ __device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, 
         cur_t = world_cull;
         nodes = sc.tlas_nodes;
         sp_base = 0;
-        if (sp == 0) break;
+        if (sp == 0) return false;
         sp -= 1;
-        uint32_t nx = stk.get(sp);
-        n_lo = nodes[2 * nx];
-        n_hi = nodes[2 * nx + 1];
+        load_node(nodes, stk.get(sp));
+        return true;
     }
+
+    // The child-pair index / first primitive and the primitive count of node
+    // i (words 3 and 7 of the 32-byte node; its box was tested by the parent).
+    __device__ __forceinline__ void load_node(const float4 *base, uint32_t i) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(base + 2 * i);
+        node_first = w[3];
+        node_count = w[7];
+    }
+};
+
+// Closest hit (ANY_HIT = false): rec_* updated in place, returns false.
+// Occlusion (ANY_HIT = true): returns true iff the reference's closest-hit t
+// would be < threshold (see Traversal).
+template <bool ANY_HIT>
+__device__ __forceinline__ bool trace_unified(const DevScene &sc, V3 wo, V3 wd, float &rec_t, uint32_t &rec_inst,
+                                              uint32_t &rec_tri, float &rec_u, float &rec_v, float threshold,
+                                              LdsStack &stk) {
+    Traversal<ANY_HIT> tr;
+    if (tr.begin(sc, wo, wd, rec_t, threshold)) {
+        while (tr.step(sc, stk)) {
+        }
+    }
+    if (ANY_HIT) return tr.occluded;
+    rec_t = tr.rec_t;
+    rec_inst = tr.rec_inst;
+    rec_tri = tr.rec_tri;
+    rec_u = tr.rec_u;
+    rec_v = tr.rec_v;
     return false;
 }
 

@@ -42,6 +42,8 @@ struct WfBuffers {
     // counters: ext[k] = rays traced at trace index k; sh[k] = shadow rays
     uint32_t *ext_count;  // [kMaxTraces + 1]
     uint32_t *sh_count;   // [kMaxTraces + 1]
+    uint32_t *ext_head;   // [kMaxTraces + 1] rays taken by the persistent extend launches
+    uint32_t *sh_head;    // [kMaxTraces + 1] rays taken by the persistent shadow launches
 };
 
 // generate: primary rays (path_tracer.wgsl:60-75) in 8x8 pixel tiles.
@@ -131,6 +133,85 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
     if (!occluded) {
         float4 c = wb.sh_c[i];
         wb.state_c[__float_as_uint(oa.w)] = c;
+    }
+}
+
+// Persistent extend (SHADOW = false, queue q at trace index k) or shadow
+// (SHADOW = true) over a fixed grid of one-wave workgroups.  A lane whose ray
+// is done takes the next one from the queue (one atomic per wave for all
+// idle lanes) once at least `refill` lanes of its wave are idle, so a wave
+// does not idle behind its longest ray.  Each ray is traversed exactly as by
+// k_extend / k_shadow (same steps, same results); only the assignment of
+// rays to lanes changes.
+//
+// Instance changes (the longest code path of a step) are gathered: a lane
+// whose next operation is one waits until at least `gather` lanes of the
+// wave are waiting for one, or no lane of the wave has anything else to do.
+//
+// The top LDS_ENTRIES entries of each lane's stack live in LDS, the rest (up
+// to `capacity`) in `spill`, one column of capacity - LDS_ENTRIES words per
+// lane of the grid.
+template <int LDS_ENTRIES, bool SHADOW>
+__global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill,
+                                                    uint32_t gather, int capacity, uint32_t *spill) {
+    __shared__ uint32_t smem[LDS_ENTRIES * 64];
+    LdsStack stk{smem + threadIdx.x, 64, capacity};
+    if (capacity > LDS_ENTRIES) {
+        stk.lds_cap = LDS_ENTRIES;
+        stk.spill = spill + blockIdx.x * 64u + threadIdx.x;
+        stk.spill_stride = gridDim.x * 64u;
+    }
+    const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
+    uint32_t *head = SHADOW ? &wb.sh_head[k] : &wb.ext_head[k];
+    const uint32_t lane = __lane_id();
+    Traversal<SHADOW> tr;
+    bool alive = false;
+    bool exhausted = false;  // wave-uniform: the queue has no rays left
+    uint32_t slot = 0, pixel = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!alive);
+        const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+        if (!exhausted && (n_idle >= refill || n_idle == 64u)) {
+            uint32_t base = 0;
+            if (lane == 0u) base = atomicAdd(head, n_idle);
+            base = __shfl(base, 0);
+            if (base + n_idle >= count) exhausted = true;
+            if (!alive) {
+                const uint32_t idx = base + static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
+                if (idx < count) {
+                    slot = idx;
+                    if (SHADOW) {
+                        float4 oa = wb.sh_o[idx], da = wb.sh_d[idx];
+                        pixel = __float_as_uint(oa.w);
+                        alive = tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, da.w);
+                        if (!alive) wb.state_c[pixel] = wb.sh_c[idx];  // empty scene: visible
+                    } else {
+                        float4 oa = wb.ray_o[q][idx], da = wb.ray_d[q][idx];
+                        alive = tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, 0.0f);
+                        if (!alive) {
+                            wb.hit[idx] = make_float4(kTFar, 0.0f, 0.0f, 0.0f);
+                            wb.hit_v[idx] = 0.0f;
+                        }
+                    }
+                }
+            }
+        }
+        const uint64_t act = __ballot(alive);
+        if (act == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        const uint64_t waiting = __ballot(alive && tr.want_advance);
+        const bool run_advance = waiting == act || static_cast<uint32_t>(__popcll(waiting)) >= gather;
+        if (alive && !tr.step(sc, stk, run_advance)) {
+            alive = false;
+            if (SHADOW) {
+                if (!tr.occluded) wb.state_c[pixel] = wb.sh_c[slot];
+            } else {
+                wb.hit[slot] = make_float4(tr.rec_t, __uint_as_float(tr.rec_inst), __uint_as_float(tr.rec_tri), tr.rec_u);
+                wb.hit_v[slot] = tr.rec_v;
+            }
+        }
     }
 }
 

@@ -48,6 +48,10 @@ enum SceneBuf {
     B_CDFS, B_AREAS, B_LIGHTS, B_COUNT
 };
 
+// Queue counter words of the single-view path: extension and shadow queue
+// lengths, then the persistent launches' taken-ray heads.
+constexpr int kCounterWords = 4 * (pt::kMaxTraces + 1);
+
 enum WfBuf {
     W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_HIT_V, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C,
     W_COUNTERS, W_COUNT
@@ -91,6 +95,11 @@ struct pulse_ctx {
     uint64_t generation = 0;
     uint32_t stack_entries = 0;  // traversal stack capacity in use
     int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
+    int persistent = 24;         // persistent traversal waves per CU (0: one ray per lane)
+    int refill = 16;             // idle lanes that make a persistent wave take new rays
+    int gather = 1;              // persistent: lanes gathered before an instance change runs
+    int lds_entries = 24;        // persistent: stack entries kept in LDS (0: all); the rest spill
+    pulse_host::DevBuf spill;    // persistent: spilled stack entries
     pulse_host::DevBuf loss_acc; // f64 accumulator of pulse_image_loss
     pulse_stats stats{};
     hipEvent_t ev[3 * pt::kMaxTraces + 8] = {};

@@ -102,6 +102,61 @@ void launch_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ, ui
     hipLaunchKernelGGL((k_shadow_rays<kTraceBlock, STACK, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
                        dev_scene(ctx), rays, occ, n);
 }
+// Persistent grid: `persistent` waves per CU (never more than the queue
+// could need).
+uint32_t persistent_grid(pulse_ctx *ctx, uint32_t cap) {
+    uint32_t g = static_cast<uint32_t>(ctx->num_cus) * static_cast<uint32_t>(ctx->persistent);
+    return std::max<uint32_t>(1, std::min<uint32_t>(g, (cap + kTraceBlock - 1) / kTraceBlock));
+}
+// LDS entries of the persistent kernels' stacks: the compiled size at most
+// the requested one (and at most the stack's capacity).
+constexpr int kLdsSizes[] = {8, 16, 24, 32, 40, 48, 64, 128};
+int persistent_lds(pulse_ctx *ctx) {
+    int want = ctx->lds_entries > 0 ? std::min<int>(ctx->lds_entries, ctx->stack_entries) : ctx->stack_entries;
+    int got = kLdsSizes[0];
+    for (int s : kLdsSizes)
+        if (s <= want) got = s;
+    return got;
+}
+template <int LDS>
+void launch_queue_lds(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k, uint32_t *spill) {
+    const int capacity = static_cast<int>(ctx->stack_entries);
+    if (shadow)
+        hipLaunchKernelGGL((k_trace_queue<LDS, true>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx), wb,
+                           q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather), capacity, spill);
+    else
+        hipLaunchKernelGGL((k_trace_queue<LDS, false>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx),
+                           wb, q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather), capacity,
+                           spill);
+}
+// Spill area of the persistent launches over queues of up to `cap` rays:
+// the stack entries below the LDS part, for every lane of the grid.  Grows
+// only after the stream has drained (no launch in flight uses it).
+int ensure_spill(pulse_ctx *ctx, uint32_t cap) {
+    const int lds = persistent_lds(ctx);
+    if (static_cast<int>(ctx->stack_entries) <= lds) return PULSE_OK;
+    size_t bytes = sizeof(uint32_t) * persistent_grid(ctx, cap) * kTraceBlock * (ctx->stack_entries - lds);
+    if (bytes <= ctx->spill.cap && ctx->spill.ptr) return PULSE_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ctx->spill.ensure(bytes);
+}
+// Persistent extend/shadow (ensure_spill first).
+int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow, int q, int k) {
+    const uint32_t grid = persistent_grid(ctx, cap);
+    const int lds = persistent_lds(ctx);
+    uint32_t *spill = static_cast<int>(ctx->stack_entries) > lds ? ctx->spill.as<uint32_t>() : nullptr;
+    switch (lds) {
+        case 8: launch_queue_lds<8>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 16: launch_queue_lds<16>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 24: launch_queue_lds<24>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 32: launch_queue_lds<32>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 40: launch_queue_lds<40>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 48: launch_queue_lds<48>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 64: launch_queue_lds<64>(ctx, wb, grid, shadow, q, k, spill); break;
+        default: launch_queue_lds<128>(ctx, wb, grid, shadow, q, k, spill); break;
+    }
+    return PULSE_OK;
+}
 template <int STACK, int LOOP>
 void launch_extend(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int q, int k) {
     uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
@@ -144,7 +199,7 @@ int ensure_wavefront(pulse_ctx *ctx, size_t npix) {
     for (int b : {W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C})
         if ((rc = ctx->wf[b].ensure(f4)) != PULSE_OK) return rc;
     if ((rc = ctx->wf[W_HIT_V].ensure(sizeof(float) * npix)) != PULSE_OK) return rc;
-    if ((rc = ctx->wf[W_COUNTERS].ensure(sizeof(uint32_t) * 2 * (kMaxTraces + 1))) != PULSE_OK) return rc;
+    if ((rc = ctx->wf[W_COUNTERS].ensure(sizeof(uint32_t) * kCounterWords)) != PULSE_OK) return rc;
     ctx->wf_pixels = npix;
     return PULSE_OK;
 }
@@ -164,6 +219,8 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
     wb.sh_c = ctx->wf[W_SH_C].as<float4>();
     wb.ext_count = ctx->wf[W_COUNTERS].as<uint32_t>();
     wb.sh_count = wb.ext_count + (kMaxTraces + 1);
+    wb.ext_head = wb.ext_count + 2 * (kMaxTraces + 1);
+    wb.sh_head = wb.ext_count + 3 * (kMaxTraces + 1);
     return wb;
 }
 
@@ -405,6 +462,7 @@ int pulse_destroy(pulse_ctx *ctx) {
     for (auto &b : ctx->wf) b.release();
     ctx->errors.release();
     ctx->loss_acc.release();
+    ctx->spill.release();
     if (ctx->events_created)
         for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -578,25 +636,35 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
     const uint32_t shade_grid = (cap + kShadeBlock - 1) / kShadeBlock;
 
+    const bool persistent = ctx->loop_variant == 1 && ctx->persistent > 0;
+    if (persistent && (rc = ensure_spill(ctx, cap)) != PULSE_OK) return rc;
     ctx->stats = pulse_stats{};
     double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0;
     uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
     std::vector<uint32_t> counts(2 * (kMaxTraces + 1));
     for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
         fa.accumulation_count = prm->accumulation_count + s;
-        HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * 2 * (kMaxTraces + 1), ctx->stream));
+        HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * kCounterWords, ctx->stream));
         int e = 0;
         if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
         hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, ctx->stream, wb, fa);
         int q = 0;
         for (uint32_t k = 0; k < traces; ++k) {
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
-            PULSE_STACK_DISPATCH(launch_extend, ctx, wb, cap, q, static_cast<int>(k));
+            if (persistent) {
+                if ((rc = launch_queue(ctx, wb, cap, false, q, static_cast<int>(k))) != PULSE_OK) return rc;
+            } else {
+                PULSE_STACK_DISPATCH(launch_extend, ctx, wb, cap, q, static_cast<int>(k));
+            }
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
             hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, q,
                                static_cast<int>(k));
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
-            PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, cap, static_cast<int>(k));
+            if (persistent) {
+                if ((rc = launch_queue(ctx, wb, cap, true, 0, static_cast<int>(k))) != PULSE_OK) return rc;
+            } else {
+                PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, cap, static_cast<int>(k));
+            }
             q ^= 1;
         }
         if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
@@ -732,6 +800,22 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (ctx->has_scene()) ctx->stack_entries = st;
             return PULSE_OK;
         }
+        case PULSE_OPT_PERSISTENT_WAVES:
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "persistent waves per CU must be 0..64");
+            ctx->persistent = static_cast<int>(value);
+            return PULSE_OK;
+        case PULSE_OPT_REFILL_LANES:
+            if (value < 1 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "refill lanes must be 1..64");
+            ctx->refill = static_cast<int>(value);
+            return PULSE_OK;
+        case PULSE_OPT_LDS_STACK_ENTRIES:
+            if (value < 0 || value > 128) PULSE_FAIL(PULSE_E_INVALID_ARG, "LDS stack entries must be 0..128");
+            ctx->lds_entries = static_cast<int>(value);
+            return PULSE_OK;
+        case PULSE_OPT_GATHER_LANES:
+            if (value < 1 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "gather lanes must be 1..64");
+            ctx->gather = static_cast<int>(value);
+            return PULSE_OK;
         default:
             PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
     }
@@ -741,6 +825,10 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
     if (!ctx || !value) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: null argument");
     switch (option) {
         case PULSE_OPT_TRAVERSAL_LOOP: *value = ctx->loop_variant; return PULSE_OK;
+        case PULSE_OPT_PERSISTENT_WAVES: *value = ctx->persistent; return PULSE_OK;
+        case PULSE_OPT_REFILL_LANES: *value = ctx->refill; return PULSE_OK;
+        case PULSE_OPT_GATHER_LANES: *value = ctx->gather; return PULSE_OK;
+        case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
         case PULSE_OPT_FAST_DIVISION: *value = ctx->cur->fast_div ? 1 : 0; return PULSE_OK;
         default: PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: unknown option %d", option);

@@ -15,6 +15,8 @@ from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
 
 pytestmark = pytest.mark.gpu
 
+DEFAULT_PERSISTENT, DEFAULT_REFILL, DEFAULT_GATHER, DEFAULT_LDS = 24, 16, 1, 24
+
 MODES = [(N.MODE_REF_GGX, N.NEE_LAMBERT, 5), (N.MODE_REF_NEE_LAMBERT, N.NEE_LAMBERT, 5),
          (N.MODE_GGX_NEE, N.NEE_LAMBERT, 4), (N.MODE_GGX_NEE, N.NEE_GGX, 4)]
 
@@ -197,6 +199,37 @@ def test_traversal_variants_bit_exact(dev, field, loop):
         assert same_bits(node.readback(), oi).all()
     finally:
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
+@pytest.mark.parametrize("waves,refill,gather,lds", [(0, 16, 1, 0), (4, 1, 1, 0), (16, 16, 1, 0), (16, 64, 8, 24),
+                                                      (64, 32, 64, 16), (16, 8, 16, 8), (8, 16, 32, 32),
+                                                      (24, 16, 1, 24)])
+def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds):
+    """Persistent extend/shadow launches (lanes take new rays as theirs
+    finish; instance changes gathered across lanes; the deep part of the
+    stack in device memory) give the oracle's image for every grid size,
+    refill and gather threshold and LDS stack size, in all three modes."""
+    scene, camf = field
+    dev.upload_scene(scene)
+    dev.set_option(N.OPT_PERSISTENT_WAVES, waves)
+    dev.set_option(N.OPT_REFILL_LANES, refill)
+    dev.set_option(N.OPT_GATHER_LANES, gather)
+    dev.set_option(N.OPT_LDS_STACK_ENTRIES, lds)
+    try:
+        assert dev.get_option(N.OPT_PERSISTENT_WAVES) == waves
+        cam = camf(96, 56)
+        orc = OracleScene(scene)
+        for mode, rr in ((N.MODE_GGX_NEE, 1), (N.MODE_REF_GGX, 0), (N.MODE_REF_NEE_LAMBERT, 2)):
+            prm = make_params(spp=2, max_depth=4, mode=mode, rr_start_depth=rr)
+            node = PulsePathTracerNode(dev, cam.width, cam.height)
+            node.run(cam, prm)
+            oi, _, _ = orc.render(cam, prm)
+            assert same_bits(node.readback(), oi).all(), (mode, waves, refill, gather)
+    finally:
+        dev.set_option(N.OPT_PERSISTENT_WAVES, DEFAULT_PERSISTENT)
+        dev.set_option(N.OPT_REFILL_LANES, DEFAULT_REFILL)
+        dev.set_option(N.OPT_GATHER_LANES, DEFAULT_GATHER)
+        dev.set_option(N.OPT_LDS_STACK_ENTRIES, DEFAULT_LDS)
 
 
 @pytest.mark.parametrize("mode", [N.MODE_REF_GGX, N.MODE_GGX_NEE])
