@@ -16,9 +16,14 @@ Prints one JSON line.  --gpus N > 1 is not supported on this build.
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -144,6 +149,77 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
                       "65 materials x 7 parameters"}
 
 
+def _pmc_run(counter: str, extra_args, timeout_s: float = 150.0):
+    """Runs this script under `rocprofv3 --pmc <counter>` (a child process,
+    its own process group, killed at the time limit) and returns the rows of
+    the counter CSV in dispatch order, or None."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    out = tempfile.mkdtemp(prefix="pulse_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, str(Path(__file__).resolve())] + list(extra_args)
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                             start_new_session=True)
+        try:
+            p.wait(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            return None
+        if p.returncode != 0:
+            return None
+        files = list(Path(out).rglob("*counter_collection.csv"))
+        if not files:
+            return None
+        rows = list(csv.DictReader(open(files[0])))
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return rows
+    except OSError:
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def pmc_traffic(max_depth: int) -> dict | None:
+    """HBM bytes per dispatch of the traversal launches, from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch, one counter per run) over one frame of this
+    workload, each scaled by what the same counter reads for a plain 256 MiB
+    device copy (bytes moved / counter reading).  Dispatches are told apart
+    by their order within the frame: extend (trace index 0 = primary, then
+    bounces) and shadow launches alternate with shading."""
+    frame_args = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-adjoint", "--no-batch", "--no-pmc",
+                  "--max-depth", str(max_depth)]
+    scale = {}
+    per_kind = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = _pmc_run(counter, ["--copy-calibration", "256"])
+        if not cal:
+            return None
+        copies = [float(r["Counter_Value"]) for r in cal if "copy" in r["Kernel_Name"].lower()]
+        if not copies:
+            return None
+        scale[counter] = (256 * 1024) / (sum(copies) / len(copies))  # KiB moved per KiB counted
+        rows = _pmc_run(counter, frame_args)
+        if not rows:
+            return None
+        ext = [float(r["Counter_Value"]) for r in rows if "k_trace_queue" in r["Kernel_Name"] and "false" in r["Kernel_Name"]]
+        sh = [float(r["Counter_Value"]) for r in rows if "k_trace_queue" in r["Kernel_Name"] and "true" in r["Kernel_Name"]]
+        if not ext or not sh:
+            return None
+        # the measured frame is the last max_depth extend / shadow dispatches
+        ext, sh = ext[-max_depth:], sh[-max_depth:]
+        per_kind.setdefault("extend_primary", {})[counter] = ext[0]
+        per_kind.setdefault("extend_bounce", {})[counter] = sum(ext[1:]) / max(len(ext) - 1, 1)
+        per_kind.setdefault("shadow", {})[counter] = sum(sh) / len(sh)
+    out = {"scale": {k: round(v, 3) for k, v in scale.items()}}
+    for kind, v in per_kind.items():
+        out[kind] = (v["FETCH_SIZE"] * scale["FETCH_SIZE"] + v["WRITE_SIZE"] * scale["WRITE_SIZE"]) * 1024.0
+    return out
+
+
 def bench_batch(frames: int = 2, check_views: int = 4) -> dict:
     """Batched form: 32 views, resolutions from {512^2, 720p, 1080p}, icosphere
     fields of 10k-2M unique triangles (log-uniform), 2 spp, 4 bounces, GGX +
@@ -238,6 +314,7 @@ def main() -> int:
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -364,18 +441,34 @@ def main() -> int:
                "bounce": bytes_per_ray(work["bounce"], B_RAY_EXTEND),
                "shadow": bytes_per_ray(work["shadow"], B_RAY_SHADOW)}
         # dominant kernel by total stage time
+        persistent = result["persistent_waves_per_cu"] > 0 and args.loop == 1
+        ext_name = "k_trace_queue<..., false>" if persistent else "k_extend"
+        sh_name = "k_trace_queue<..., true>" if persistent else "k_shadow"
         cands = {
-            "k_extend (depth>=1)": (agg["ms_extend_bounce"], agg["extension_rays"] - agg["primary_rays"], bpr["bounce"],
-                                    agg["extend_launches"] - args.steps),
-            "k_shadow": (agg["ms_shadow"], agg["shadow_rays"], bpr["shadow"], agg["shadow_launches"]),
-            "k_extend (depth 0)": (agg["ms_primary_extend"], agg["primary_rays"], bpr["primary"], args.steps),
+            f"{ext_name} (extension rays, depth >= 1)": (
+                agg["ms_extend_bounce"], agg["extension_rays"] - agg["primary_rays"], bpr["bounce"],
+                agg["extend_launches"] - args.steps, "extend_bounce"),
+            f"{sh_name} (shadow rays)": (agg["ms_shadow"], agg["shadow_rays"], bpr["shadow"], agg["shadow_launches"],
+                                         "shadow"),
+            f"{ext_name} (primary rays)": (agg["ms_primary_extend"], agg["primary_rays"], bpr["primary"], args.steps,
+                                           "extend_primary"),
         }
-        name, (ms, rays, b, launches) = max(cands.items(), key=lambda kv: kv[1][0])
+        name, (ms, rays, b, launches, kind) = max(cands.items(), key=lambda kv: kv[1][0])
         achieved = rays * b / (ms / 1e3) / 1e9
+        traffic = None
+        pmc = None
+        if not args.no_pmc and persistent:
+            pmc = pmc_traffic(args.max_depth)
+            if pmc is not None:
+                traffic = round(pmc[kind])
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
                               "avg_launch_ms": round(ms / max(launches, 1), 4),
+                              "algorithmic_bytes_per_launch": round(rays * b / max(launches, 1)),
                               "bytes_per_ray": {k: round(v, 1) for k, v in bpr.items()}}
+        if pmc is not None:
+            result["roofline"]["hbm_bytes_per_launch"] = {k: round(v) for k, v in pmc.items() if k != "scale"}
+            result["roofline"]["pmc_scale"] = pmc["scale"]
     print(json.dumps(result))
     dev.close()
     return 0

@@ -342,13 +342,18 @@ typedef struct {
 typedef struct {
     uint32_t persistent_waves_per_cu; /* 0: one wave per chunk of rays; n: a fixed */
                                       /* grid of n waves per CU drains the queues   */
-    uint32_t chunk_rays;              /* rays per work item: 64, 128, 256 or 512    */
+    uint32_t chunk_rays;              /* rays per work item: a power of two, 64..4096 */
     uint32_t stack_entries;           /* LDS traversal stack (0: smallest that fits) */
     uint32_t traversal_loop;          /* as PULSE_OPT_TRAVERSAL_LOOP                */
     uint32_t regroup_every;           /* compact the extension queue every n bounces */
                                       /* (in between, rays stay in place)           */
     uint32_t tile_width;              /* primary-ray tile: width x (64 / width)      */
     uint32_t use_graph;               /* 1: replay one captured hipGraph per frame   */
+    uint32_t lds_stack_entries;       /* persistent single loop: stack entries per   */
+                                      /* lane in LDS, the rest in device memory     */
+                                      /* (0: all in LDS)                            */
+    uint32_t refill_lanes;            /* persistent single loop: idle lanes of a     */
+                                      /* wave that take the next rays of its chunk  */
     uint32_t _reserved;
 } pulse_batch_config;
 

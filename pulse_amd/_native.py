@@ -208,6 +208,8 @@ class BatchConfig(C.Structure):
         ("regroup_every", C.c_uint32),
         ("tile_width", C.c_uint32),
         ("use_graph", C.c_uint32),
+        ("lds_stack_entries", C.c_uint32),
+        ("refill_lanes", C.c_uint32),
         ("_reserved", C.c_uint32),
     ]
 

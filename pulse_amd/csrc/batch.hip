@@ -28,7 +28,7 @@ struct pulse_batch {
     uint32_t max_chunks = 0;    // sum over views of ceil(npix / chunk)
     std::vector<BatchView> views;
     std::vector<pulse_scene *> scenes;
-    DevBuf d_views, counts, work, totals, frame;
+    DevBuf d_views, counts, work, totals, frame, spill;
     DevBuf slot_buf[11];
     pulse_batch_config cfg{};
     uint32_t stack_entries = 0;  // resolved from cfg
@@ -57,8 +57,13 @@ uint32_t resolve_stack(const pulse_batch *b, const pulse_batch_config &c) {
 }
 
 int check_config(const pulse_batch *b, const pulse_batch_config &c) {
-    if (c.chunk_rays != 64 && c.chunk_rays != 128 && c.chunk_rays != 256 && c.chunk_rays != 512)
-        PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: chunk_rays must be 64, 128, 256 or 512 (got %u)", c.chunk_rays);
+    if (c.chunk_rays < 64 || c.chunk_rays > 4096 || (c.chunk_rays & (c.chunk_rays - 1)) != 0)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: chunk_rays must be a power of two in 64..4096 (got %u)",
+                   c.chunk_rays);
+    if (c.refill_lanes < 1 || c.refill_lanes > 64)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: refill_lanes in 1..64 (got %u)", c.refill_lanes);
+    if (c.lds_stack_entries > 128)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: lds_stack_entries in 0..128 (got %u)", c.lds_stack_entries);
     if (c.persistent_waves_per_cu > 64)
         PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: persistent_waves_per_cu above 64 (got %u)", c.persistent_waves_per_cu);
     if (c.traversal_loop > 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: traversal_loop must be 0 or 1");
@@ -107,6 +112,9 @@ BatchArgs batch_args(const pulse_batch *b) {
     a.horizon_clamp = (b->prm.flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     a.chunk = b->cfg.chunk_rays;
     a.persistent = b->cfg.persistent_waves_per_cu;
+    a.refill = b->cfg.refill_lanes;
+    a.stack_cap = static_cast<int>(b->stack_entries);
+    a.spill = b->spill.as<uint32_t>();
     a.regroup = b->cfg.regroup_every;
     a.tile_w = b->cfg.tile_width;
     return a;
@@ -131,19 +139,56 @@ int layout_views(pulse_batch *b) {
     return PULSE_OK;
 }
 
+uint32_t trace_grid(const pulse_batch *b);
 template <int STACK, int LOOP>
 void launch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
-    uint32_t grid = b->max_chunks;
-    if (b->cfg.persistent_waves_per_cu)
-        grid = std::min<uint32_t>(grid, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
-    grid = std::max<uint32_t>(grid, 1);
+    const uint32_t grid = trace_grid(b);
     if (shadow)
         hipLaunchKernelGGL((k_b_trace<STACK, LOOP, true>), dim3(grid), dim3(64), 0, b->ctx->stream, a, q, k);
     else
         hipLaunchKernelGGL((k_b_trace<STACK, LOOP, false>), dim3(grid), dim3(64), 0, b->ctx->stream, a, q, k);
 }
 
+// Persistent single-loop traversal with per-lane refill: the compiled LDS
+// stack size at most the requested one (0: the whole stack).
+constexpr int kLdsSizes[] = {8, 16, 24, 32, 40, 48, 64, 128};
+bool refill_path(const pulse_batch *b) { return b->cfg.persistent_waves_per_cu > 0 && b->cfg.traversal_loop == 1; }
+int refill_lds(const pulse_batch *b) {
+    int want = b->cfg.lds_stack_entries > 0 ? std::min<int>(b->cfg.lds_stack_entries, b->stack_entries)
+                                            : static_cast<int>(b->stack_entries);
+    int got = kLdsSizes[0];
+    for (int s : kLdsSizes)
+        if (s <= want) got = s;
+    return got;
+}
+uint32_t trace_grid(const pulse_batch *b) {
+    uint32_t grid = b->max_chunks;
+    if (b->cfg.persistent_waves_per_cu)
+        grid = std::min<uint32_t>(grid, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
+    return std::max<uint32_t>(grid, 1);
+}
+template <int LDS>
+void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+    if (shadow)
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, true>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q, k);
+    else
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, false>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q, k);
+}
+
 void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+    if (refill_path(b)) {
+        switch (refill_lds(b)) {
+            case 8: launch_refill<8>(b, a, shadow, q, k); break;
+            case 16: launch_refill<16>(b, a, shadow, q, k); break;
+            case 24: launch_refill<24>(b, a, shadow, q, k); break;
+            case 32: launch_refill<32>(b, a, shadow, q, k); break;
+            case 40: launch_refill<40>(b, a, shadow, q, k); break;
+            case 48: launch_refill<48>(b, a, shadow, q, k); break;
+            case 64: launch_refill<64>(b, a, shadow, q, k); break;
+            default: launch_refill<128>(b, a, shadow, q, k); break;
+        }
+        return;
+    }
     if (b->cfg.traversal_loop == 0) {
         switch (b->stack_entries) {
             case 32: launch_trace<32, 0>(b, a, shadow, q, k); break;
@@ -212,7 +257,13 @@ int apply_config(pulse_batch *b, const pulse_batch_config &c) {
     b->cfg = c;
     b->stack_entries = resolve_stack(b, c);
     drop_graph(b);
-    return layout_views(b);
+    if ((rc = layout_views(b)) != PULSE_OK) return rc;  // (drains the stream)
+    if (refill_path(b) && static_cast<int>(b->stack_entries) > refill_lds(b)) {
+        size_t bytes = sizeof(uint32_t) * static_cast<size_t>(trace_grid(b)) * 64u *
+                       (b->stack_entries - static_cast<uint32_t>(refill_lds(b)));
+        if ((rc = b->spill.ensure(bytes)) != PULSE_OK) return rc;
+    }
+    return PULSE_OK;
 }
 
 int render(pulse_batch *b, uint32_t acc) {
@@ -236,13 +287,15 @@ extern "C" {
 int pulse_batch_default_config(pulse_batch_config *out) {
     if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_batch_default_config: null out");
     *out = pulse_batch_config{};
-    out->persistent_waves_per_cu = 0;
-    out->chunk_rays = 64;
+    out->persistent_waves_per_cu = 24;
+    out->chunk_rays = 256;
     out->stack_entries = 0;
     out->traversal_loop = 1;
     out->regroup_every = 1;
     out->tile_width = 8;
     out->use_graph = 1;
+    out->lds_stack_entries = 24;
+    out->refill_lanes = 16;
     return PULSE_OK;
 }
 
@@ -327,6 +380,7 @@ int pulse_batch_destroy(pulse_batch *b) {
     b->work.release();
     b->totals.release();
     b->frame.release();
+    b->spill.release();
     delete b;
     return PULSE_OK;
 }
@@ -420,7 +474,7 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
         for (uint32_t s : kStacks)
             if (s > pick_stack(b->stack_need, 1) && stack_opts.size() < 2) stack_opts.push_back(s);
         for (int pass = 0; pass < 2 && rc == PULSE_OK; ++pass) {
-            for (int knob = 0; knob < 6 && rc == PULSE_OK; ++knob) {
+            for (int knob = 0; knob < 8 && rc == PULSE_OK; ++knob) {
                 std::vector<pulse_batch_config> cands;
                 auto add = [&](auto set) {
                     pulse_batch_config c = best;
@@ -429,10 +483,17 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
                 };
                 switch (knob) {
                     case 0:
-                        for (uint32_t w : {0u, 4u, 8u, 16u}) add([&](pulse_batch_config &c) { c.persistent_waves_per_cu = w; });
+                        for (uint32_t w : {0u, 16u, 24u, 32u})
+                            add([&](pulse_batch_config &c) { c.persistent_waves_per_cu = w; });
                         break;
                     case 1:
-                        for (uint32_t ch : {64u, 128u, 256u}) add([&](pulse_batch_config &c) { c.chunk_rays = ch; });
+                        for (uint32_t ch : {64u, 256u, 1024u}) add([&](pulse_batch_config &c) { c.chunk_rays = ch; });
+                        break;
+                    case 6:
+                        for (uint32_t l : {16u, 24u, 32u}) add([&](pulse_batch_config &c) { c.lds_stack_entries = l; });
+                        break;
+                    case 7:
+                        for (uint32_t r : {8u, 16u, 32u}) add([&](pulse_batch_config &c) { c.refill_lanes = r; });
                         break;
                     case 2:
                         for (uint32_t r : {1u, 2u}) add([&](pulse_batch_config &c) { c.regroup_every = r; });

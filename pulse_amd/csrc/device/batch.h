@@ -67,6 +67,9 @@ struct BatchArgs {
     uint32_t mode, nee_ggx, max_traces, rr_start, horizon_clamp;
     uint32_t chunk;        // rays per chunk, a multiple of 64
     uint32_t persistent;   // 0: one wave per chunk; else a fixed grid drains the queue
+    uint32_t refill;       // persistent single-loop traversal: idle lanes that take new rays
+    int stack_cap;         // traversal stack entries per lane
+    uint32_t *spill;       // stack entries below the LDS part, one column per lane of the grid
     uint32_t regroup;      // compact the extension queue of index k when k % regroup == 0
     uint32_t tile_w;       // generate tile width; a tile is tile_w x (64 / tile_w) pixels
 };
@@ -215,6 +218,82 @@ __global__ void __launch_bounds__(64) k_b_trace(BatchArgs b, int q, int k) {
                                            stk);
                 b.hit[slot] = make_float4(t, __uint_as_float(inst), __uint_as_float(tri), u);
                 b.hit_v[slot] = vv;
+            }
+        }
+    }
+}
+
+// Persistent extend / shadow with per-lane refill: a wave takes a chunk of
+// one view's queue (so the scene is wave-uniform), its lanes take the
+// chunk's rays one by one as theirs finish (once at least `refill` lanes are
+// idle), and the wave takes the next chunk when the current one is drained.
+// The top LDS entries of each lane's stack are in LDS, the rest in `spill`.
+// Each ray is traversed exactly as by k_b_trace.
+template <int LDS, bool SHADOW>
+__global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k) {
+    __shared__ uint32_t smem[LDS * 64];
+    const uint32_t ci = SHADOW ? b.max_traces + 1u + static_cast<uint32_t>(k) : static_cast<uint32_t>(k);
+    const uint32_t wslot = 2u * static_cast<uint32_t>(k) + (SHADOW ? 1u : 0u);
+    const ChunkMap cm = chunk_map(b, ci);
+    LdsStack stk{smem + threadIdx.x, 64, b.stack_cap};
+    if (b.stack_cap > LDS) {
+        stk.lds_cap = LDS;
+        stk.spill = b.spill + blockIdx.x * 64u + threadIdx.x;
+        stk.spill_stride = gridDim.x * 64u;
+    }
+    const uint32_t lane = __lane_id();
+    Traversal<SHADOW> tr;
+    bool alive = false;
+    uint32_t slot = 0, pixel = 0;
+    // wave-uniform: the chunk being drained
+    uint32_t base = 0, next = 0, end = 0;
+    DevScene sc = b.views[0].sc;
+    for (;;) {
+        const uint64_t idle = __ballot(!alive);
+        const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+        if (next < end && (n_idle >= b.refill || n_idle == 64u)) {
+            const uint32_t idx = next + static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
+            if (!alive && idx < end) {
+                slot = base + idx;
+                if (SHADOW) {
+                    float4 oa = b.sh_o[slot], da = b.sh_d[slot];
+                    pixel = __float_as_uint(oa.w);
+                    alive = tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, da.w);
+                    if (!alive) b.state_c[base + pixel] = b.sh_c[slot];  // empty scene: visible
+                } else {
+                    float4 oa = b.ray_o[q][slot], da = b.ray_d[q][slot];
+                    alive = __float_as_uint(oa.w) != kDeadRay &&
+                            tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, 0.0f);
+                    if (!alive) {
+                        b.hit[slot] = make_float4(kTFar, 0.0f, 0.0f, 0.0f);
+                        b.hit_v[slot] = 0.0f;
+                    }
+                }
+            }
+            next = min(next + n_idle, end);
+        }
+        if (__ballot(alive) == 0ull) {
+            if (next < end) continue;
+            uint32_t g = 0;
+            if (lane == 0u) g = atomicAdd(&b.work[wslot], 1u);
+            g = uniform_u32(__shfl(g, 0));
+            if (g >= cm.total) break;
+            uint32_t start;
+            const uint32_t v = chunk_view(cm, b.n_views, g, start);
+            sc = b.views[v].sc;
+            base = b.views[v].pix_base;
+            const uint32_t cnt = b.counts[v * b.cstride + ci];
+            next = (g - start) * b.chunk;
+            end = min(next + b.chunk, cnt);
+            continue;
+        }
+        if (alive && !tr.step(sc, stk)) {
+            alive = false;
+            if (SHADOW) {
+                if (!tr.occluded) b.state_c[base + pixel] = b.sh_c[slot];
+            } else {
+                b.hit[slot] = make_float4(tr.rec_t, __uint_as_float(tr.rec_inst), __uint_as_float(tr.rec_tri), tr.rec_u);
+                b.hit_v[slot] = tr.rec_v;
             }
         }
     }

@@ -68,11 +68,14 @@ def test_batch_equals_sequential(dev, views, pkw):
 
 CONFIGS = [
     dict(persistent_waves_per_cu=0, chunk_rays=64),
-    dict(persistent_waves_per_cu=8, chunk_rays=128),
+    dict(persistent_waves_per_cu=8, chunk_rays=128, traversal_loop=0),
     dict(persistent_waves_per_cu=16, chunk_rays=64, regroup_every=2),
     dict(persistent_waves_per_cu=0, chunk_rays=256, regroup_every=3, tile_width=16),
     dict(traversal_loop=0, tile_width=64),
     dict(stack_entries=64, tile_width=4, use_graph=0),
+    dict(persistent_waves_per_cu=16, chunk_rays=1024, lds_stack_entries=8, refill_lanes=1),
+    dict(persistent_waves_per_cu=32, chunk_rays=64, lds_stack_entries=0, refill_lanes=64, regroup_every=2),
+    dict(persistent_waves_per_cu=4, chunk_rays=4096, lds_stack_entries=16, refill_lanes=32, use_graph=0),
 ]
 
 
@@ -139,7 +142,7 @@ def test_autotune_keeps_images(dev, views):
     ref = _sequential(dev, views, prm, [0])
     b = Batch(dev, views, prm)
     cfg, ms = b.autotune(reps=1)
-    assert ms > 0 and cfg.chunk_rays in (64, 128, 256, 512) and cfg.stack_entries >= 24
+    assert ms > 0 and cfg.chunk_rays in (64, 128, 256, 512, 1024, 2048, 4096) and cfg.stack_entries >= 24
     b.render(0)
     for i, (img, r) in enumerate(zip(b.images(), ref)):
         assert same_bits(img, r).all(), f"view {i} after autotune {cfg.as_dict()}"
@@ -153,6 +156,8 @@ def test_batch_rejects_bad_views(dev, views):
     b = Batch(dev, views[:1], prm)
     with pytest.raises(N.PulseError, match="chunk_rays"):
         b.set_config(chunk_rays=100)
+    with pytest.raises(N.PulseError, match="refill_lanes"):
+        b.set_config(refill_lanes=0)
     with pytest.raises(N.PulseError, match="stack_entries"):
         b.set_config(stack_entries=8)
     b.close()
