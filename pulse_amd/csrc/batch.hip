@@ -470,6 +470,24 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
     // coordinate descent over the knobs, starting from the current config
     // (one untimed frame first: first-touch of the queues and scenes)
     if ((rc = render(b, 0)) == PULSE_OK && (rc = time_config(best, best_ms)) == PULSE_OK) {
+        // the work distribution and the chunk size interact: search them jointly first
+        for (uint32_t w : {0u, 16u, 24u, 32u}) {
+            for (uint32_t ch : {64u, 256u, 1024u}) {
+                pulse_batch_config c = best;
+                c.persistent_waves_per_cu = w;
+                c.chunk_rays = ch;
+                double ms = 0;
+                if (check_config(b, c) != PULSE_OK) continue;
+                if ((rc = time_config(c, ms)) != PULSE_OK) break;
+                if (ms < best_ms * 0.98) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+            if (rc) break;
+        }
+    }
+    if (rc == PULSE_OK) {
         std::vector<uint32_t> stack_opts = {0};
         for (uint32_t s : kStacks)
             if (s > pick_stack(b->stack_need, 1) && stack_opts.size() < 2) stack_opts.push_back(s);
