@@ -320,6 +320,8 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 #define PULSE_OPT_LDS_STACK_ENTRIES 7 /* persistent: stack entries per lane kept in   */
                                       /* LDS (0: all); deeper entries go to device    */
                                       /* memory, so more waves fit per CU             */
+#define PULSE_OPT_TRIS_PER_STEP 8  /* persistent: leaf triangles per traversal step  */
+                                   /* (1 or 2; same order, same results)            */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

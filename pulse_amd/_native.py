@@ -43,6 +43,7 @@ OPT_PERSISTENT_WAVES = 4
 OPT_REFILL_LANES = 5
 OPT_GATHER_LANES = 6
 OPT_LDS_STACK_ENTRIES = 7
+OPT_TRIS_PER_STEP = 8
 
 
 class PulseError(RuntimeError):

@@ -71,9 +71,9 @@ __device__ __forceinline__ float slab_pre(const PreRay &p, float rec_t, float4 l
 }
 
 // Resumable form of the single-loop traversal: begin() sets a ray up, each
-// step() visits one node (or tests one triangle of a leaf, or enters/leaves
-// one instance), so a persistent kernel can hand a lane a new ray as soon as
-// its ray is done.
+// step() visits one node (or tests TRIS_PER_STEP triangles of a leaf, in
+// order, or enters/leaves one instance), so a persistent kernel can hand a
+// lane a new ray as soon as its ray is done.
 //
 // Closest hit (ANY_HIT = false): rec_* hold the closest hit so far.
 // Occlusion (ANY_HIT = true): `occluded` becomes true iff the reference's
@@ -83,7 +83,7 @@ __device__ __forceinline__ float slab_pre(const PreRay &p, float rec_t, float4 l
 // stops at the first hit whose object t is below a bound that converts to a
 // world t below the threshold for certain (the instance's closest hit is
 // then below it too).
-template <bool ANY_HIT>
+template <bool ANY_HIT, int TRIS_PER_STEP = 1>
 struct Traversal {
     V3 wo, wd;
     PreRay world;
@@ -197,6 +197,10 @@ struct Traversal {
             if (test_tri) {
                 mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
                 tri_next += 1;
+                if (TRIS_PER_STEP == 2 && tri_next < tri_end) {
+                    mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
+                    tri_next += 1;
+                }
                 if (tri_next < tri_end) return true;
                 if (ANY_HIT && h.found && h.t < safe_t) {
                     occluded = true;

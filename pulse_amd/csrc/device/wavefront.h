@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
 // The top LDS_ENTRIES entries of each lane's stack live in LDS, the rest (up
 // to `capacity`) in `spill`, one column of capacity - LDS_ENTRIES words per
 // lane of the grid.
-template <int LDS_ENTRIES, bool SHADOW>
+template <int LDS_ENTRIES, bool SHADOW, int TPS>
 __global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill,
                                                     uint32_t gather, int capacity, uint32_t *spill) {
     __shared__ uint32_t smem[LDS_ENTRIES * 64];
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, i
     const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
     uint32_t *head = SHADOW ? &wb.sh_head[k] : &wb.ext_head[k];
     const uint32_t lane = __lane_id();
-    Traversal<SHADOW> tr;
+    Traversal<SHADOW, TPS> tr;
     bool alive = false;
     bool exhausted = false;  // wave-uniform: the queue has no rays left
     uint32_t slot = 0, pixel = 0;

@@ -99,6 +99,7 @@ struct pulse_ctx {
     int refill = 16;             // idle lanes that make a persistent wave take new rays
     int gather = 1;              // persistent: lanes gathered before an instance change runs
     int lds_entries = 24;        // persistent: stack entries kept in LDS (0: all); the rest spill
+    int tris_per_step = 2;       // persistent: leaf triangles tested per traversal step (1 or 2)
     pulse_host::DevBuf spill;    // persistent: spilled stack entries
     pulse_host::DevBuf loss_acc; // f64 accumulator of pulse_image_loss
     pulse_stats stats{};

@@ -118,16 +118,21 @@ int persistent_lds(pulse_ctx *ctx) {
         if (s <= want) got = s;
     return got;
 }
+template <int LDS, bool SHADOW, int TPS>
+void launch_queue_k(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, int q, int k, uint32_t *spill) {
+    hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx),
+                       wb, q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather),
+                       static_cast<int>(ctx->stack_entries), spill);
+}
 template <int LDS>
 void launch_queue_lds(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k, uint32_t *spill) {
-    const int capacity = static_cast<int>(ctx->stack_entries);
-    if (shadow)
-        hipLaunchKernelGGL((k_trace_queue<LDS, true>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx), wb,
-                           q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather), capacity, spill);
-    else
-        hipLaunchKernelGGL((k_trace_queue<LDS, false>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx),
-                           wb, q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather), capacity,
-                           spill);
+    if (ctx->tris_per_step == 2) {
+        if (shadow) launch_queue_k<LDS, true, 2>(ctx, wb, grid, q, k, spill);
+        else launch_queue_k<LDS, false, 2>(ctx, wb, grid, q, k, spill);
+    } else {
+        if (shadow) launch_queue_k<LDS, true, 1>(ctx, wb, grid, q, k, spill);
+        else launch_queue_k<LDS, false, 1>(ctx, wb, grid, q, k, spill);
+    }
 }
 // Spill area of the persistent launches over queues of up to `cap` rays:
 // the stack entries below the LDS part, for every lane of the grid.  Grows
@@ -808,6 +813,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value < 1 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "refill lanes must be 1..64");
             ctx->refill = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_TRIS_PER_STEP:
+            if (value != 1 && value != 2) PULSE_FAIL(PULSE_E_INVALID_ARG, "triangles per step must be 1 or 2");
+            ctx->tris_per_step = static_cast<int>(value);
+            return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES:
             if (value < 0 || value > 128) PULSE_FAIL(PULSE_E_INVALID_ARG, "LDS stack entries must be 0..128");
             ctx->lds_entries = static_cast<int>(value);
@@ -829,6 +838,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_REFILL_LANES: *value = ctx->refill; return PULSE_OK;
         case PULSE_OPT_GATHER_LANES: *value = ctx->gather; return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
+        case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
         case PULSE_OPT_FAST_DIVISION: *value = ctx->cur->fast_div ? 1 : 0; return PULSE_OK;
         default: PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_get_option: unknown option %d", option);

@@ -201,10 +201,10 @@ def test_traversal_variants_bit_exact(dev, field, loop):
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
 
 
-@pytest.mark.parametrize("waves,refill,gather,lds", [(0, 16, 1, 0), (4, 1, 1, 0), (16, 16, 1, 0), (16, 64, 8, 24),
-                                                      (64, 32, 64, 16), (16, 8, 16, 8), (8, 16, 32, 32),
-                                                      (24, 16, 1, 24)])
-def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds):
+@pytest.mark.parametrize("waves,refill,gather,lds,tps", [(0, 16, 1, 0, 1), (4, 1, 1, 0, 1), (16, 16, 1, 0, 2),
+                                                          (16, 64, 8, 24, 1), (64, 32, 64, 16, 2), (16, 8, 16, 8, 1),
+                                                          (8, 16, 32, 32, 1), (24, 16, 1, 24, 2)])
+def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds, tps):
     """Persistent extend/shadow launches (lanes take new rays as theirs
     finish; instance changes gathered across lanes; the deep part of the
     stack in device memory) give the oracle's image for every grid size,
@@ -215,6 +215,7 @@ def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds):
     dev.set_option(N.OPT_REFILL_LANES, refill)
     dev.set_option(N.OPT_GATHER_LANES, gather)
     dev.set_option(N.OPT_LDS_STACK_ENTRIES, lds)
+    dev.set_option(N.OPT_TRIS_PER_STEP, tps)
     try:
         assert dev.get_option(N.OPT_PERSISTENT_WAVES) == waves
         cam = camf(96, 56)
@@ -230,6 +231,7 @@ def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds):
         dev.set_option(N.OPT_REFILL_LANES, DEFAULT_REFILL)
         dev.set_option(N.OPT_GATHER_LANES, DEFAULT_GATHER)
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, DEFAULT_LDS)
+        dev.set_option(N.OPT_TRIS_PER_STEP, 2)
 
 
 @pytest.mark.parametrize("mode", [N.MODE_REF_GGX, N.MODE_GGX_NEE])
