@@ -198,6 +198,11 @@ typedef struct pulse_params {
 #define PULSE_FLAG_ADJOINT_REPARAM 8u /* adjoint: differentiate roughness through */
                                       /* the moving sampled directions instead of */
                                       /* at fixed directions (the default)        */
+#define PULSE_FLAG_LIGHT_POWER 16u /* NEE picks the light by emission strength   */
+                                   /* (light_emission_strength_cdf, pdf = its    */
+                                   /* step) as the reference intended            */
+                                   /* (utilities.wgsl:501, 541-546), instead of   */
+                                   /* uniformly (its live code)                  */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */

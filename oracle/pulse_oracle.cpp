@@ -88,6 +88,8 @@ oracle_scene *oracle_scene_create(const pulse_scene_desc *d) {
     s.materials = reinterpret_cast<const pulse_material *>(o->storage[8].data());
     s.light_triangle_area_cdfs = reinterpret_cast<const float *>(o->storage[10].data());
     s.light_mesh_areas = reinterpret_cast<const float *>(o->storage[11].data());
+    s.light_emission_strength_cdf = reinterpret_cast<const float *>(o->storage[9].data());
+    s.n_strength_cdf = d->light_emission_strength_cdf.bytes / 4;
     s.light_indices = reinterpret_cast<const pulse_light_data_index *>(o->storage[12].data());
     s.n_primitives = d->primitives.bytes / sizeof(pulse_primitive);
     s.n_instances = d->instances.bytes / sizeof(pulse_mesh_instance);
@@ -414,6 +416,14 @@ uint32_t oracle_cdf_scan_array(const float *cdf, uint32_t count, float e, int li
     Scene s;
     s.light_triangle_area_cdfs = cdf;
     return linear ? sample_light_triangle_area_cdf(s, e, 0, count) : sample_light_triangle_area_cdf_bsearch(s, e, 0, count);
+}
+// The emission-strength pick (utilities.wgsl:659-673) over a caller's CDF of
+// `count` lights.
+uint32_t oracle_strength_pick_array(const float *cdf, uint32_t count, float e) {
+    Scene s;
+    s.light_emission_strength_cdf = cdf;
+    s.uniform.light_count = count;
+    return sample_light_emission_strength_cdf(s, e);
 }
 // Lambert NEE at (p0, n0) with the scene's lights: out = direct[3].
 void oracle_direct_light_f32(oracle_scene *sc, const float *p0, const float *n0, const float *base, uint32_t *rng,

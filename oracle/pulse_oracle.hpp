@@ -161,6 +161,8 @@ struct Scene {
     const pulse_material *materials = nullptr;
     const float *light_triangle_area_cdfs = nullptr;
     const float *light_mesh_areas = nullptr;
+    const float *light_emission_strength_cdf = nullptr;
+    size_t n_strength_cdf = 0;
     const pulse_light_data_index *light_indices = nullptr;
     size_t n_primitives = 0, n_instances = 0, n_tlas = 0, n_blas = 0;
     bool cdf_linear_scan = false;  // true: run the reference's O(n) scan as written
@@ -839,12 +841,40 @@ struct ShadowSetup {
     pulse_primitive prim;
     uint32_t light_index;
     uint32_t material_index;
+    bool light_power;  // light picked by emission strength (else uniformly)
 };
 
+// sample_light_emission_strength_cdf, utilities.wgsl:659-673, as written:
+// `mid = l + (r - l)` makes it a top-down scan; the result is
+// min(#(cdf[i] <= e), light_count - 1).
+inline uint32_t sample_light_emission_strength_cdf(const Scene &s, float e) {
+    int64_t l = 0;
+    int64_t r = static_cast<int64_t>(s.uniform.light_count) - 1;
+    while (l <= r) {
+        int64_t mid = l + (r - l);
+        if (s.light_emission_strength_cdf[mid] <= e) {
+            l = mid + 1;
+        } else {
+            r = mid - 1;
+        }
+    }
+    return std::min(static_cast<uint32_t>(l), s.uniform.light_count - 1u);
+}
+// The pdf of light i under that pick, as the reference's commented-out code
+// (utilities.wgsl:541-546): cdf[0] for the first light, else the difference.
 template <class R>
-ShadowSetup<R> nee_setup(const Scene &s, V3<R> p0, V3<R> n0, uint32_t &rng, DecisionTape &tape) {
+inline R light_strength_pdf(const Scene &s, uint32_t i) {
+    if (i == 0) return static_cast<R>(s.light_emission_strength_cdf[0]);
+    return static_cast<R>(s.light_emission_strength_cdf[i]) - static_cast<R>(s.light_emission_strength_cdf[i - 1]);
+}
+
+template <class R>
+ShadowSetup<R> nee_setup(const Scene &s, V3<R> p0, V3<R> n0, uint32_t &rng, DecisionTape &tape, bool light_power) {
     ShadowSetup<R> st;
-    uint32_t light_index = rand_range_u(s.uniform.light_count, rng);
+    // the reference's intended pick (utilities.wgsl:501) or the live one (502)
+    uint32_t light_index = light_power ? sample_light_emission_strength_cdf(s, rand_f(rng))
+                                       : rand_range_u(s.uniform.light_count, rng);
+    st.light_power = light_power;
     const pulse_light_data_index &ldi = s.light_indices[light_index];
     const pulse_mesh_instance &mi = s.instances[ldi.mesh_instance_index];
     float e0 = rand_f(rng);
@@ -887,7 +917,8 @@ void nee_light_terms(const Scene &s, const ShadowSetup<R> &st, V3<R> &nl, R &pdf
     V3<R> side_b = load3<R>(st.prim.p_third) - load3<R>(st.prim.p_first);
     nl = normalize(cross(side_a, side_b));
     if (tape.take_bool(dot(nl, -st.ray.dir) < R(0))) nl = -nl;
-    R light_pdf = R(1) / static_cast<R>(static_cast<float>(s.uniform.light_count));
+    R light_pdf = st.light_power ? light_strength_pdf<R>(s, st.light_index)
+                                 : R(1) / static_cast<R>(static_cast<float>(s.uniform.light_count));
     R triangle_pdf = R(1) / static_cast<R>(s.light_mesh_areas[st.light_index]);
     pdf = light_pdf * triangle_pdf;
 }
@@ -895,8 +926,8 @@ void nee_light_terms(const Scene &s, const ShadowSetup<R> &st, V3<R> &nl, R &pdf
 // sample_direct_light (Lambert), utilities.wgsl:500-558
 template <class R>
 V3<R> sample_direct_light(const Scene &s, V3<R> p0, V3<R> n0, V3<R> base_color, uint32_t &rng, Counters &c,
-                          DecisionTape &tape) {
-    ShadowSetup<R> st = nee_setup(s, p0, n0, rng, tape);
+                          DecisionTape &tape, bool light_power = false) {
+    ShadowSetup<R> st = nee_setup(s, p0, n0, rng, tape, light_power);
     if (nee_occluded(s, st, c, tape)) return {R(0), R(0), R(0)};
     V3<R> nl;
     R pdf;
@@ -914,8 +945,8 @@ V3<R> sample_direct_light(const Scene &s, V3<R> p0, V3<R> n0, V3<R> base_color, 
 // reference; its unused theta/phi/F0 computations are omitted).
 template <class R>
 V3<R> sample_direct_light_ggx(const Scene &s, V3<R> p0, V3<R> n0, const MaterialR<R> &mat, V3<R> wo, uint32_t &rng,
-                              Counters &c, DecisionTape &tape) {
-    ShadowSetup<R> st = nee_setup(s, p0, n0, rng, tape);
+                              Counters &c, DecisionTape &tape, bool light_power = false) {
+    ShadowSetup<R> st = nee_setup(s, p0, n0, rng, tape, light_power);
     if (nee_occluded(s, st, c, tape)) return {R(0), R(0), R(0)};
     V3<R> nl;
     R pdf;
@@ -1000,6 +1031,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
     V3<R> throughput{R(1), R(1), R(1)};
     V3<R> color{R(0), R(0), R(0)};
     const bool have_lights = s.uniform.light_count > 0;
+    const bool light_power = (prm.flags & PULSE_FLAG_LIGHT_POWER) != 0;
 
     c.primary_rays++;
     if (prm.mode == PULSE_MODE_REF_NEE_LAMBERT) {
@@ -1017,7 +1049,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
         next.dir = scatter;
         next.t = sentinel<R>();
         V3<R> direct{R(0), R(0), R(0)};
-        if (have_lights) direct = sample_direct_light(s, h0.position, h0.normal, m0.base, rng, c, tape);
+        if (have_lights) direct = sample_direct_light(s, h0.position, h0.normal, m0.base, rng, c, tape, light_power);
         color = m0.emissive + direct;
         throughput = m0.base;
         ray = next;
@@ -1027,7 +1059,7 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
             HitShading<R> h = shade_hit(s, ray);
             MaterialR<R> m = scene_material<R>(s, h.material_index);
             V3<R> dl{R(0), R(0), R(0)};
-            if (have_lights) dl = sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape);
+            if (have_lights) dl = sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape, light_power);
             color = color + throughput * dl;
             throughput = throughput * m.base;
             if (depth >= prm.rr_start_depth && !russian_roulette(throughput, rng, tape)) break;
@@ -1053,8 +1085,8 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
             if (depth == 0) color = color + throughput * m.emissive;
             if (have_lights) {
                 V3<R> dl = prm.nee_eval == PULSE_NEE_GGX
-                               ? sample_direct_light_ggx(s, h.position, h.normal, m, wo, rng, c, tape)
-                               : sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape);
+                               ? sample_direct_light_ggx(s, h.position, h.normal, m, wo, rng, c, tape, light_power)
+                               : sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape, light_power);
                 color = color + throughput * dl;
             }
         }

@@ -35,6 +35,7 @@ FLAG_COLLECT_STATS = 1
 FLAG_STAGE_TIMING = 2
 FLAG_HORIZON_CLAMP = 4
 FLAG_ADJOINT_REPARAM = 8
+FLAG_LIGHT_POWER = 16
 GRAD_PER_MATERIAL = 7
 OPT_TRAVERSAL_LOOP = 1
 OPT_STACK_ENTRIES = 2

@@ -110,6 +110,7 @@ BatchArgs batch_args(const pulse_batch *b) {
     a.max_traces = b->traces;
     a.rr_start = b->prm.rr_start_depth;
     a.horizon_clamp = (b->prm.flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
+    a.light_power = (b->prm.flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
     a.chunk = b->cfg.chunk_rays;
     a.persistent = b->cfg.persistent_waves_per_cu;
     a.refill = b->cfg.refill_lanes;
@@ -323,6 +324,12 @@ int pulse_batch_create(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views
     };
     for (uint32_t i = 0; i < n_views; ++i) {
         const pulse_view &v = views[i];
+        if (v.scene && (prm->flags & PULSE_FLAG_LIGHT_POWER) && v.scene->data.valid &&
+            v.scene->data.n_strength_cdf < v.scene->data.uniform.light_count) {
+            pulse_batch_destroy(b);
+            PULSE_FAIL(PULSE_E_LAYOUT, "pulse_batch_create: view %u: PULSE_FLAG_LIGHT_POWER needs one "
+                       "light_emission_strength_cdf entry per light", i);
+        }
         if (!v.scene || v.scene->ctx != ctx || !v.scene->data.valid || !v.accum_rgba || v.camera.width == 0 ||
             v.camera.height == 0) {
             pulse_batch_destroy(b);

@@ -64,7 +64,7 @@ struct BatchArgs {
     uint32_t *work;        // persistent work counters: extend k at 2k, shadow k at 2k+1, shade k at 2 max_traces + k
     uint64_t *totals;      // [2]: extension rays, shadow rays (stats)
     const uint32_t *frame; // frame[0]: accumulation count of the first sample
-    uint32_t mode, nee_ggx, max_traces, rr_start, horizon_clamp;
+    uint32_t mode, nee_ggx, max_traces, rr_start, horizon_clamp, light_power;
     uint32_t chunk;        // rays per chunk, a multiple of 64
     uint32_t persistent;   // 0: one wave per chunk; else a fixed grid drains the queue
     uint32_t refill;       // persistent single-loop traversal: idle lanes that take new rays
@@ -314,6 +314,7 @@ __global__ void __launch_bounds__(256) k_b_shade(BatchArgs b, int q, int k) {
     fa.max_traces = b.max_traces;
     fa.rr_start = b.rr_start;
     fa.horizon_clamp = b.horizon_clamp;
+    fa.light_power = b.light_power;
     fa.accumulation_count = 0;
     for (uint32_t g = next_chunk(b, wslot, wave_global, true); g < cm.total;
          g = next_chunk(b, wslot, wave_global, false)) {

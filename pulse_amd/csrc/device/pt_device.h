@@ -47,6 +47,7 @@ struct DevScene {
     const float *light_cdfs;
     const float *light_areas;
     const pulse_light_data_index *lights;
+    const float *strength_cdf;          // light_emission_strength_cdf (may be short: checked on the host)
     uint32_t instance_count;
     uint32_t light_count;
     uint32_t *error_flags;
@@ -624,11 +625,17 @@ struct NeeSampleT {
 // sample_direct_light / sample_direct_light_ggx (utilities.wgsl:500-558,
 // 562-655) up to the occlusion test, plus the contribution they return when
 // the light is visible.  Consumes 4 random numbers.
+// light_power: the light is picked by emission strength (the reference's
+// intended sample_light_emission_strength_cdf, utilities.wgsl:501, 659-673,
+// as a binary search with the same result as its scan) with pdf = the CDF
+// step (541-546); otherwise uniformly (502, 540).
 template <class S>
 __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3T<S> &p0, const V3T<S> &n0,
-                                                    const MaterialT<S> &m, const V3T<S> &wo, bool ggx, uint32_t &rng) {
+                                                    const MaterialT<S> &m, const V3T<S> &wo, bool ggx, uint32_t &rng,
+                                                    bool light_power = false) {
     NeeSampleT<S> ns;
-    uint32_t light_index = rand_u(rng) % sc.light_count;
+    uint32_t light_index = light_power ? cdf_pick(sc.strength_cdf, rand_f(rng), sc.light_count)
+                                       : rand_u(rng) % sc.light_count;
     pulse_light_data_index ldi = sc.lights[light_index];
     const pulse_mesh_instance &mi = sc.instances[ldi.mesh_instance_index];
     float e0 = rand_f(rng);
@@ -657,7 +664,9 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
     // light normal from the object-space triangle (utilities.wgsl:533-538)
     V3 nl = normalize(cross(sa, sb));
     if (val(dot(nl, neg(ns.dir))) < 0.0f) nl = neg(nl);
-    float light_pdf = 1.0f / static_cast<float>(sc.light_count);
+    float light_pdf = light_power ? (light_index == 0u ? sc.strength_cdf[0]
+                                                       : sc.strength_cdf[light_index] - sc.strength_cdf[light_index - 1u])
+                                  : 1.0f / static_cast<float>(sc.light_count);
     float triangle_pdf = 1.0f / sc.light_areas[light_index];
     float pdf = light_pdf * triangle_pdf;
     S cos_r = dot(ns.dir, n0);

@@ -27,6 +27,7 @@ struct FrameArgs {
     uint32_t rr_start;        // first loop depth with Russian roulette
     uint32_t accumulation_count;
     uint32_t horizon_clamp;   // PULSE_FLAG_HORIZON_CLAMP
+    uint32_t light_power;     // PULSE_FLAG_LIGHT_POWER
 };
 
 // Appends `valid` items of a wave to a queue: returns this lane's slot.

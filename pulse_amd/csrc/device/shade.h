@@ -36,7 +36,7 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
             V3 scatter = sample_cosine_hemisphere(h.normal, a0, a1);
             V3 dl_vis = mk(0.0f, 0.0f, 0.0f);
             if (lights) {
-                NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, false, rng);
+                NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, false, rng, fa.light_power != 0u);
                 out.emit_shadow = true;
                 out.sh_o = ns.origin;
                 out.sh_d = ns.dir;
@@ -55,7 +55,7 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
         } else {
             // loop body, path_tracer.wgsl:175-190
             if (lights) {
-                NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, false, rng);
+                NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, false, rng, fa.light_power != 0u);
                 out.emit_shadow = true;
                 out.sh_o = ns.origin;
                 out.sh_d = ns.dir;
@@ -84,7 +84,7 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
     } else {
         if (fa.mode == PULSE_MODE_REF_GGX || k == 0) C = add(C, mul(T, m.emissive));
         if (fa.mode == PULSE_MODE_GGX_NEE && lights) {
-            NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, fa.nee_ggx != 0u, rng);
+            NeeSample ns = nee_sample(sc, h.position, h.normal, m, wo, fa.nee_ggx != 0u, rng, fa.light_power != 0u);
             out.emit_shadow = true;
             out.sh_o = ns.origin;
             out.sh_d = ns.dir;

@@ -45,7 +45,7 @@ struct DevBuf {
 
 enum SceneBuf {
     B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
-    B_CDFS, B_AREAS, B_LIGHTS, B_COUNT
+    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_COUNT
 };
 
 // Queue counter words of the single-view path: extension and shadow queue
@@ -65,6 +65,7 @@ struct SceneData {
     uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
     uint32_t tlas_depth = 0, blas_depth = 0;
     uint32_t n_materials = 0;
+    uint32_t n_strength_cdf = 0;  // entries of light_emission_strength_cdf
     bool fast_div = false;       // node coordinates allow the exact fast division
     bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
     bool valid = false;

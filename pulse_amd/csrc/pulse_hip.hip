@@ -262,6 +262,7 @@ DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     s.light_cdfs = sd.buf[B_CDFS].as<float>();
     s.light_areas = sd.buf[B_AREAS].as<float>();
     s.lights = sd.buf[B_LIGHTS].as<pulse_light_data_index>();
+    s.strength_cdf = sd.buf[B_SCDF].as<float>();
     s.instance_count = sd.uniform.instance_count;
     s.light_count = sd.uniform.light_count;
     s.error_flags = errors;
@@ -294,7 +295,6 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = count_of(d->light_mesh_areas, 4, "light_mesh_areas", n_area)) ||
         (rc = count_of(d->light_indices, sizeof(pulse_light_data_index), "light_indices", n_light)))
         return rc;
-    (void)n_scdf;
     if (n_tlas == 0) PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: tlas_nodes is empty (needs at least the root)");
     if (d->uniform.instance_count > n_inst)
         PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: instance_count %u > %zu instances", d->uniform.instance_count, n_inst);
@@ -399,9 +399,11 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_MATERIALS, d->materials.data, d->materials.bytes)) ||
         (rc = up(B_CDFS, d->light_triangle_area_cdfs.data, d->light_triangle_area_cdfs.bytes)) ||
         (rc = up(B_AREAS, d->light_mesh_areas.data, d->light_mesh_areas.bytes)) ||
-        (rc = up(B_LIGHTS, lights, d->light_indices.bytes)))
+        (rc = up(B_LIGHTS, lights, d->light_indices.bytes)) ||
+        (rc = up(B_SCDF, d->light_emission_strength_cdf.data, d->light_emission_strength_cdf.bytes)))
         return rc;
     sd.uniform = d->uniform;
+    sd.n_strength_cdf = static_cast<uint32_t>(n_scdf);
     sd.stack_need = need;
     sd.n_materials = static_cast<uint32_t>(n_mat);
     sd.any_metallic = false;
@@ -636,6 +638,10 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     fa.max_traces = traces;
     fa.rr_start = prm->rr_start_depth;
     fa.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
+    fa.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
+    if (fa.light_power && ctx->cur->n_strength_cdf < ctx->cur->uniform.light_count)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_trace: PULSE_FLAG_LIGHT_POWER needs one light_emission_strength_cdf entry "
+                   "per light (%u for %u lights)", ctx->cur->n_strength_cdf, ctx->cur->uniform.light_count);
     const uint32_t cap = static_cast<uint32_t>(npix);
     const uint32_t tiles = ((cam->width + 7) / 8) * ((cam->height + 7) / 8);
     const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
@@ -748,6 +754,8 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     if (prm->accumulation_count != 0 || prm->spp == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs accumulation_count 0 and spp >= 1");
     if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: materials with metallic != 0");
+    if (prm->flags & PULSE_FLAG_LIGHT_POWER)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: PULSE_FLAG_LIGHT_POWER (the light pdf depends on emission)");
     if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: BVH too deep for the adjoint stack");
     int rc = set_device(ctx);
     if (rc) return rc;

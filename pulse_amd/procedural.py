@@ -198,6 +198,36 @@ def cornell(seed: int = 0, threads: int = 8):
     return scene, camera
 
 
+def cornell_lights(strengths=(2.0, 40.0, 8.0), seed: int = 0, threads: int = 8):
+    """The Cornell box with several ceiling quad emitters of different
+    radiance (one emissive material each), for light selection by emission
+    strength; returns (SceneBuffers, camera factory)."""
+    rng = CounterRng(seed)
+    u = rng.uniform(1, 8)
+    sb = SceneBuilder()
+    walls = sb.add_mesh(cornell_walls_mesh())
+    box = sb.add_mesh(box_mesh())
+    quad = sb.add_mesh(quad_mesh(0.3))
+    m_walls = sb.add_material(MaterialDesc(base_color=(*(0.2 + 0.6 * u[0:3]), 1.0),
+                                           perceptual_roughness=float(0.1 + 0.9 * u[3])))
+    m_boxes = sb.add_material(MaterialDesc(base_color=(*(0.2 + 0.6 * u[4:7]), 1.0),
+                                           perceptual_roughness=float(0.1 + 0.9 * u[7])))
+    sb.add_instance(walls, m_walls, np.eye(4))
+    sb.add_instance(box, m_boxes, trs((-0.35, -0.4, -0.3), 0.3, (0.6, 1.2, 0.6)))
+    n = len(strengths)
+    for k, e in enumerate(strengths):
+        m = sb.add_material(MaterialDesc(base_color=(0.8, 0.8, 0.8, 1.0), emissive=(e, e, e, 1.0),
+                                         perceptual_roughness=0.5))
+        x = -0.6 + 1.2 * (k + 0.5) / n
+        sb.add_instance(quad, m, trs((x, 0.99, 0.1 * k)))
+    scene = sb.build(threads)
+
+    def camera(width=256, height=256):
+        return make_camera(width, height, eye=(0.0, 0.0, 3.0), target=(0.0, 0.0, 0.0))
+
+    return scene, camera
+
+
 def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800, n_emissive: int = 32,
                     box: float = 100.0, radius=(1.5, 4.0), roughness=(0.05, 1.0), emission: float = 8.0,
                     seed: int = 2, threads: int = 8, n_materials: int | None = None):

@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
                                          C.c_double, _P, _P, C.c_int]),
             ("oracle_grad_fd_ex", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
                                             C.c_double, _P, _P, C.c_int, C.c_int]),
+            ("oracle_strength_pick_array", _U32, [_P, _U32, C.c_float]),
             ("oracle_rand_u", _U32, [C.POINTER(_U32)]),
             ("oracle_rand_f", C.c_float, [C.POINTER(_U32)]),
             ("oracle_pixel_seed", _U32, [_U32, _U32]),

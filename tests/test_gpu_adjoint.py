@@ -115,6 +115,9 @@ def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen):
                                       flags=N.FLAG_HORIZON_CLAMP))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
         adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_GGX, rr_start_depth=3))
+    with pytest.raises(N.PulseError, match="UNSUPPORTED"):
+        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3,
+                                      flags=N.FLAG_HORIZON_CLAMP | N.FLAG_LIGHT_POWER))
 
 
 def _field_fd_case():

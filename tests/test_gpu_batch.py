@@ -50,6 +50,8 @@ def _sequential(dev, views, params, acc_list):
 
 PARAMS = [
     pytest.param(dict(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1), id="ggx_nee"),
+    pytest.param(dict(spp=1, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=1, flags=N.FLAG_LIGHT_POWER),
+                 id="ggx_nee_light_power"),
     pytest.param(dict(spp=1, max_depth=5, mode=N.MODE_REF_GGX), id="ref_ggx"),
     pytest.param(dict(spp=1, max_depth=4, mode=N.MODE_REF_NEE_LAMBERT, rr_start_depth=2), id="ref_nee_lambert"),
 ]

@@ -242,3 +242,18 @@ def test_horizon_clamp_bit_exact(dev, field, mode):
     g = _render_gpu(dev, scene, cam, prm)
     o, _, _ = OracleScene(scene).render(cam, prm)
     assert same_bits(g, o).all()
+
+
+@pytest.mark.parametrize("mode,nee,depth", [(N.MODE_GGX_NEE, N.NEE_LAMBERT, 4), (N.MODE_GGX_NEE, N.NEE_GGX, 4),
+                                            (N.MODE_REF_NEE_LAMBERT, N.NEE_LAMBERT, 4)])
+def test_light_power_bit_exact(dev, mode, nee, depth):
+    """NEE with the light picked by emission strength: the oracle's image on
+    a box with three emitters of different radiance."""
+    scene, camf = P.cornell_lights()
+    cam = camf(72, 64)
+    prm = make_params(spp=2, max_depth=depth, mode=mode, nee_eval=nee, rr_start_depth=1, flags=N.FLAG_LIGHT_POWER)
+    g = _render_gpu(dev, scene, cam, prm)
+    o, _, _ = OracleScene(scene).render(cam, prm)
+    assert same_bits(g, o).all()
+    u = _render_gpu(dev, scene, cam, make_params(spp=2, max_depth=depth, mode=mode, nee_eval=nee, rr_start_depth=1))
+    assert not same_bits(g, u).all()  # the flag changes the estimator
