@@ -149,6 +149,34 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
                       "65 materials x 7 parameters"}
 
 
+def bench_instance_update(dev, scene, full_upload_s: float) -> dict:
+    """Per-frame scene change as the reference does it every frame (new
+    instance transforms -> instance records + TLAS rebuilt on the host,
+    src/scene/mod.rs:561-697): 80 of the 800 instances moved, host rebuild
+    (pulse_sb_update_instances) and device update of the three instance
+    buffers (pulse_scene_update_instances), against a full 13-buffer upload."""
+    sb = scene.builder
+    u = np.random.default_rng(5).uniform(-3.0, 3.0, (80, 3))
+    inst = scene.buffers["instances"]
+    for k, i in enumerate(range(0, 800, 10)):
+        m = np.asarray(inst[i]["object_world"], np.float64).reshape(4, 4).T.copy()  # column-major -> row-major
+        m[:3, 3] += u[k]
+        sb.set_instance_transform(i, m)
+    t0 = time.perf_counter()
+    moved = sb.update_instances(scene)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.update_instances(moved)
+    dev.sync()
+    device_ms = (time.perf_counter() - t0) * 1e3
+    nbytes = sum(moved.buffers[k].nbytes for k in ("instances", "tlas_nodes", "instance_indices"))
+    return {"moved_instances": 80, "host_rebuild_ms": round(host_ms, 2),
+            "builder_instances_tlas_ms": round(moved.info["ms_instances_tlas"], 2),
+            "device_update_ms": round(device_ms, 2), "device_update_bytes": int(nbytes),
+            "full_upload_ms": round(full_upload_s * 1e3, 1)}
+
+
 def _pmc_run(counter: str, extra_args, timeout_s: float = 150.0):
     """Runs this script under `rocprofv3 --pmc <counter>` (a child process,
     its own process group, killed at the time limit) and returns the rows of
@@ -339,7 +367,7 @@ def main() -> int:
     from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
 
     t0 = time.perf_counter()
-    scene, camf = P.icosphere_field(subdiv=args.subdiv, threads=min(16, os.cpu_count() or 1))
+    scene, camf = P.icosphere_field(subdiv=args.subdiv, threads=min(16, os.cpu_count() or 1), keep_builder=True)
     host_build_s = time.perf_counter() - t0
     cam = camf(args.width, args.height)
     print(f"bench: scene built in {host_build_s:.1f} s: {scene.info['unique_triangles']} unique / "
@@ -385,6 +413,7 @@ def main() -> int:
     value = agg["incoherent_rays"] / (agg["ms_incoherent"] / 1e3) / 1e6
     all_rays = agg["extension_rays"] + agg["shadow_rays"]
     img = node.readback()
+    instance_update = bench_instance_update(dev, scene, upload_s)
     print(f"bench: {args.steps} frames, {ms_per_step:.2f} ms/frame, {value:.1f} incoherent Mrays/s", file=sys.stderr,
           flush=True)
 
@@ -423,6 +452,7 @@ def main() -> int:
         "stack_entries": dev.get_option(N.OPT_STACK_ENTRIES),
         "fast_division": dev.get_option(N.OPT_FAST_DIVISION),
         "image_mean": float(np.mean(img[..., :3])),
+        "instance_update": instance_update,
     }
 
     if not args.no_adjoint:

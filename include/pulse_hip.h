@@ -284,6 +284,14 @@ int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials
  * scene belongs to its context; destroy scenes before the context. */
 typedef struct pulse_scene pulse_scene;
 int pulse_scene_create(pulse_ctx *ctx, const pulse_scene_desc *desc, pulse_scene **out);
+/* Per-frame instance update of a resident scene (`scene`, or the uploaded
+ * one if NULL): replaces only the instance records, TLAS nodes and instance
+ * indices (bindings 7, 5, 6) -- the buffers the reference's
+ * prepare_mesh_instances and build_tlas (src/scene/mod.rs:561-697,
+ * tlas.rs:29-136) rewrite every frame -- instead of all 13.  The meshes,
+ * their BLAS and the light table must be unchanged (checked; otherwise
+ * PULSE_E_LAYOUT: upload the scene).  Synchronises the context's stream. */
+int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse_scene_desc *desc);
 int pulse_scene_destroy(pulse_scene *scene);
 int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
 

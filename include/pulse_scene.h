@@ -77,6 +77,14 @@ int pulse_sb_add_instance(pulse_scene_builder *sb, uint32_t mesh_id, uint32_t ma
 /* Builds every BLAS (threads > 1 builds meshes in parallel), then instances,
  * light tables and the TLAS. */
 int pulse_sb_build(pulse_scene_builder *sb, int threads);
+/* Moves instance `instance_id` (column-major object->world matrix); takes
+ * effect at the next pulse_sb_update_instances / pulse_sb_build. */
+int pulse_sb_set_instance_transform(pulse_scene_builder *sb, uint32_t instance_id, const float *transform16);
+/* Recomputes the instance records, light tables and TLAS of a built scene
+ * (the reference's per-frame prepare_mesh_instances + build_tlas,
+ * src/scene/mod.rs:561-697) without touching meshes or BLAS; the result is
+ * what pulse_sb_build would produce with the new transforms. */
+int pulse_sb_update_instances(pulse_scene_builder *sb);
 /* Views into builder-owned buffers, valid until the next build/destroy. */
 int pulse_sb_get_desc(pulse_scene_builder *sb, pulse_scene_desc *out);
 int pulse_sb_get_info(pulse_scene_builder *sb, pulse_scene_build_info *out);

@@ -246,6 +246,9 @@ SIGNATURES = [
     ("pulse_scene_create", _I, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
     ("pulse_scene_destroy", _I, [_P]),
     ("pulse_scene_bind", _I, [_P, _P]),
+    ("pulse_scene_update_instances", _I, [_P, _P, C.POINTER(SceneDesc)]),
+    ("pulse_sb_set_instance_transform", _I, [_P, _U32, _P]),
+    ("pulse_sb_update_instances", _I, [_P]),
     ("pulse_batch_default_config", _I, [C.POINTER(BatchConfig)]),
     ("pulse_batch_create", _I, [_P, C.POINTER(View), _U32, C.POINTER(Params), C.POINTER(_P)]),
     ("pulse_batch_destroy", _I, [_P]),

@@ -28,6 +28,7 @@ struct pulse_batch {
     uint32_t max_chunks = 0;    // sum over views of ceil(npix / chunk)
     std::vector<BatchView> views;
     std::vector<pulse_scene *> scenes;
+    std::vector<uint64_t> scene_gen;  // scene generations the view table was written for
     DevBuf d_views, counts, work, totals, frame, spill;
     DevBuf slot_buf[11];
     pulse_batch_config cfg{};
@@ -267,7 +268,30 @@ int apply_config(pulse_batch *b, const pulse_batch_config &c) {
     return PULSE_OK;
 }
 
+// Picks up instance updates of the views' scenes (new TLAS, maybe moved
+// buffers): refreshes the view table read by the kernels.
+int refresh_scenes(pulse_batch *b) {
+    bool changed = false;
+    for (uint32_t i = 0; i < b->n_views; ++i) {
+        const SceneData &sd = b->scenes[i]->data;
+        if (sd.generation == b->scene_gen[i]) continue;
+        if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_batch_render: view %u: its scene is not valid", i);
+        if (sd.stack_need > b->stack_entries)
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_batch_render: view %u: the updated scene needs %u stack entries, the "
+                       "batch has %u (set a larger stack_entries)", i, sd.stack_need, b->stack_entries);
+        b->views[i].sc = dev_scene(sd, b->ctx->errors.as<uint32_t>());
+        b->scene_gen[i] = sd.generation;
+        changed = true;
+    }
+    if (!changed) return PULSE_OK;
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    HIP_TRY(hipMemcpy(b->d_views.ptr, b->views.data(), sizeof(BatchView) * b->views.size(), hipMemcpyHostToDevice));
+    return PULSE_OK;
+}
+
 int render(pulse_batch *b, uint32_t acc) {
+    int rc0 = refresh_scenes(b);
+    if (rc0) return rc0;
     hipStream_t st = b->ctx->stream;
     hipLaunchKernelGGL(k_b_set_frame, dim3(1), dim3(1), 0, st, b->frame.as<uint32_t>(), acc);
     if (b->cfg.use_graph) {
@@ -349,6 +373,7 @@ int pulse_batch_create(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views
         slots += (bv.npix + 255u) / 256u * 256u;
         b->views.push_back(bv);
         b->scenes.push_back(v.scene);
+        b->scene_gen.push_back(v.scene->data.generation);
         b->stack_need = std::max(b->stack_need, v.scene->data.stack_need);
     }
     if (slots > (1ull << 31)) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../device/pt_device.h"
 #include "../device/queue.h"
@@ -67,8 +68,14 @@ struct SceneData {
     uint32_t n_materials = 0;
     uint32_t n_strength_cdf = 0;  // entries of light_emission_strength_cdf
     bool fast_div = false;       // node coordinates allow the exact fast division
+    bool fast_div_blas = false;  // the BLAS part of fast_div
     bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
     bool valid = false;
+    uint64_t generation = 0;     // bumped by every upload or instance update
+    // host copies checked by instance updates: per instance {triangle_offset,
+    // triangle_count, index_offset, node_offset}, and the light table
+    std::vector<uint32_t> inst_ranges;
+    std::vector<pulse_light_data_index> light_indices;
     void release() {
         for (auto &b : buf) b.release();
         valid = false;
@@ -77,6 +84,9 @@ struct SceneData {
 
 // Validates `d` and uploads it into `s` (left unchanged on failure).
 int scene_upload(SceneData &s, const pulse_scene_desc *d);
+// Replaces the instances, TLAS and instance indices of an uploaded scene
+// (same meshes, same lights); validation failures leave `s` unchanged.
+int scene_update_instances(SceneData &s, const pulse_scene_desc *d);
 // Device view of a scene; `errors` is the context's device error word.
 pt::DevScene dev_scene(const SceneData &s, uint32_t *errors);
 // Smallest compiled stack size that holds `need` entries (0: none).

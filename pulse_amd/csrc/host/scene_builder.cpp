@@ -434,6 +434,9 @@ struct pulse_scene_builder {
         std::vector<pulse_blas_node> nodes;
         std::vector<uint32_t> tri_indices;
         uint32_t depth = 0;
+        std::vector<float> area_cdf;  // light triangle-area CDF, once the mesh is used as a light
+        float area_total = 0.f;
+        bool has_area_cdf = false;
     };
     struct Instance {
         uint32_t mesh, material;
@@ -458,7 +461,19 @@ struct pulse_scene_builder {
     std::vector<pulse_light_data_index> light_indices;
     pulse_scene_build_info info{};
     bool built = false;
+    // where each mesh landed in the concatenated buffers (set by the build)
+    struct MeshIndex {
+        uint32_t triangle_offset, triangle_count, index_offset, node_offset;
+    };
+    std::vector<MeshIndex> mesh_index;
 };
+
+namespace {
+// prepare_mesh_instances (scene/mod.rs:561-697), create_emission_strength_cdf
+// (726-742) and build_tlas (693): everything that depends on the instances,
+// given the meshes' places in the concatenated buffers.
+int build_instances_and_tlas(pulse_scene_builder &S);
+}  // namespace
 
 extern "C" int pulse_sb_create(pulse_scene_builder **out) {
     if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sb_create: null out");
@@ -577,10 +592,8 @@ extern "C" int pulse_sb_build(pulse_scene_builder *sb, int threads) {
     S.triangle_data.clear();
     S.triangle_indices.clear();
     S.blas_nodes.clear();
-    struct MeshIndex {
-        uint32_t triangle_offset, triangle_count, index_offset, node_offset;
-    };
-    std::vector<MeshIndex> mesh_index(nm);
+    std::vector<pulse_scene_builder::MeshIndex> &mesh_index = S.mesh_index;
+    mesh_index.assign(nm, {});
     S.info = pulse_scene_build_info{};
     for (size_t k = 0; k < nm; ++k) {
         auto &m = S.meshes[k];
@@ -594,6 +607,21 @@ extern "C" int pulse_sb_build(pulse_scene_builder *sb, int threads) {
         S.info.max_blas_depth = std::max(S.info.max_blas_depth, m.depth);
     }
 
+    int rc = build_instances_and_tlas(S);
+    if (rc != PULSE_OK) return rc;
+    double t2 = now_ms();
+    S.info.ms_blas = t1 - t0;
+    S.info.ms_instances_tlas = t2 - t1;
+    S.built = true;
+    return PULSE_OK;
+}
+
+namespace {
+int build_instances_and_tlas(pulse_scene_builder &S) {
+    using MeshIndex = pulse_scene_builder::MeshIndex;
+    const auto &mesh_index = S.mesh_index;
+    const size_t nm = S.meshes.size();
+    S.info.instanced_triangles = 0;
     // --- prepare_mesh_instances, scene/mod.rs:561-697 ---
     S.out_instances.clear();
     S.triangle_cdfs.clear();
@@ -620,8 +648,14 @@ extern "C" int pulse_sb_build(pulse_scene_builder *sb, int threads) {
         V3 em{mat.emissive[0], mat.emissive[1], mat.emissive[2]};
         if (vlength(em) > 0.0001f) {
             strengths.push_back(vlength_sq(em));
-            std::vector<float> cdf;
-            float total = triangle_area_cdf(&S.primitives[mx.triangle_offset], mx.triangle_count, cdf);
+            // the area CDF depends on the mesh only: computed once per mesh
+            auto &mesh = S.meshes[in.mesh];
+            if (!mesh.has_area_cdf) {
+                mesh.area_total = triangle_area_cdf(&S.primitives[mx.triangle_offset], mx.triangle_count, mesh.area_cdf);
+                mesh.has_area_cdf = true;
+            }
+            const std::vector<float> &cdf = mesh.area_cdf;
+            const float total = mesh.area_total;
             pulse_light_data_index li;
             li.cdf_offset = static_cast<uint32_t>(S.triangle_cdfs.size());
             li.mesh_instance_index = static_cast<uint32_t>(S.out_instances.size() - 1);
@@ -670,8 +704,6 @@ extern "C" int pulse_sb_build(pulse_scene_builder *sb, int threads) {
                               &tcount, S.instance_indices.data());
     if (rc != PULSE_OK) return rc;
     S.tlas_nodes.resize(tcount);
-    double t2 = now_ms();
-
     S.uniform.instance_count = ni;
     S.uniform.light_count = static_cast<uint32_t>(S.light_indices.size());
     S.info.mesh_count = static_cast<uint32_t>(nm);
@@ -684,9 +716,33 @@ extern "C" int pulse_sb_build(pulse_scene_builder *sb, int threads) {
                                             [](const pulse_tlas_node &x) { return x.instance_count; },
                                             [](const pulse_tlas_node &x) { return x.a_or_first_instance; })
                                : 0;
-    S.info.ms_blas = t1 - t0;
-    S.info.ms_instances_tlas = t2 - t1;
-    S.built = true;
+    return PULSE_OK;
+}
+}  // namespace
+
+// Moves instance `instance_id` (column-major object->world matrix); takes
+// effect at the next pulse_sb_update_instances or pulse_sb_build.
+extern "C" int pulse_sb_set_instance_transform(pulse_scene_builder *sb, uint32_t instance_id, const float *transform16) {
+    if (!sb || !transform16) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sb_set_instance_transform: null argument");
+    if (instance_id >= sb->instances.size())
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sb_set_instance_transform: bad instance id %u", instance_id);
+    std::memcpy(sb->instances[instance_id].m, transform16, sizeof(float) * 16);
+    return PULSE_OK;
+}
+
+// Recomputes the instance records, light tables and TLAS of a built scene
+// (the reference's per-frame prepare_mesh_instances + build_tlas) without
+// touching the meshes, their BLAS or the concatenated triangle buffers.
+extern "C" int pulse_sb_update_instances(pulse_scene_builder *sb) {
+    if (!sb) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sb_update_instances: null builder");
+    if (!sb->built) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_sb_update_instances: call pulse_sb_build first");
+    double t0 = now_ms();
+    int rc = build_instances_and_tlas(*sb);
+    if (rc != PULSE_OK) {
+        sb->built = false;
+        return rc;
+    }
+    sb->info.ms_instances_tlas = now_ms() - t0;
     return PULSE_OK;
 }
 

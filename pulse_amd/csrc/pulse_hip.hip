@@ -379,6 +379,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     bool fast_div = true;
     for (size_t k = 0; k < n_blas && fast_div; ++k)
         for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(blas[k].aabb_min[c]) && coord_ok(blas[k].aabb_max[c]);
+    const bool fast_div_blas = fast_div;
     for (size_t k = 0; k < n_tlas && fast_div; ++k)
         for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(tlas[k].aabb_min[c]) && coord_ok(tlas[k].aabb_max[c]);
     if (pick_stack(need, 0) == 0 && pick_stack(need, 1) == 0)
@@ -410,9 +411,84 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     for (size_t k = 0; k < n_mat; ++k)
         if (static_cast<const pulse_material *>(d->materials.data)[k].metallic != 0.0f) sd.any_metallic = true;
     sd.fast_div = fast_div;
+    sd.fast_div_blas = fast_div_blas;
     sd.tlas_depth = tlas_depth;
     sd.blas_depth = blas_depth;
+    sd.inst_ranges.resize(4 * n_inst);
+    for (size_t i = 0; i < n_inst; ++i) {
+        sd.inst_ranges[4 * i] = inst[i].triangle_offset;
+        sd.inst_ranges[4 * i + 1] = inst[i].triangle_count;
+        sd.inst_ranges[4 * i + 2] = inst[i].index_offset;
+        sd.inst_ranges[4 * i + 3] = inst[i].node_offset;
+    }
+    sd.light_indices.assign(lights, lights + n_light);
+    sd.generation += 1;
     sd.valid = true;
+    return PULSE_OK;
+}
+
+int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
+    if (!d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_update_instances: null argument");
+    if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_update_instances: no scene uploaded");
+    const size_t n_inst = d->instances.bytes / sizeof(pulse_mesh_instance);
+    const size_t n_tlas = d->tlas_nodes.bytes / sizeof(pulse_tlas_node);
+    const size_t n_iidx = d->instance_indices.bytes / 4;
+    if (d->instances.bytes % sizeof(pulse_mesh_instance) || d->tlas_nodes.bytes % sizeof(pulse_tlas_node) ||
+        d->instance_indices.bytes % 4 || (d->instances.bytes && !d->instances.data) || !d->tlas_nodes.data ||
+        (d->instance_indices.bytes && !d->instance_indices.data) || n_tlas == 0)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: instances / tlas_nodes / instance_indices layout");
+    if (4 * n_inst != sd.inst_ranges.size() || d->uniform.instance_count != sd.uniform.instance_count ||
+        d->uniform.light_count != sd.uniform.light_count)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: instance or light count changed (upload the scene)");
+    const auto *inst = static_cast<const pulse_mesh_instance *>(d->instances.data);
+    for (size_t i = 0; i < n_inst; ++i) {
+        if (inst[i].triangle_offset != sd.inst_ranges[4 * i] || inst[i].triangle_count != sd.inst_ranges[4 * i + 1] ||
+            inst[i].index_offset != sd.inst_ranges[4 * i + 2] || inst[i].node_offset != sd.inst_ranges[4 * i + 3])
+            PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: instance %zu changed mesh (upload the scene)", i);
+        if (inst[i].material_index >= sd.n_materials)
+            PULSE_FAIL(PULSE_E_LAYOUT, "instance %zu: material_index %u >= %u", i, inst[i].material_index, sd.n_materials);
+    }
+    if (d->light_indices.bytes != sd.light_indices.size() * sizeof(pulse_light_data_index) ||
+        (d->light_indices.bytes &&
+         std::memcmp(d->light_indices.data, sd.light_indices.data(), d->light_indices.bytes) != 0))
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: the light table changed (upload the scene)");
+    const auto *tlas = static_cast<const pulse_tlas_node *>(d->tlas_nodes.data);
+    const auto *iidx = static_cast<const uint32_t *>(d->instance_indices.data);
+    uint32_t tlas_depth = 0;
+    bool empty_tlas = tlas[0].a_or_first_instance == 0 && tlas[0].instance_count == 0;
+    if (!empty_tlas) {
+        if (!walk_tlas(tlas, n_tlas, n_iidx, tlas_depth))
+            PULSE_FAIL(PULSE_E_LAYOUT, "tlas_nodes: child index or instance range out of range");
+        for (size_t k = 0; k < n_iidx; ++k)
+            if (iidx[k] >= n_inst) PULSE_FAIL(PULSE_E_LAYOUT, "instance_indices[%zu] = %u >= %zu", k, iidx[k], n_inst);
+    }
+    const uint32_t need = tlas_depth + sd.blas_depth + 1;
+    if (pick_stack(need, 0) == 0 && pick_stack(need, 1) == 0)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
+                   tlas_depth, sd.blas_depth);
+    auto coord_ok = [](float c) {
+        float a = std::fabs(c);
+        return c == 0.0f || (a >= 0x1p-56f && a <= 0x1p40f);
+    };
+    bool fast_div = sd.fast_div_blas;
+    for (size_t k = 0; k < n_tlas && fast_div; ++k)
+        for (int c = 0; c < 3; ++c) fast_div = fast_div && coord_ok(tlas[k].aabb_min[c]) && coord_ok(tlas[k].aabb_max[c]);
+    auto up = [&](int b, const void *src, size_t bytes) -> int {
+        int r = sd.buf[b].ensure(bytes);
+        if (r) return r;
+        if (bytes) HIP_TRY(hipMemcpy(sd.buf[b].ptr, src, bytes, hipMemcpyHostToDevice));
+        return PULSE_OK;
+    };
+    int rc;
+    if ((rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) ||
+        (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes))) {
+        sd.valid = false;  // partially replaced: unusable until the next upload
+        return rc;
+    }
+    sd.tlas_depth = tlas_depth;
+    sd.stack_need = need;
+    sd.fast_div = fast_div;
+    sd.generation += 1;
     return PULSE_OK;
 }
 
@@ -550,6 +626,26 @@ int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials
     ctx->cur->any_metallic = false;
     for (uint32_t k = 0; k < count; ++k)
         if (materials[k].metallic != 0.0f) ctx->cur->any_metallic = true;
+    return PULSE_OK;
+}
+
+int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse_scene_desc *desc) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_update_instances: null ctx");
+    if (scene && scene->ctx != ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_update_instances: scene of another context");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SceneData *sd = scene ? &scene->data : &ctx->own;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));  // launches in flight read the old buffers
+    if ((rc = scene_update_instances(*sd, desc)) != PULSE_OK) return rc;
+    if (ctx->cur == sd) {
+        uint32_t st = pick_stack(sd->stack_need, ctx->loop_variant);
+        if (st == 0) {
+            ctx->cur = nullptr;
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep for the current traversal loop");
+        }
+        ctx->stack_entries = st;
+        ctx->generation += 1;
+    }
     return PULSE_OK;
 }
 

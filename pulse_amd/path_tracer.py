@@ -71,6 +71,14 @@ class Device:
         """A device-resident scene beside the uploaded one (pulse_scene_create)."""
         return DeviceScene(self, scene)
 
+    def update_instances(self, scene: SceneBuffers, target: "DeviceScene" = None) -> None:
+        """Per-frame instance update (pulse_scene_update_instances) of the
+        uploaded scene, or of `target`: only instances, TLAS and instance
+        indices travel to the device."""
+        d = scene.desc()
+        N.check(N.lib().pulse_scene_update_instances(self._ctx, target.handle if target is not None else None,
+                                                     C.byref(d)))
+
     def bind_scene(self, scene: "DeviceScene" = None) -> None:
         """Makes `scene` (None: the uploaded scene) the one pulse_trace uses."""
         N.check(N.lib().pulse_scene_bind(self._ctx, scene.handle if scene is not None else None))

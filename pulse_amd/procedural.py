@@ -230,7 +230,7 @@ def cornell_lights(strengths=(2.0, 40.0, 8.0), seed: int = 0, threads: int = 8):
 
 def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800, n_emissive: int = 32,
                     box: float = 100.0, radius=(1.5, 4.0), roughness=(0.05, 1.0), emission: float = 8.0,
-                    seed: int = 2, threads: int = 8, n_materials: int | None = None):
+                    seed: int = 2, threads: int = 8, n_materials: int | None = None, keep_builder: bool = False):
     """Config-2 style scene; returns (SceneBuffers, camera factory).
 
     Mesh k gets material k % n_materials (default: one material per mesh);
@@ -258,7 +258,7 @@ def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800,
         m[:3, 3] = tr[i]
         mesh = meshes[i % n_meshes]
         sb.add_instance(mesh, m_emit if i in emissive else mats[(i % n_meshes) % n_materials], m)
-    scene = sb.build(threads)
+    scene = sb.build(threads, keep_builder=keep_builder)
 
     def camera(width=1920, height=1080):
         return make_camera(width, height, eye=(0.0, 0.0, 1.3 * box), target=(0.0, 0.0, 0.0))

@@ -55,6 +55,7 @@ class SceneBuffers:
     light_count: int
     buffers: dict
     info: dict = field(default_factory=dict)
+    builder: object = None  # the SceneBuilder that made them, when kept for instance updates
 
     def desc(self) -> N.SceneDesc:
         d = N.SceneDesc()
@@ -102,14 +103,28 @@ class SceneBuilder:
         cm = np.ascontiguousarray(np.asarray(transform, dtype=np.float32).T.reshape(16))  # column-major
         N.check(N.lib().pulse_sb_add_instance(self._h, mesh_id, material_id, cm.ctypes.data))
 
-    def build(self, threads: int = 8) -> SceneBuffers:
+    def build(self, threads: int = 8, keep_builder: bool = False) -> SceneBuffers:
         N.check(N.lib().pulse_sb_build(self._h, threads))
+        return self._read(N.BUFFER_NAMES, None, keep_builder)
+
+    def set_instance_transform(self, instance_id: int, transform: np.ndarray) -> None:
+        """New object->world matrix (row-major numpy) for an instance."""
+        cm = np.ascontiguousarray(np.asarray(transform, dtype=np.float32).T.reshape(16))
+        N.check(N.lib().pulse_sb_set_instance_transform(self._h, instance_id, cm.ctypes.data))
+
+    def update_instances(self, prev: SceneBuffers) -> SceneBuffers:
+        """Instance records, light tables and TLAS for the current transforms
+        (pulse_sb_update_instances); the mesh buffers are shared with `prev`."""
+        N.check(N.lib().pulse_sb_update_instances(self._h))
+        return self._read(INSTANCE_BUFFERS, prev, prev.builder is not None)
+
+    def _read(self, names, prev, keep_builder) -> SceneBuffers:
         d = N.SceneDesc()
         N.check(N.lib().pulse_sb_get_desc(self._h, C.byref(d)))
         info = N.BuildInfo()
         N.check(N.lib().pulse_sb_get_info(self._h, C.byref(info)))
-        bufs = {}
-        for name in N.BUFFER_NAMES:
+        bufs = dict(prev.buffers) if prev is not None else {}
+        for name in names:
             v = getattr(d, name)
             dt = N.BUFFER_DTYPES[name]
             n = v.bytes // dt.itemsize
@@ -119,7 +134,14 @@ class SceneBuilder:
             else:
                 bufs[name] = np.zeros(0, dtype=dt)
         return SceneBuffers(d.uniform.instance_count, d.uniform.light_count, bufs,
-                            {f: getattr(info, f) for f, _ in N.BuildInfo._fields_})
+                            {f: getattr(info, f) for f, _ in N.BuildInfo._fields_},
+                            self if keep_builder else None)
+
+
+# The buffers an instance update rewrites (plus the light tables, which it
+# recomputes identically when only transforms change).
+INSTANCE_BUFFERS = ("instances", "tlas_nodes", "instance_indices", "light_emission_strength_cdf",
+                    "light_triangle_area_cdfs", "light_mesh_areas", "light_indices")
 
 
 # ---------------------------------------------------------------------------
