@@ -338,6 +338,19 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 
+/* Replaces the path tracer's upscaling pass (src/path_tracer/pt_upscaling/
+ * node.rs:29-97, pt_upscaling.wgsl:39-50): draws the rgba32f accumulation
+ * buffer (src_w x src_h) into a dst_w x dst_h target of the given format,
+ * nearest texel at each target pixel centre, clamped to the edge (wgpu's
+ * default sampler), converted as a target of that format stores it.  Both
+ * buffers are device memory; asynchronous on the context stream. */
+#define PULSE_PRESENT_RGBA32F 0
+#define PULSE_PRESENT_RGBA16F 1     /* the HDR view target: IEEE half, round to nearest even */
+#define PULSE_PRESENT_RGBA8_SRGB 2  /* linear -> sRGB on rgb, 8-bit, round to nearest        */
+#define PULSE_PRESENT_RGBA8_UNORM 3
+int pulse_present(pulse_ctx *ctx, const float *accum_rgba, uint32_t src_w, uint32_t src_h, void *dst,
+                  uint32_t dst_w, uint32_t dst_h, int format);
+
 /* ------------------------------------------------------------------------ */
 /* Batched form: many views rendered by one captured launch sequence         */
 /* ------------------------------------------------------------------------ */

@@ -36,6 +36,10 @@ FLAG_STAGE_TIMING = 2
 FLAG_HORIZON_CLAMP = 4
 FLAG_ADJOINT_REPARAM = 8
 FLAG_LIGHT_POWER = 16
+PRESENT_RGBA32F = 0
+PRESENT_RGBA16F = 1
+PRESENT_RGBA8_SRGB = 2
+PRESENT_RGBA8_UNORM = 3
 GRAD_PER_MATERIAL = 7
 OPT_TRAVERSAL_LOOP = 1
 OPT_STACK_ENTRIES = 2
@@ -240,6 +244,7 @@ SIGNATURES = [
     ("pulse_get_stats", _I, [_P, C.POINTER(Stats)]),
     ("pulse_set_option", _I, [_P, _I, C.c_int64]),
     ("pulse_image_loss", _I, [_P, _P, _P, _U32, C.POINTER(C.c_double)]),
+    ("pulse_present", _I, [_P, _P, _U32, _U32, _P, _U32, _U32, _I]),
     ("pulse_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, _P]),
     ("pulse_forward_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, C.POINTER(C.c_double)]),
     ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),

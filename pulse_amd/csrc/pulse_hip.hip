@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "device/adjoint.h"
+#include "device/present.h"
 #include "device/wavefront.h"
 #include "host/common.h"
 #include "host/context.h"
@@ -820,6 +821,24 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.ms_shade = ms_total - ms_primary - ms_ext_bounce - ms_shadow;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
+    return PULSE_OK;
+}
+
+int pulse_present(pulse_ctx *ctx, const float *accum_rgba, uint32_t src_w, uint32_t src_h, void *dst, uint32_t dst_w,
+                  uint32_t dst_h, int format) {
+    if (!ctx || !accum_rgba || !dst) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_present: null argument");
+    if (src_w == 0 || src_h == 0 || dst_w == 0 || dst_h == 0)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_present: empty image");
+    if (format < PULSE_PRESENT_RGBA32F || format > PULSE_PRESENT_RGBA8_UNORM)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_present: unknown format %d", format);
+    const uint64_t n = static_cast<uint64_t>(dst_w) * dst_h;
+    if (n > 0xffffffffull || static_cast<uint64_t>(src_w) * src_h > 0xffffffffull)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_present: image too large");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_present, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const float4 *>(accum_rgba), src_w, src_h, dst, dst_w, dst_h, format);
+    HIP_TRY(hipGetLastError());
     return PULSE_OK;
 }
 

@@ -309,6 +309,17 @@ class PulsePathTracerNode:
         out = np.zeros((self.height, self.width, 4), np.float32)
         return self.target.copy_to(out)
 
+    def present(self, width: int, height: int, fmt: int = N.PRESENT_RGBA16F) -> np.ndarray:
+        """The upscaling node's draw of the accumulation buffer into a
+        width x height view target (pulse_present), read back to the host."""
+        item = {N.PRESENT_RGBA32F: 16, N.PRESENT_RGBA16F: 8}.get(fmt, 4)
+        dt = {N.PRESENT_RGBA32F: np.float32, N.PRESENT_RGBA16F: np.float16}.get(fmt, np.uint8)
+        dst = self.device.alloc(width * height * item)
+        N.check(N.lib().pulse_present(self.device.handle, self.target.ptr, self.width, self.height, dst.ptr, width,
+                                      height, fmt))
+        self.device.sync()
+        return dst.copy_to(np.zeros((height, width, 4), dt))
+
 
 class PulsePlugin:
     """src/lib.rs:13-19: owns the scene side (device context + scene upload)."""
