@@ -308,6 +308,28 @@ int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
 int pulse_trace(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params,
                 float *accum_rgba);
 
+/* The realtime GI pass (src/pulse/gi.wgsl:56-126, dispatched by the GI
+ * plugin): paths start at the caller's G-buffer instead of camera rays.
+ * Device arrays of width*height float4 each (w unused): world position,
+ * world normal, base colour, emissive -- what the reference decodes from
+ * Bevy's deferred prepass (pbr_input_from_deferred_gbuffer).  Per pixel: RNG
+ * seeded with pixel_index * 5817321 and run on across params->spp samples
+ * (the reference uses 2); each sample scatters (cosine) from the G-buffer
+ * point with Lambert NEE there, then up to params->max_depth bounces (the
+ * reference uses 5) with NEE and Russian roulette at every depth; gi_rgba
+ * (device, width*height*4) = the average colour, alpha 1 (not accumulated
+ * across calls).  params: spp, max_depth, flags (PULSE_FLAG_LIGHT_POWER);
+ * nee_eval must be PULSE_NEE_LAMBERT; mode, rr_start_depth and
+ * accumulation_count are not used. */
+typedef struct {
+    const float *position;
+    const float *normal;
+    const float *base_color;
+    const float *emissive;
+} pulse_gbuffer;
+int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gbuffer, uint32_t width, uint32_t height,
+                        const pulse_params *params, float *gi_rgba);
+
 /* Closest hit for a batch of world-space rays (trace_ray,
  * utilities.wgsl:112-114 -> traverse_tlas 127-188 -> traverse_blas
  * 259-326).  rays/hits are device arrays of n records. */

@@ -200,6 +200,33 @@ double oracle_render_f32(oracle_scene *sc, const pulse_camera *cam, const pulse_
     return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
 
+// The realtime GI pass over a G-buffer (host arrays of W*H float4: world
+// position, world normal, base colour, emissive); out: W*H float4 = the
+// averaged colour, alpha 1.  Returns ms.
+double oracle_render_gi_f32(oracle_scene *sc, const pulse_params *prm, const float *pos, const float *nrm,
+                            const float *base, const float *emis, uint32_t width, uint32_t height, float *out,
+                            int threads, uint64_t *counters) {
+    const Scene &s = S(sc);
+    std::vector<Counters> pc(std::max(1, threads));
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_for(height, threads, [&](uint32_t y, int tid) {
+        DecisionTape tape;
+        for (uint32_t x = 0; x < width; ++x) {
+            const size_t i = static_cast<size_t>(y) * width + x;
+            V3<float> c = trace_gi_pixel<float>(s, *prm, static_cast<uint32_t>(i), load3<float>(pos + 4 * i),
+                                                load3<float>(nrm + 4 * i), load3<float>(base + 4 * i),
+                                                load3<float>(emis + 4 * i), pc[tid], tape);
+            out[4 * i] = c.x;
+            out[4 * i + 1] = c.y;
+            out[4 * i + 2] = c.z;
+            out[4 * i + 3] = 1.0f;
+        }
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (counters) export_counters(pc, counters);
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
 // Renders the same samples in f32 and in f64.  replay != 0: the f64 path takes
 // every branch decision from the f32 path (decision replay); flip_stats[0]
 // counts decisions the f64 arithmetic would have taken differently and

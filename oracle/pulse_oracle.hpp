@@ -1105,6 +1105,56 @@ V3<R> trace_path(const Scene &s, const pulse_camera &cam, const pulse_params &pr
     return color;
 }
 
+// The realtime GI kernel `gi` (src/pulse/gi.wgsl:56-126) for one pixel whose
+// first hit comes from the G-buffer (position, normal, base colour,
+// emissive): the RNG is seeded once per pixel with pixel_index * 5817321 and
+// runs on across the `spp` samples; per sample a cosine scatter from the
+// G-buffer point (2 draws), Lambert NEE there (4), then up to max_depth
+// bounces of: closest hit, NEE, throughput *= base colour, Russian roulette
+// at every depth (1 draw), cosine scatter (2); the output is the running sum
+// of colour * (1 / spp).
+template <class R>
+V3<R> trace_gi_pixel(const Scene &s, const pulse_params &prm, uint32_t pixel_index, V3<R> pos, V3<R> nrm,
+                     V3<R> base, V3<R> emissive, Counters &c, DecisionTape &tape) {
+    uint32_t rng = pixel_index * 5817321u;
+    const bool have_lights = s.uniform.light_count > 0;
+    const bool light_power = (prm.flags & PULSE_FLAG_LIGHT_POWER) != 0;
+    const uint32_t spp = std::max(prm.spp, 1u);
+    const R spp_inv = static_cast<R>(1.0f / static_cast<float>(spp));
+    V3<R> out{R(0), R(0), R(0)};
+    for (uint32_t sample = 0; sample < spp; ++sample) {
+        R a0 = static_cast<R>(rand_f(rng));
+        R a1 = static_cast<R>(rand_f(rng));
+        Ray<R> ray;
+        ray.dir = sample_cosine_hemisphere(nrm, a0, a1);
+        ray.origin = pos + R(0.001f) * nrm;
+        ray.t = sentinel<R>();
+        V3<R> direct{R(0), R(0), R(0)};
+        if (have_lights) direct = sample_direct_light(s, pos, nrm, base, rng, c, tape, light_power);
+        V3<R> color = emissive + direct;
+        V3<R> throughput = base;
+        c.primary_rays++;
+        for (uint32_t depth = 0; depth < prm.max_depth; ++depth) {
+            { CatScope scope(c, 1); trace_closest(s, ray, c, tape); }
+            if (ray.t >= sentinel<R>()) break;
+            HitShading<R> h = shade_hit(s, ray);
+            MaterialR<R> m = scene_material<R>(s, h.material_index);
+            V3<R> dl{R(0), R(0), R(0)};
+            if (have_lights) dl = sample_direct_light(s, h.position, h.normal, m.base, rng, c, tape, light_power);
+            color = color + throughput * dl;
+            throughput = throughput * m.base;
+            if (!russian_roulette(throughput, rng, tape)) break;
+            R b0 = static_cast<R>(rand_f(rng));
+            R b1 = static_cast<R>(rand_f(rng));
+            ray.dir = sample_cosine_hemisphere(h.normal, b0, b1);
+            ray.origin = h.position + R(0.001f) * h.normal;
+            ray.t = sentinel<R>();
+        }
+        out = out + color * spp_inv;
+    }
+    return out;
+}
+
 // Running mean, path_tracer.wgsl:116-123
 template <class R>
 inline V3<R> accumulate(V3<R> old, V3<R> color, uint32_t n) {

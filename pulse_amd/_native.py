@@ -223,6 +223,11 @@ class BatchConfig(C.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_ if not f.startswith("_")}
 
 
+class GBuffer(C.Structure):
+    """pulse_gbuffer: device pointers of width*height float4 arrays."""
+    _fields_ = [("position", C.c_void_p), ("normal", C.c_void_p), ("base_color", C.c_void_p), ("emissive", C.c_void_p)]
+
+
 SIGNATURES = [
     ("pulse_last_error", C.c_char_p, []),
     ("pulse_version", C.c_char_p, []),
@@ -245,6 +250,7 @@ SIGNATURES = [
     ("pulse_set_option", _I, [_P, _I, C.c_int64]),
     ("pulse_image_loss", _I, [_P, _P, _P, _U32, C.POINTER(C.c_double)]),
     ("pulse_present", _I, [_P, _P, _U32, _U32, _P, _U32, _U32, _I]),
+    ("pulse_trace_gbuffer", _I, [_P, C.POINTER(GBuffer), _U32, _U32, C.POINTER(Params), _P]),
     ("pulse_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, _P]),
     ("pulse_forward_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, C.POINTER(C.c_double)]),
     ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),

@@ -315,6 +315,7 @@ __global__ void __launch_bounds__(256) k_b_shade(BatchArgs b, int q, int k) {
     fa.rr_start = b.rr_start;
     fa.horizon_clamp = b.horizon_clamp;
     fa.light_power = b.light_power;
+    fa.gi = 0;
     fa.accumulation_count = 0;
     for (uint32_t g = next_chunk(b, wslot, wave_global, true); g < cm.total;
          g = next_chunk(b, wslot, wave_global, false)) {

@@ -28,6 +28,7 @@ struct FrameArgs {
     uint32_t accumulation_count;
     uint32_t horizon_clamp;   // PULSE_FLAG_HORIZON_CLAMP
     uint32_t light_power;     // PULSE_FLAG_LIGHT_POWER
+    uint32_t gi;              // the G-buffer GI pass (every trace index is a loop-body hit)
 };
 
 // Appends `valid` items of a wave to a queue: returns this lane's slot.

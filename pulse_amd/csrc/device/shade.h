@@ -28,7 +28,7 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
     const bool last = static_cast<uint32_t>(k) + 1u >= fa.max_traces;
     const bool lights = sc.light_count > 0u;
     if (fa.mode == PULSE_MODE_REF_NEE_LAMBERT) {
-        if (k == 0) {
+        if (k == 0 && !fa.gi) {
             // first hit stands in for the G-buffer sample
             // (path_tracer.wgsl:143-153)
             float a0 = rand_f(rng);
@@ -64,10 +64,12 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
             }
             // without lights the reference adds throughput * 0 too
             C = add(C, mul(T, mk(0.0f, 0.0f, 0.0f)));
-            if (!last) {
+            // the GI pass (gi.wgsl:112-120) draws roulette and scatter at the
+            // last depth too: its RNG runs on into the next sample
+            if (!last || fa.gi) {
                 T = mul(T, m.base);
                 bool alive = true;
-                if (static_cast<uint32_t>(k - 1) >= fa.rr_start) {
+                if (fa.gi || static_cast<uint32_t>(k - 1) >= fa.rr_start) {
                     float p = max3(T);
                     if (rand_f(rng) > p) alive = false;
                     else T = scale(1.0f / p, T);
@@ -75,7 +77,7 @@ __device__ __forceinline__ ShadeOut shade_ray(const DevScene &sc, const FrameArg
                 if (alive) {
                     float b0 = rand_f(rng);
                     float b1 = rand_f(rng);
-                    out.emit_ray = true;
+                    out.emit_ray = !last;
                     out.next_d = sample_cosine_hemisphere(h.normal, b0, b1);
                     out.next_o = add(h.position, scale(0.001f, h.normal));
                 }

@@ -215,6 +215,81 @@ __global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, i
     }
 }
 
+// G-buffer GI pass (gi.wgsl:56-126), the part before the first trace: per
+// pixel, from its G-buffer sample, a cosine scatter ray and the NEE shadow
+// ray.  Sample 0 seeds the RNG with pixel_index * 5817321; later samples go
+// on from where the previous sample's path left it.
+struct GBufferArgs {
+    const float4 *position, *normal, *base_color, *emissive;
+    uint32_t npix;
+};
+__global__ void __launch_bounds__(kShadeBlock) k_gi_generate(DevScene sc, WfBuffers wb, FrameArgs fa, GBufferArgs g,
+                                                             uint32_t s) {
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pixel < g.npix;
+    ShadeOut so;
+    if (valid) {
+        uint32_t rng = s == 0u ? pixel * 5817321u : __float_as_uint(wb.state_t[pixel].w);
+        const float4 p4 = g.position[pixel], n4 = g.normal[pixel], b4 = g.base_color[pixel], e4 = g.emissive[pixel];
+        const V3 p = mk(p4.x, p4.y, p4.z), n = mk(n4.x, n4.y, n4.z);
+        Material m;
+        m.base = mk(b4.x, b4.y, b4.z);
+        m.emissive = mk(e4.x, e4.y, e4.z);
+        m.roughness = 0.5f;
+        m.reflectance = 0.5f;
+        m.metallic = 0.0f;
+        float a0 = rand_f(rng);
+        float a1 = rand_f(rng);
+        V3 scatter = sample_cosine_hemisphere(n, a0, a1);
+        V3 dl_vis = mk(0.0f, 0.0f, 0.0f);
+        const bool lights = sc.light_count > 0u;
+        if (lights) {
+            NeeSample ns = nee_sample(sc, p, n, m, neg(n), false, rng, fa.light_power != 0u);
+            so.emit_shadow = true;
+            so.sh_o = ns.origin;
+            so.sh_d = ns.dir;
+            so.sh_thr = ns.threshold;
+            dl_vis = ns.contrib;
+        }
+        so.sh_c = add(m.emissive, dl_vis);
+        V3 C = lights ? add(m.emissive, mk(0.0f, 0.0f, 0.0f)) : so.sh_c;
+        V3 T = m.base;
+        wb.state_t[pixel] = make_float4(T.x, T.y, T.z, __uint_as_float(rng));
+        wb.state_c[pixel] = make_float4(C.x, C.y, C.z, 0.0f);
+        so.emit_ray = fa.max_traces > 0u;
+        so.next_o = add(p, scale(0.001f, n));
+        so.next_d = scatter;
+    }
+    uint32_t rs = wave_append(so.emit_ray, &wb.ext_count[0]);
+    uint32_t ss = wave_append(so.emit_shadow, &wb.sh_count[kMaxTraces]);
+    if (so.emit_ray) {
+        wb.ray_o[0][rs] = make_float4(so.next_o.x, so.next_o.y, so.next_o.z, __uint_as_float(pixel));
+        wb.ray_d[0][rs] = make_float4(so.next_d.x, so.next_d.y, so.next_d.z, 0.0f);
+    }
+    if (so.emit_shadow) {
+        wb.sh_o[ss] = make_float4(so.sh_o.x, so.sh_o.y, so.sh_o.z, __uint_as_float(pixel));
+        wb.sh_d[ss] = make_float4(so.sh_d.x, so.sh_d.y, so.sh_d.z, so.sh_thr);
+        wb.sh_c[ss] = make_float4(so.sh_c.x, so.sh_c.y, so.sh_c.z, 0.0f);
+    }
+}
+
+// G-buffer GI pass output: the running sum of colour * (1 / spp), alpha 1
+// (gi.wgsl:122, 130).
+__global__ void __launch_bounds__(kShadeBlock) k_gi_accumulate(WfBuffers wb, float4 *out, uint32_t npix, uint32_t s,
+                                                               float spp_inv) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float4 c = wb.state_c[p];
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    if (s > 0u) {
+        const float4 o = out[p];
+        ox = o.x;
+        oy = o.y;
+        oz = o.z;
+    }
+    out[p] = make_float4(ox + c.x * spp_inv, oy + c.y * spp_inv, oz + c.z * spp_inv, 1.0f);
+}
+
 // accumulate: running mean (path_tracer.wgsl:116-123).
 __global__ void __launch_bounds__(kShadeBlock) k_accumulate(WfBuffers wb, float4 *accum, uint32_t npix, uint32_t n) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -736,6 +736,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     fa.rr_start = prm->rr_start_depth;
     fa.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     fa.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
+    fa.gi = 0;
     if (fa.light_power && ctx->cur->n_strength_cdf < ctx->cur->uniform.light_count)
         PULSE_FAIL(PULSE_E_LAYOUT, "pulse_trace: PULSE_FLAG_LIGHT_POWER needs one light_emission_strength_cdf entry "
                    "per light (%u for %u lights)", ctx->cur->n_strength_cdf, ctx->cur->uniform.light_count);
@@ -821,6 +822,73 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.ms_shade = ms_total - ms_primary - ms_ext_bounce - ms_shadow;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
+    return PULSE_OK;
+}
+
+int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width, uint32_t height,
+                        const pulse_params *prm, float *gi_rgba) {
+    if (!ctx || !gb || !prm || !gi_rgba || !gb->position || !gb->normal || !gb->base_color || !gb->emissive)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_gbuffer: null argument");
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_gbuffer: no scene uploaded");
+    if (width == 0 || height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_gbuffer: empty image");
+    if (prm->max_depth > static_cast<uint32_t>(kMaxTraces - 1))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_gbuffer: max_depth %u above %d", prm->max_depth, kMaxTraces - 1);
+    if (prm->nee_eval != PULSE_NEE_LAMBERT)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_gbuffer: the GI pass evaluates NEE with the Lambert BRDF only");
+    const uint64_t npix64 = static_cast<uint64_t>(width) * height;
+    if (npix64 > 0xffffffffull / 4) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_gbuffer: image too large");
+    if ((prm->flags & PULSE_FLAG_LIGHT_POWER) && ctx->cur->n_strength_cdf < ctx->cur->uniform.light_count)
+        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_trace_gbuffer: PULSE_FLAG_LIGHT_POWER needs one light_emission_strength_cdf "
+                   "entry per light");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const uint32_t npix = static_cast<uint32_t>(npix64);
+    if ((rc = ensure_wavefront(ctx, npix)) != PULSE_OK) return rc;
+    WfBuffers wb = wf_buffers(ctx);
+    FrameArgs fa{};
+    fa.mode = PULSE_MODE_REF_NEE_LAMBERT;
+    fa.nee_ggx = 0;
+    fa.max_traces = prm->max_depth;
+    fa.rr_start = 0;
+    fa.horizon_clamp = 0;
+    fa.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
+    fa.gi = 1;
+    GBufferArgs g;
+    g.position = reinterpret_cast<const float4 *>(gb->position);
+    g.normal = reinterpret_cast<const float4 *>(gb->normal);
+    g.base_color = reinterpret_cast<const float4 *>(gb->base_color);
+    g.emissive = reinterpret_cast<const float4 *>(gb->emissive);
+    g.npix = npix;
+    const bool persistent = ctx->loop_variant == 1 && ctx->persistent > 0;
+    if (persistent && (rc = ensure_spill(ctx, npix)) != PULSE_OK) return rc;
+    const uint32_t grid = (npix + kShadeBlock - 1) / kShadeBlock;
+    const uint32_t spp = std::max(prm->spp, 1u);
+    const float spp_inv = 1.0f / static_cast<float>(spp);
+    auto shadow = [&](int k) -> int {
+        if (persistent) return launch_queue(ctx, wb, npix, true, 0, k);
+        PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, npix, k);
+        return PULSE_OK;
+    };
+    for (uint32_t s = 0; s < spp; ++s) {
+        HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * kCounterWords, ctx->stream));
+        hipLaunchKernelGGL(k_gi_generate, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, g, s);
+        if ((rc = shadow(kMaxTraces)) != PULSE_OK) return rc;  // NEE at the G-buffer points
+        int q = 0;
+        for (uint32_t k = 0; k < fa.max_traces; ++k) {
+            if (persistent) {
+                if ((rc = launch_queue(ctx, wb, npix, false, q, static_cast<int>(k))) != PULSE_OK) return rc;
+            } else {
+                PULSE_STACK_DISPATCH(launch_extend, ctx, wb, npix, q, static_cast<int>(k));
+            }
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, q,
+                               static_cast<int>(k));
+            if ((rc = shadow(static_cast<int>(k))) != PULSE_OK) return rc;
+            q ^= 1;
+        }
+        hipLaunchKernelGGL(k_gi_accumulate, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, wb,
+                           reinterpret_cast<float4 *>(gi_rgba), npix, s, spp_inv);
+    }
+    HIP_TRY(hipGetLastError());
     return PULSE_OK;
 }
 

@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
             ("oracle_grad_fd_ex", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
                                             C.c_double, _P, _P, C.c_int, C.c_int]),
             ("oracle_strength_pick_array", _U32, [_P, _U32, C.c_float]),
+            ("oracle_render_gi_f32", C.c_double, [_P, C.POINTER(N.Params), _P, _P, _P, _P, _U32, _U32, _P, C.c_int,
+                                                  _P]),
             ("oracle_rand_u", _U32, [C.POINTER(_U32)]),
             ("oracle_rand_f", C.c_float, [C.POINTER(_U32)]),
             ("oracle_pixel_seed", _U32, [_U32, _U32]),
@@ -127,6 +129,16 @@ class OracleScene:
         ms = lib().oracle_render_f32(self.h, C.byref(camera), C.byref(params), accum.ctypes.data, threads,
                                      cnt.ctypes.data, rows[0], rows[1])
         return accum, parse_counters(cnt), ms
+
+    def render_gi(self, gbuffer, params, threads=8):
+        """The realtime GI pass over a G-buffer: dict of (H, W, 4) float32
+        arrays position / normal / base_color / emissive."""
+        arrs = [np.ascontiguousarray(gbuffer[k], np.float32) for k in ("position", "normal", "base_color", "emissive")]
+        h, w = arrs[0].shape[:2]
+        out = np.zeros((h, w, 4), np.float32)
+        lib().oracle_render_gi_f32(self.h, C.byref(params), *[a.ctypes.data for a in arrs], w, h, out.ctypes.data,
+                                   threads, None)
+        return out
 
     def grad_fd(self, camera, params, target, mat_params, ids, h=1e-6, threads=8, frozen=False):
         """dL/dparam[ids] by f64 central differences along the f32 decisions;
