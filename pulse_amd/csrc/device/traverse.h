@@ -231,7 +231,7 @@ struct Traversal {
                 node_count = __float_as_uint(a_hi.w);
                 if (db != kTFar) {
                     if (sp < stk.capacity) {
-                        stk.put(sp, b);
+                        stk.put(sp, pack_entry(b, __float_as_uint(b_lo.w), __float_as_uint(b_hi.w)));
                         sp += 1;
                     } else {
                         atomicOr(sc.error_flags, kErrStackOverflow);
@@ -244,7 +244,7 @@ struct Traversal {
         if (do_pop) {
             if (sp > sp_base) {
                 sp -= 1;
-                load_node(nodes, stk.get(sp));
+                pop_node(stk.get(sp));
                 return true;
             }
             if (!in_blas) return false;  // TLAS stack empty: done
@@ -326,8 +326,24 @@ struct Traversal {
         sp_base = 0;
         if (sp == 0) return false;
         sp -= 1;
-        load_node(nodes, stk.get(sp));
+        pop_node(stk.get(sp));
         return true;
+    }
+
+    // Stack entries: a pushed node carries its (first, count) words when
+    // they fit, (count + 1) << 27 | first (first < 2^27, count < 31), so a
+    // pop needs no memory access; otherwise its index (top 5 bits zero: the
+    // host keeps node arrays below 2^27 entries) and the pop loads them.
+    __device__ __forceinline__ static uint32_t pack_entry(uint32_t index, uint32_t first, uint32_t count) {
+        return (first < (1u << 27) && count < 31u) ? ((count + 1u) << 27) | first : index;
+    }
+    __device__ __forceinline__ void pop_node(uint32_t e) {
+        if (e >> 27) {
+            node_first = e & ((1u << 27) - 1u);
+            node_count = (e >> 27) - 1u;
+        } else {
+            load_node(nodes, e);
+        }
     }
 
     // The child-pair index / first primitive and the primitive count of node

@@ -297,6 +297,9 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = count_of(d->light_indices, sizeof(pulse_light_data_index), "light_indices", n_light)))
         return rc;
     if (n_tlas == 0) PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: tlas_nodes is empty (needs at least the root)");
+    if (n_blas >= (1u << 27) || n_tlas >= (1u << 27))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_scene_upload: node arrays of 2^27 entries or more (traversal stack "
+                   "entries hold node indices in 27 bits)");
     if (d->uniform.instance_count > n_inst)
         PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_upload: instance_count %u > %zu instances", d->uniform.instance_count, n_inst);
     if (d->uniform.light_count > n_light || d->uniform.light_count > n_area)
@@ -438,6 +441,7 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
         d->instance_indices.bytes % 4 || (d->instances.bytes && !d->instances.data) || !d->tlas_nodes.data ||
         (d->instance_indices.bytes && !d->instance_indices.data) || n_tlas == 0)
         PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: instances / tlas_nodes / instance_indices layout");
+    if (n_tlas >= (1u << 27)) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_scene_update_instances: TLAS of 2^27 nodes or more");
     if (4 * n_inst != sd.inst_ranges.size() || d->uniform.instance_count != sd.uniform.instance_count ||
         d->uniform.light_count != sd.uniform.light_count)
         PULSE_FAIL(PULSE_E_LAYOUT, "pulse_scene_update_instances: instance or light count changed (upload the scene)");
