@@ -15,10 +15,17 @@
 // (decision replay) and the number of decisions that would have flipped is
 // counted separately.
 //
-// Pinning (see tests/test_oracle_*.py): PCG known answers (closed form),
-// BVH closest hit == brute force over all instanced triangles, sampler
-// statistics against their densities, GGX closed forms, FP32 vs FP64
-// agreement under decision replay, NEE against a closed-form emitter case.
+// Pinning (see tests/test_oracle_*.py and the other CPU tests): PCG known
+// answers (closed form), BVH closest hit == brute force over all instanced
+// triangles, sampler statistics against their densities, GGX closed forms,
+// FP32 vs FP64 agreement under decision replay, NEE against a closed-form
+// emitter case, the emission-strength pick == clamped upper bound with an
+// unchanged image mean, the GI pass against emission-only and closed-form
+// direct-light cases, the gradient by step independence, Euler's identity
+// and free-decision differences.  Parity partially pinned: the GGX NEE
+// evaluator (sample_direct_light_ggx, dead code in the reference, with an
+// unnormalised wi) has no closed form here; it is checked only through
+// f32/f64 replay agreement and GPU bit-exactness.
 #pragma once
 
 #include <algorithm>
