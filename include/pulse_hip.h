@@ -343,6 +343,11 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 /* Tuning options of a context (defaults are the measured best on MI355X). */
 #define PULSE_OPT_TRAVERSAL_LOOP 1 /* 0: nested TLAS/BLAS loops as in the reference, */
                                    /* 1: one loop over both levels (default)         */
+                                   /* 2: 4-wide BVHs collapsed at upload from the     */
+                                   /*    reference's BVHs: same leaves, fewer steps;  */
+                                   /*    among hits with equal t (or within rounding  */
+                                   /*    of each other) the first found is kept, so   */
+                                   /*    such ties may resolve differently            */
 #define PULSE_OPT_STACK_ENTRIES 2  /* read-only: LDS traversal stack entries in use  */
 #define PULSE_OPT_FAST_DIVISION 3  /* read-only: 1 if the scene's coordinates allow   */
                                    /* the exact reciprocal-multiply slab division     */

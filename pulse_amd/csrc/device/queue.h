@@ -5,16 +5,19 @@
 
 #include "pt_device.h"
 #include "traverse.h"
+#include "wide.h"
 
 namespace pt {
 
 constexpr int kMaxTraces = 64;
 
 // Traversal variants: 0 = the reference's nested TLAS/BLAS loops,
-// 1 = single loop over both levels (traverse.h).  Same results.
+// 1 = single loop over both levels (traverse.h), same results; 2 = the
+// 4-wide BVHs (wide.h), same results up to the order among exactly tied hits.
 template <int LOOP, bool ANY_HIT>
 __device__ __forceinline__ bool trace_any(const DevScene &sc, V3 o, V3 d, float &t, uint32_t &inst, uint32_t &tri,
                                           float &u, float &v, float threshold, LdsStack &stk) {
+    if (LOOP == 2) return trace_wide<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
     if (LOOP == 1) return trace_unified<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
     return traverse_tlas<ANY_HIT>(sc, o, d, t, inst, tri, u, v, threshold, stk);
 }

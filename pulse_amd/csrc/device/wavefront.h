@@ -15,6 +15,8 @@
 // does not depend on queue order.
 #pragma once
 
+#include <type_traits>
+
 #include "pt_device.h"
 #include "queue.h"
 #include "shade.h"
@@ -150,9 +152,10 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
 //
 // The top LDS_ENTRIES entries of each lane's stack live in LDS, the rest (up
 // to `capacity`) in `spill`, one column of capacity - LDS_ENTRIES words per
-// lane of the grid.
-template <int LDS_ENTRIES, bool SHADOW, int TPS>
-__global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill,
+// lane of the grid.  LOOP 1 walks the reference's binary BVHs, LOOP 2 the
+// 4-wide ones (wide.h).
+template <int LDS_ENTRIES, bool SHADOW, int TPS, int LOOP = 1>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LOOP == 2 ? 5 : 6))) k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill,
                                                     uint32_t gather, int capacity, uint32_t *spill) {
     __shared__ uint32_t smem[LDS_ENTRIES * 64];
     LdsStack stk{smem + threadIdx.x, 64, capacity};
@@ -164,7 +167,7 @@ __global__ void __launch_bounds__(64) k_trace_queue(DevScene sc, WfBuffers wb, i
     const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
     uint32_t *head = SHADOW ? &wb.sh_head[k] : &wb.ext_head[k];
     const uint32_t lane = __lane_id();
-    Traversal<SHADOW, TPS> tr;
+    std::conditional_t<LOOP == 2, WideTraversal<SHADOW, TPS>, Traversal<SHADOW, TPS>> tr;
     bool alive = false;
     bool exhausted = false;  // wave-uniform: the queue has no rays left
     uint32_t slot = 0, pixel = 0;

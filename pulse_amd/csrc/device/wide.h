@@ -1,0 +1,320 @@
+// wide.h -- two-level traversal over the 4-wide BVHs derived at upload
+// (host/wide_bvh.h has the node layout).
+//
+// Same interface and the same closest-hit / occlusion definitions as
+// Traversal (traverse.h), fewer and wider steps: one step fetches one
+// 112-byte node, tests its four child boxes, visits the nearest hit child
+// next and pushes the others (farthest first).  The leaves, the triangle
+// slots, the triangle test, the per-instance object-space closest hit and its
+// world-t conversion are the reference's (utilities.wgsl:259-326); only the
+// order in which leaves are reached differs from the reference's binary
+// near-first order.  That order decides which of two hits with exactly equal
+// t is kept (the first found), and a box whose computed entry distance is
+// rounded above a hit inside it; the box test here is padded by 2^-20 of the
+// distances so that no box holding a hit below the current bound is skipped.
+#pragma once
+
+#include "pt_device.h"
+#include "traverse.h"
+
+namespace pt {
+
+constexpr uint32_t kWideEmpty = 0xffffffffu;
+
+// Origin, direction and clamped reciprocal direction (components below 2^-80
+// in magnitude count as 2^-80: the slab of a parallel axis stays finite).
+struct WideRay {
+    V3 o, d, r;
+};
+
+__device__ __forceinline__ float wide_rcp(float v) {
+    return 1.0f / (fabsf(v) < 0x1p-80f ? copysignf(0x1p-80f, v) : v);
+}
+__device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
+    WideRay w;
+    w.o = o;
+    w.d = d;
+    w.r = mk(wide_rcp(d.x), wide_rcp(d.y), wide_rcp(d.z));
+    return w;
+}
+
+// Padded slab test of one child box: its entry distance, or kTFar when the
+// box is missed, behind the origin or at or beyond `bound`.
+__device__ __forceinline__ float wide_slab(const WideRay &w, float bound, float lx, float hx, float ly, float hy,
+                                           float lz, float hz) {
+    float x0 = (lx - w.o.x) * w.r.x, x1 = (hx - w.o.x) * w.r.x;
+    float y0 = (ly - w.o.y) * w.r.y, y1 = (hy - w.o.y) * w.r.y;
+    float z0 = (lz - w.o.z) * w.r.z, z1 = (hz - w.o.z) * w.r.z;
+    float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+    float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+    tmin = tmin - fabsf(tmin) * 0x1p-20f;
+    tmax = tmax + fabsf(tmax) * 0x1p-20f;
+    return (tmax >= tmin && tmin < bound && tmax > 0.0f) ? tmin : kTFar;
+}
+
+__device__ __forceinline__ void wide_cswap(float &da, uint32_t &ea, float &db, uint32_t &eb) {
+    const bool s = db < da;
+    const float d = s ? db : da;
+    const uint32_t e = s ? eb : ea;
+    db = s ? da : db;
+    eb = s ? ea : eb;
+    da = d;
+    ea = e;
+}
+
+template <bool ANY_HIT, int TRIS_PER_STEP = 2>
+struct WideTraversal {
+    V3 wo, wd, world_r;  // world ray and its clamped reciprocal direction
+    float threshold, slack_thr, world_cull;
+    WideRay cur;       // ray of the current level
+    float cur_t;       // culling bound of the current level
+    const float4 *nodes;
+    bool in_blas;
+    int sp, sp_base;
+    int iter, iter_tlas;  // node visits at the current level; the TLAS's while in a BLAS
+    uint32_t leaf_next, leaf_end, inst, idx_off;
+    uint32_t tri_next, tri_end;  // triangles of the BLAS leaf in progress
+    bool want_advance;           // next: leave the BLAS and/or enter the next instance
+    float safe_t;
+    ObjHit h;
+    uint32_t entry;              // current entry (wide node or leaf range)
+    float rec_t, rec_u, rec_v;
+    uint32_t rec_inst, rec_tri;
+    bool occluded;
+
+    // false: nothing to traverse (empty TLAS); the result is already final.
+    __device__ __forceinline__ bool begin(const DevScene &sc, V3 o, V3 d, float t_max, float thr) {
+        rec_t = t_max;
+        rec_inst = 0;
+        rec_tri = 0;
+        rec_u = 0.0f;
+        rec_v = 0.0f;
+        occluded = false;
+        const uint32_t *root = reinterpret_cast<const uint32_t *>(sc.tlas_nodes);
+        if (root[3] == 0u && root[7] == 0u) return false;  // utilities.wgsl:129-131
+        wo = o;
+        wd = d;
+        world_r = wide_ray(o, d).r;
+        threshold = thr;
+        world_cull = rec_t;
+        slack_thr = 0.0f;
+        if (ANY_HIT) {
+            slack_thr = world_bound_slack(wo, threshold);
+            world_cull = threshold + slack_thr;
+        }
+        cur.o = wo;
+        cur.d = wd;
+        cur.r = world_r;
+        cur_t = world_cull;
+        nodes = sc.wide_tlas;
+        entry = 0u;
+        in_blas = false;
+        sp = 0;
+        sp_base = 0;
+        iter = 0;
+        iter_tlas = 0;
+        leaf_next = 0;
+        leaf_end = 0;
+        tri_next = 0;
+        tri_end = 0;
+        want_advance = false;
+        inst = 0;
+        idx_off = 0;
+        safe_t = -1.0f;
+        h.t = kTFar;
+        h.found = false;
+        h.tri = 0;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        return true;
+    }
+
+    // One step; false when the traversal is complete (run_advance as in
+    // Traversal::step).
+    __device__ __forceinline__ bool step(const DevScene &sc, LdsStack &stk, bool run_advance = true) {
+        if (want_advance) {
+            if (!run_advance) return true;
+            want_advance = false;
+            return advance(sc, stk);
+        }
+        const uint32_t e = entry;
+        bool test_tri = in_blas && tri_next < tri_end;
+        if (!test_tri && (e >> 31)) {
+            const uint32_t first = e & ((1u << 26) - 1u);
+            const uint32_t count = ((e >> 26) & 31u) + 1u;
+            if (!in_blas) {
+                leaf_next = first;
+                leaf_end = first + count;
+                inst = 0xffffffffu;
+                want_advance = true;
+                return true;
+            }
+            tri_next = first;
+            tri_end = first + count;
+            test_tri = true;
+        }
+        if (test_tri) {
+            mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
+            tri_next += 1;
+            if (TRIS_PER_STEP == 2 && tri_next < tri_end) {
+                mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
+                tri_next += 1;
+            }
+            if (tri_next < tri_end) return true;
+            if (ANY_HIT && h.found && h.t < safe_t) {
+                occluded = true;
+                return false;
+            }
+            cur_t = h.t;
+            return pop(sc, stk);
+        }
+
+        // interior wide node of the current level
+        if (iter >= kMaxIterations) {
+            atomicOr(sc.error_flags, kErrIterationCap);
+            if (!in_blas) return false;
+            want_advance = true;
+            return true;
+        }
+        iter += 1;
+        const float4 *p = nodes + 8u * e;
+        const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+        const uint4 en = *reinterpret_cast<const uint4 *>(p + 6);
+        uint32_t e0 = en.x, e1 = en.y, e2 = en.z, e3 = en.w;
+        float d0 = e0 != kWideEmpty ? wide_slab(cur, cur_t, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x) : kTFar;
+        float d1 = e1 != kWideEmpty ? wide_slab(cur, cur_t, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y) : kTFar;
+        float d2 = e2 != kWideEmpty ? wide_slab(cur, cur_t, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z) : kTFar;
+        float d3 = e3 != kWideEmpty ? wide_slab(cur, cur_t, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w) : kTFar;
+        wide_cswap(d0, e0, d1, e1);
+        wide_cswap(d2, e2, d3, e3);
+        wide_cswap(d0, e0, d2, e2);
+        wide_cswap(d1, e1, d3, e3);
+        wide_cswap(d1, e1, d2, e2);
+        if (d0 == kTFar) return pop(sc, stk);
+        entry = e0;
+        push(sc, stk, d3, e3);
+        push(sc, stk, d2, e2);
+        push(sc, stk, d1, e1);
+        return true;
+    }
+
+    __device__ __forceinline__ void push(const DevScene &sc, LdsStack &stk, float d, uint32_t e) {
+        if (d == kTFar) return;
+        if (sp < stk.capacity) {
+            stk.put(sp, e);
+            sp += 1;
+        } else {
+            atomicOr(sc.error_flags, kErrStackOverflow);
+        }
+    }
+
+    __device__ __forceinline__ bool pop(const DevScene &sc, LdsStack &stk) {
+        if (sp > sp_base) {
+            sp -= 1;
+            entry = stk.get(sp);
+            return true;
+        }
+        if (!in_blas) return false;  // TLAS stack empty: done
+        want_advance = true;
+        return true;
+    }
+
+    // Finishes the BLAS in progress (if any), then enters the next instance
+    // of the TLAS leaf or returns to the TLAS.
+    __device__ __forceinline__ bool advance(const DevScene &sc, LdsStack &stk) {
+        if (in_blas) {
+            const pulse_mesh_instance &mi = sc.instances[inst];
+            if (!h.found) h.t = kTFar;
+            float ow[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
+            V3 hit_world = transform_position(ow, add(cur.o, scale(h.t, cur.d)));
+            float new_t = length(sub(hit_world, wo));
+            if (ANY_HIT) {
+                if (new_t < threshold) {
+                    occluded = true;
+                    return false;
+                }
+            } else if (new_t < rec_t) {
+                rec_t = new_t;
+                rec_inst = inst;
+                rec_tri = h.tri;
+                rec_u = h.u;
+                rec_v = h.v;
+                world_cull = rec_t;
+            }
+        }
+        if (leaf_next < leaf_end) {
+            inst = sc.instance_indices[leaf_next];
+            leaf_next += 1;
+            const pulse_mesh_instance &mi = sc.instances[inst];
+            float wm[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) wm[i] = mi.world_object[i];
+            idx_off = mi.index_offset;
+            V3 oo = transform_position(wm, wo);
+            V3 dd = transform_direction(wm, wd);
+            const float dlen = length(dd);
+            cur = wide_ray(oo, divs(dd, dlen));
+            float start_t = kTFar;
+            if (world_cull < kTFar) {
+                float wb = world_cull + world_bound_slack(wo, world_cull);
+                float ob = wb * dlen * (1.0f + 0x1p-10f);
+                if (ob < kTFar) start_t = ob;
+            }
+            if (ANY_HIT) {
+                float sw = threshold - slack_thr;
+                safe_t = sw > 0.0f ? sw * dlen * (1.0f - 0x1p-10f) : -1.0f;
+            }
+            h.t = start_t;
+            h.found = false;
+            h.tri = 0;
+            h.u = 0.0f;
+            h.v = 0.0f;
+            cur_t = h.t;
+            nodes = sc.wide_blas + 8u * sc.wide_root[inst];
+            if (!in_blas) iter_tlas = iter;
+            in_blas = true;
+            sp_base = sp;
+            iter = 0;
+            // (tri_next == tri_end here: every leaf runs to its end)
+            entry = 0u;
+            return true;
+        }
+        // back to the TLAS
+        if (in_blas) iter = iter_tlas;
+        in_blas = false;
+        cur.o = wo;
+        cur.d = wd;
+        cur.r = world_r;
+        cur_t = world_cull;
+        nodes = sc.wide_tlas;
+        sp_base = 0;
+        if (sp == 0) return false;
+        sp -= 1;
+        entry = stk.get(sp);
+        return true;
+    }
+};
+
+// Closest hit / occlusion of one ray over the 4-wide BVHs (trace_unified's
+// contract).
+template <bool ANY_HIT>
+__device__ __forceinline__ bool trace_wide(const DevScene &sc, V3 wo, V3 wd, float &rec_t, uint32_t &rec_inst,
+                                           uint32_t &rec_tri, float &rec_u, float &rec_v, float threshold,
+                                           LdsStack &stk) {
+    WideTraversal<ANY_HIT> tr;
+    if (tr.begin(sc, wo, wd, rec_t, threshold)) {
+        while (tr.step(sc, stk)) {
+        }
+    }
+    if (ANY_HIT) return tr.occluded;
+    rec_t = tr.rec_t;
+    rec_inst = tr.rec_inst;
+    rec_tri = tr.rec_tri;
+    rec_u = tr.rec_u;
+    rec_v = tr.rec_v;
+    return false;
+}
+
+}  // namespace pt

@@ -46,7 +46,7 @@ struct DevBuf {
 
 enum SceneBuf {
     B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
-    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_COUNT
+    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_WTLAS, B_WBLAS, B_WROOT, B_COUNT
 };
 
 // Queue counter words of the single-view path: extension and shadow queue
@@ -64,6 +64,9 @@ struct SceneData {
     DevBuf buf[B_COUNT];
     pulse_scene_uniform uniform{};
     uint32_t stack_need = 0;     // TLAS depth + deepest BLAS + 1
+    uint32_t wide_need = 0;      // the same bound for the 4-wide BVHs
+    uint32_t wide_blas_need = 0; // deepest 4-wide BLAS part of wide_need
+    bool has_wide = false;       // the 4-wide BVHs were built
     uint32_t tlas_depth = 0, blas_depth = 0;
     uint32_t n_materials = 0;
     uint32_t n_strength_cdf = 0;  // entries of light_emission_strength_cdf
@@ -91,6 +94,9 @@ int scene_update_instances(SceneData &s, const pulse_scene_desc *d);
 pt::DevScene dev_scene(const SceneData &s, uint32_t *errors);
 // Smallest compiled stack size that holds `need` entries (0: none).
 uint32_t pick_stack(uint32_t need, int loop_variant);
+// Stack entries the traversal `loop_variant` needs on scene `s` (0: the
+// variant cannot run on it).
+uint32_t stack_for(const SceneData &s, int loop_variant);
 
 }  // namespace pulse_host
 
@@ -105,7 +111,7 @@ struct pulse_ctx {
     size_t wf_pixels = 0;
     uint64_t generation = 0;
     uint32_t stack_entries = 0;  // traversal stack capacity in use
-    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop
+    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop, 2 4-wide BVH
     int persistent = 24;         // persistent traversal waves per CU (0: one ray per lane)
     int refill = 16;             // idle lanes that make a persistent wave take new rays
     int gather = 1;              // persistent: lanes gathered before an instance change runs

@@ -1,0 +1,164 @@
+// wide_bvh.cpp -- collapse of the reference's binary BVHs into 4-wide nodes
+// (layout in wide_bvh.h).  A wide node takes the two children of a binary
+// interior node and keeps replacing its largest-area interior child by that
+// child's two children until it has four children or only leaves: the
+// leaves, their order and their primitive ranges are the binary tree's.
+
+#include "wide_bvh.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace pulse_host {
+namespace {
+
+struct Item {
+    float lo[3], hi[3];
+    bool interior;   // binary interior node `bin` (else a primitive range)
+    uint32_t bin;
+    uint32_t first, count;
+};
+
+struct Builder {
+    const uint32_t *nodes;
+    size_t n;
+    WideTree &out;
+    bool ok = true;
+
+    Item read(uint32_t i) const {
+        Item it{};
+        const uint32_t *w = nodes + 8 * static_cast<size_t>(i);
+        std::memcpy(it.lo, w, 12);
+        std::memcpy(it.hi, w + 4, 12);
+        it.bin = i;
+        it.first = w[3];
+        it.count = w[7];
+        it.interior = it.count == 0;
+        return it;
+    }
+    static float half_area(const Item &it) {
+        float dx = it.hi[0] - it.lo[0], dy = it.hi[1] - it.lo[1], dz = it.hi[2] - it.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    uint32_t alloc() {
+        uint32_t idx = static_cast<uint32_t>(out.words.size() / kWideNodeWords);
+        out.words.resize(out.words.size() + kWideNodeWords, 0u);
+        for (int k = 0; k < 4; ++k) out.words[static_cast<size_t>(idx) * kWideNodeWords + 24 + k] = kWideEmpty;
+        return idx;
+    }
+    void set_child(uint32_t node, int k, const Item &it, uint32_t entry) {
+        uint32_t *w = out.words.data() + static_cast<size_t>(node) * kWideNodeWords;
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(w + 8 * a + k, &it.lo[a], 4);
+            std::memcpy(w + 8 * a + 4 + k, &it.hi[a], 4);
+        }
+        w[24 + k] = entry;
+    }
+    uint32_t leaf_entry(uint32_t first, uint32_t count) {
+        if (first >= kWideFirstLimit) ok = false;
+        return kWideLeafBit | ((count - 1u) << 26) | (first & ((1u << 26) - 1u));
+    }
+
+    // Node over a primitive range of more than 32 (chunks of 32 in order);
+    // returns its index, `need` = the stack entries below it.
+    uint32_t range_node(const Item &box, uint32_t first, uint32_t count, uint32_t &need) {
+        uint32_t node = alloc();
+        uint32_t chunks = (count + kWideMaxLeaf - 1) / kWideMaxLeaf;
+        if (chunks <= 4) {
+            for (uint32_t c = 0; c < chunks; ++c) {
+                uint32_t f = first + c * kWideMaxLeaf;
+                set_child(node, static_cast<int>(c), box, leaf_entry(f, std::min(kWideMaxLeaf, first + count - f)));
+            }
+            need = chunks - 1;
+            return node;
+        }
+        // four groups of whole chunks
+        uint32_t per = (chunks + 3) / 4, deepest = 0;
+        int k = 0;
+        for (uint32_t c = 0; c < chunks; c += per, ++k) {
+            uint32_t f = first + c * kWideMaxLeaf;
+            uint32_t cnt = std::min(per * kWideMaxLeaf, first + count - f);
+            uint32_t sub_need = 0, entry;
+            if (cnt <= kWideMaxLeaf) {
+                entry = leaf_entry(f, cnt);
+            } else {
+                entry = range_node(box, f, cnt, sub_need);
+            }
+            set_child(node, k, box, entry);
+            deepest = std::max(deepest, sub_need);
+        }
+        need = static_cast<uint32_t>(k - 1) + deepest;
+        return node;
+    }
+
+    // Entry for a child item; `need` = stack entries below it.
+    uint32_t entry_of(const Item &it, uint32_t &need) {
+        need = 0;
+        if (!it.interior) {
+            if (it.count <= kWideMaxLeaf) return leaf_entry(it.first, it.count);
+            return range_node(it, it.first, it.count, need);
+        }
+        return interior_node(it.bin, need);
+    }
+
+    // Wide node for binary interior node `b`.
+    uint32_t interior_node(uint32_t b, uint32_t &need) {
+        Item root = read(b);
+        std::vector<Item> items{read(root.first), read(root.first + 1u)};
+        while (items.size() < 4) {
+            int best = -1;
+            float area = -1.0f;
+            for (size_t k = 0; k < items.size(); ++k)
+                if (items[k].interior && half_area(items[k]) > area) {
+                    area = half_area(items[k]);
+                    best = static_cast<int>(k);
+                }
+            if (best < 0) break;
+            uint32_t c = items[best].first;
+            items[best] = read(c);
+            items.insert(items.begin() + best + 1, read(c + 1u));
+        }
+        uint32_t node = alloc();
+        uint32_t deepest = 0;
+        for (size_t k = 0; k < items.size(); ++k) {
+            uint32_t sub = 0;
+            uint32_t e = entry_of(items[k], sub);
+            set_child(node, static_cast<int>(k), items[k], e);
+            deepest = std::max(deepest, sub);
+        }
+        need = static_cast<uint32_t>(items.size() - 1) + deepest;
+        return node;
+    }
+};
+
+}  // namespace
+
+bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out) {
+    out.words.clear();
+    out.stack_need = 0;
+    Builder b{nodes, n, out};
+    if (n == 0) {
+        b.alloc();
+        return true;
+    }
+    Item root = b.read(0);
+    if (root.interior && root.first == 0) {  // empty tree {0, 0}: one node without children
+        b.alloc();
+        return true;
+    }
+    uint32_t need = 0;
+    if (root.interior) {
+        b.interior_node(0, need);
+    } else {
+        uint32_t node = b.alloc();
+        uint32_t sub = 0;
+        uint32_t e = b.entry_of(root, sub);
+        b.set_child(node, 0, root, e);
+        need = sub;
+    }
+    out.stack_need = need;
+    return b.ok;
+}
+
+}  // namespace pulse_host

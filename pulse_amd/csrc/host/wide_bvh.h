@@ -1,0 +1,45 @@
+// wide_bvh.h -- 4-wide BVHs derived at upload from the reference's binary
+// TLAS and BLASes (same leaves, same triangle slots, same instance ranges).
+//
+// A wide node is 128 bytes, 32 words, laid out for one 112-byte fetch of
+// seven float4:
+//   words  0-3  child min x    words  4-7  child max x
+//   words  8-11 child min y    words 12-15 child max y
+//   words 16-19 child min z    words 20-23 child max z
+//   words 24-27 child entries  words 28-31 unused (zero)
+// A child entry is what the traversal pushes on its stack:
+//   kWideEmpty                 no child in this position
+//   bit 31 clear               interior: index of the child's wide node,
+//                              relative to the first node of its tree
+//   bit 31 set                 leaf: bits 26-30 hold count - 1 (1..32),
+//                              bits 0-25 the first triangle slot (BLAS) or
+//                              the first instance_indices entry (TLAS)
+// Binary leaves of more than 32 primitives become several leaf entries of at
+// most 32 (in order) under extra wide nodes with the leaf's box.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace pulse_host {
+
+constexpr uint32_t kWideEmpty = 0xffffffffu;
+constexpr uint32_t kWideLeafBit = 0x80000000u;
+constexpr uint32_t kWideMaxLeaf = 32;
+constexpr uint32_t kWideFirstLimit = (1u << 26) - kWideMaxLeaf;  // first + count stays below 2^26
+constexpr int kWideNodeWords = 32;
+
+struct WideTree {
+    std::vector<uint32_t> words;  // kWideNodeWords per node, root first
+    uint32_t stack_need = 0;      // most stack entries a traversal of the tree holds
+};
+
+// Collapses the binary tree whose nodes are `nodes[0..n)` (8 words each in
+// the reference's node layout: min xyz, first child / first primitive, max
+// xyz, primitive count; node 0 is the root, child indices are relative to
+// `nodes`) into `out`.  False if a leaf's first primitive is at or beyond
+// kWideFirstLimit (the entry cannot encode it).
+bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out);
+
+}  // namespace pulse_host

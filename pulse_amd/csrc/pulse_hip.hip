@@ -13,6 +13,7 @@
 #include "device/wavefront.h"
 #include "host/common.h"
 #include "host/context.h"
+#include "host/wide_bvh.h"
 
 using namespace pt;
 
@@ -79,10 +80,13 @@ DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx
 
 // Makes `sd` the scene of the context's launches.
 int bind_scene(pulse_ctx *ctx, SceneData *sd) {
-    uint32_t st = pick_stack(sd->stack_need, ctx->loop_variant);
-    if (st == 0)
+    uint32_t st = stack_for(*sd, ctx->loop_variant);
+    if (st == 0) {
+        if (ctx->loop_variant == 2 && !sd->has_wide)
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "the scene has no 4-wide BVH (a leaf starts at slot 2^26 - 32 or later)");
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the %s stack",
                    sd->tlas_depth, sd->blas_depth, ctx->loop_variant == 0 ? "nested loop's" : "single loop's");
+    }
     ctx->cur = sd;
     ctx->stack_entries = st;
     ctx->generation += 1;
@@ -119,21 +123,27 @@ int persistent_lds(pulse_ctx *ctx) {
         if (s <= want) got = s;
     return got;
 }
-template <int LDS, bool SHADOW, int TPS>
+template <int LDS, bool SHADOW, int TPS, int LOOP>
 void launch_queue_k(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, int q, int k, uint32_t *spill) {
-    hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx),
-                       wb, q, k, static_cast<uint32_t>(ctx->refill), static_cast<uint32_t>(ctx->gather),
-                       static_cast<int>(ctx->stack_entries), spill);
+    hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+                       dev_scene(ctx), wb, q, k, static_cast<uint32_t>(ctx->refill),
+                       static_cast<uint32_t>(ctx->gather), static_cast<int>(ctx->stack_entries), spill);
+}
+template <int LDS, int LOOP>
+void launch_queue_loop(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k,
+                       uint32_t *spill) {
+    if (ctx->tris_per_step == 2) {
+        if (shadow) launch_queue_k<LDS, true, 2, LOOP>(ctx, wb, grid, q, k, spill);
+        else launch_queue_k<LDS, false, 2, LOOP>(ctx, wb, grid, q, k, spill);
+    } else {
+        if (shadow) launch_queue_k<LDS, true, 1, LOOP>(ctx, wb, grid, q, k, spill);
+        else launch_queue_k<LDS, false, 1, LOOP>(ctx, wb, grid, q, k, spill);
+    }
 }
 template <int LDS>
 void launch_queue_lds(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k, uint32_t *spill) {
-    if (ctx->tris_per_step == 2) {
-        if (shadow) launch_queue_k<LDS, true, 2>(ctx, wb, grid, q, k, spill);
-        else launch_queue_k<LDS, false, 2>(ctx, wb, grid, q, k, spill);
-    } else {
-        if (shadow) launch_queue_k<LDS, true, 1>(ctx, wb, grid, q, k, spill);
-        else launch_queue_k<LDS, false, 1>(ctx, wb, grid, q, k, spill);
-    }
+    if (ctx->loop_variant == 2) launch_queue_loop<LDS, 2>(ctx, wb, grid, shadow, q, k, spill);
+    else launch_queue_loop<LDS, 1>(ctx, wb, grid, shadow, q, k, spill);
 }
 // Spill area of the persistent launches over queues of up to `cap` rays:
 // the stack entries below the LDS part, for every lane of the grid.  Grows
@@ -186,7 +196,7 @@ void launch_shadow(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int k) {
                 case 64: fn<64, 0>(ctx, __VA_ARGS__); break;                         \
                 default: fn<128, 0>(ctx, __VA_ARGS__); break;                        \
             }                                                                        \
-        } else {                                                                     \
+        } else if ((ctx)->loop_variant == 1) {                                       \
             switch ((ctx)->stack_entries) {                                          \
                 case 24: fn<24, 1>(ctx, __VA_ARGS__); break;                         \
                 case 32: fn<32, 1>(ctx, __VA_ARGS__); break;                         \
@@ -194,6 +204,15 @@ void launch_shadow(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int k) {
                 case 48: fn<48, 1>(ctx, __VA_ARGS__); break;                         \
                 case 64: fn<64, 1>(ctx, __VA_ARGS__); break;                         \
                 default: fn<128, 1>(ctx, __VA_ARGS__); break;                        \
+            }                                                                        \
+        } else {                                                                     \
+            switch ((ctx)->stack_entries) {                                          \
+                case 24: fn<24, 2>(ctx, __VA_ARGS__); break;                         \
+                case 32: fn<32, 2>(ctx, __VA_ARGS__); break;                         \
+                case 40: fn<40, 2>(ctx, __VA_ARGS__); break;                         \
+                case 48: fn<48, 2>(ctx, __VA_ARGS__); break;                         \
+                case 64: fn<64, 2>(ctx, __VA_ARGS__); break;                         \
+                default: fn<128, 2>(ctx, __VA_ARGS__); break;                        \
             }                                                                        \
         }                                                                            \
     } while (0)
@@ -249,6 +268,10 @@ uint32_t pick_stack(uint32_t need, int loop_variant) {
     return 0;
 }
 
+uint32_t stack_for(const SceneData &s, int loop_variant) {
+    if (loop_variant == 2) return s.has_wide ? pick_stack(s.wide_need, 2) : 0;
+    return pick_stack(s.stack_need, loop_variant);
+}
 
 DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     DevScene s;
@@ -264,6 +287,9 @@ DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     s.light_areas = sd.buf[B_AREAS].as<float>();
     s.lights = sd.buf[B_LIGHTS].as<pulse_light_data_index>();
     s.strength_cdf = sd.buf[B_SCDF].as<float>();
+    s.wide_tlas = sd.has_wide ? sd.buf[B_WTLAS].as<float4>() : nullptr;
+    s.wide_blas = sd.has_wide ? sd.buf[B_WBLAS].as<float4>() : nullptr;
+    s.wide_root = sd.has_wide ? sd.buf[B_WROOT].as<uint32_t>() : nullptr;
     s.instance_count = sd.uniform.instance_count;
     s.light_count = sd.uniform.light_count;
     s.error_flags = errors;
@@ -390,6 +416,34 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
                    tlas_depth, blas_depth);
 
+    // --- 4-wide BVHs over the same leaves (traversal loop 2) ---
+    WideTree wtlas;
+    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas);
+    std::vector<uint32_t> wblas;
+    std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
+    uint32_t wide_blas_need = 0;
+    {
+        std::vector<std::pair<uint32_t, uint32_t>> built;  // node_offset -> first wide node
+        for (size_t i = 0; i < n_inst && has_wide; ++i) {
+            const uint32_t off = inst[i].node_offset;
+            auto it = std::find_if(built.begin(), built.end(), [&](const auto &p) { return p.first == off; });
+            if (it != built.end()) {
+                wroot[i] = it->second;
+                continue;
+            }
+            WideTree t;
+            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, t);
+            const uint32_t root = static_cast<uint32_t>(wblas.size() / kWideNodeWords);
+            wblas.insert(wblas.end(), t.words.begin(), t.words.end());
+            wide_blas_need = std::max(wide_blas_need, t.stack_need);
+            built.push_back({off, root});
+            wroot[i] = root;
+        }
+    }
+    if (wblas.empty()) wblas.assign(kWideNodeWords, 0u);
+    const uint32_t wide_need = wtlas.stack_need + wide_blas_need + 1;
+    if (pick_stack(wide_need, 2) == 0) has_wide = false;
+
     // --- copy to device ---
     sd.valid = false;
     auto up = [&](int b, const void *src, size_t bytes) -> int {
@@ -407,6 +461,13 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = up(B_LIGHTS, lights, d->light_indices.bytes)) ||
         (rc = up(B_SCDF, d->light_emission_strength_cdf.data, d->light_emission_strength_cdf.bytes)))
         return rc;
+    if (has_wide && ((rc = up(B_WTLAS, wtlas.words.data(), wtlas.words.size() * 4)) ||
+                     (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
+                     (rc = up(B_WROOT, wroot.data(), wroot.size() * 4))))
+        return rc;
+    sd.has_wide = has_wide;
+    sd.wide_need = wide_need;
+    sd.wide_blas_need = wide_blas_need;
     sd.uniform = d->uniform;
     sd.n_strength_cdf = static_cast<uint32_t>(n_scdf);
     sd.stack_need = need;
@@ -484,12 +545,19 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
         if (bytes) HIP_TRY(hipMemcpy(sd.buf[b].ptr, src, bytes, hipMemcpyHostToDevice));
         return PULSE_OK;
     };
+    WideTree wtlas;
+    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas);
+    const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
+    if (pick_stack(wide_need, 2) == 0) has_wide = false;
     int rc;
     if ((rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) ||
-        (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes))) {
+        (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes)) ||
+        (has_wide && (rc = up(B_WTLAS, wtlas.words.data(), wtlas.words.size() * 4)))) {
         sd.valid = false;  // partially replaced: unusable until the next upload
         return rc;
     }
+    sd.has_wide = has_wide;
+    sd.wide_need = wide_need;
     sd.tlas_depth = tlas_depth;
     sd.stack_need = need;
     sd.fast_div = fast_div;
@@ -643,7 +711,7 @@ int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse
     HIP_TRY(hipStreamSynchronize(ctx->stream));  // launches in flight read the old buffers
     if ((rc = scene_update_instances(*sd, desc)) != PULSE_OK) return rc;
     if (ctx->cur == sd) {
-        uint32_t st = pick_stack(sd->stack_need, ctx->loop_variant);
+        uint32_t st = stack_for(*sd, ctx->loop_variant);
         if (st == 0) {
             ctx->cur = nullptr;
             PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep for the current traversal loop");
@@ -749,7 +817,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
     const uint32_t shade_grid = (cap + kShadeBlock - 1) / kShadeBlock;
 
-    const bool persistent = ctx->loop_variant == 1 && ctx->persistent > 0;
+    const bool persistent = ctx->loop_variant != 0 && ctx->persistent > 0;
     if (persistent && (rc = ensure_spill(ctx, cap)) != PULSE_OK) return rc;
     ctx->stats = pulse_stats{};
     double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0;
@@ -863,7 +931,7 @@ int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width,
     g.base_color = reinterpret_cast<const float4 *>(gb->base_color);
     g.emissive = reinterpret_cast<const float4 *>(gb->emissive);
     g.npix = npix;
-    const bool persistent = ctx->loop_variant == 1 && ctx->persistent > 0;
+    const bool persistent = ctx->loop_variant != 0 && ctx->persistent > 0;
     if (persistent && (rc = ensure_spill(ctx, npix)) != PULSE_OK) return rc;
     const uint32_t grid = (npix + kShadeBlock - 1) / kShadeBlock;
     const uint32_t spp = std::max(prm->spp, 1u);
@@ -993,8 +1061,8 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
     if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: null ctx");
     switch (option) {
         case PULSE_OPT_TRAVERSAL_LOOP: {
-            if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "traversal loop must be 0 or 1");
-            uint32_t st = ctx->has_scene() ? pick_stack(ctx->cur->stack_need, static_cast<int>(value)) : 1;
+            if (value < 0 || value > 2) PULSE_FAIL(PULSE_E_INVALID_ARG, "traversal loop must be 0, 1 or 2");
+            uint32_t st = ctx->has_scene() ? stack_for(*ctx->cur, static_cast<int>(value)) : 1;
             if (st == 0) PULSE_FAIL(PULSE_E_UNSUPPORTED, "no stack size for this variant and scene");
             ctx->loop_variant = static_cast<int>(value);
             if (ctx->has_scene()) ctx->stack_entries = st;

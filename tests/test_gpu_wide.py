@@ -1,0 +1,207 @@
+"""GPU: the 4-wide BVH traversal (traversal loop 2) against the f32 oracle.
+
+The wide BVHs are collapsed at upload from the reference's binary BLAS/TLAS:
+same leaves, same triangle slots, same per-instance closest hit and world-t
+conversion.  Only the order in which leaves are reached differs from the
+reference's binary near-first order, and that order matters only between
+hits whose t are equal or within rounding of each other (a tie the first
+found wins; a box whose rounded entry distance lies just above a hit inside
+it).  So: closest-hit ids, t, u and v equal the oracle's except on such t-ties
+(|dt| <= 1e-5 t), occlusion equals the oracle's except where the closest t
+ties the threshold, and images equal the oracle's bit for bit except in
+pixels whose path met such a tie (at most 1e-3 of them; the scenes here
+have none).
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleScene, random_rays, same_bits
+from pulse_amd import _native as N
+from pulse_amd import procedural as P
+from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
+
+pytestmark = pytest.mark.gpu
+
+TIE_REL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = Device(0)
+    d.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    return P.cornell()
+
+
+@pytest.fixture(scope="module")
+def field():
+    return P.icosphere_field(n_meshes=8, subdiv=3, n_instances=60, n_emissive=6, box=30.0, radius=(1.0, 3.0))
+
+
+def _check_hits(g, o, n):
+    same = (same_bits(g["t"], o["t"]) & same_bits(g["u"], o["u"]) & same_bits(g["v"], o["v"])
+            & (g["instance_index"] == o["instance_index"]) & (g["triangle_index"] == o["triangle_index"]))
+    bad = ~same
+    if bad.any():
+        dt = np.abs(g["t"][bad].astype(np.float64) - o["t"][bad])
+        assert (dt <= TIE_REL * np.abs(o["t"][bad])).all(), f"non-tie differences: {np.flatnonzero(bad)[:5]}"
+        assert bad.sum() <= max(1, n // 1000), f"{bad.sum()} t-ties of {n}"
+    return int(bad.sum())
+
+
+@pytest.mark.parametrize("which", ["cornell", "field", "bench"])
+def test_wide_trace_rays(dev, cornell, field, which):
+    if which == "bench":
+        scene, _ = P.icosphere_field()
+        spread = 50.0
+    else:
+        scene, _ = cornell if which == "cornell" else field
+        spread = 0.95 if which == "cornell" else 20.0
+    rays = random_rays(20000, np.random.default_rng(11), (0, 0, 0), spread)
+    dev.upload_scene(scene)
+    assert dev.get_option(N.OPT_TRAVERSAL_LOOP) == 2
+    g = dev.trace_rays(rays)
+    o, _ = OracleScene(scene).trace_rays(rays, threads=16)
+    _check_hits(g, o, len(rays))
+    assert (g["t"] < 1e30).mean() > 0.2
+    assert dev.stats().error_flags == 0
+
+
+def test_wide_axis_parallel_and_tmax(dev, cornell):
+    scene, _ = cornell
+    rng = np.random.default_rng(12)
+    n = 4000
+    rays = random_rays(n, rng, (0, 0, 0), 0.9)
+    axes = np.eye(3, dtype=np.float32)
+    rays["dir"][: n // 2] = axes[rng.integers(0, 3, n // 2)] * rng.choice([-1.0, 1.0], (n // 2, 1)).astype(np.float32)
+    rays["t_max"][n // 2:] = rng.uniform(0.1, 2.0, n - n // 2).astype(np.float32)
+    dev.upload_scene(scene)
+    g = dev.trace_rays(rays)
+    o, _ = OracleScene(scene).trace_rays(rays)
+    _check_hits(g, o, n)
+
+
+def test_wide_matches_bruteforce(dev, field):
+    """Against every triangle of every instance tested in order (no BVH)."""
+    scene, _ = field
+    rays = random_rays(3000, np.random.default_rng(13), (0, 0, 0), 20.0)
+    dev.upload_scene(scene)
+    g = dev.trace_rays(rays)
+    b, _ = OracleScene(scene).trace_rays(rays, threads=16, bruteforce=True)
+    _check_hits(g, b, len(rays))
+
+
+@pytest.mark.parametrize("which", ["cornell", "field"])
+def test_wide_shadow_rays(dev, cornell, field, which):
+    scene, _ = cornell if which == "cornell" else field
+    rng = np.random.default_rng(14)
+    rays = random_rays(20000, rng, (0, 0, 0), 0.95 if which == "cornell" else 20.0)
+    orc = OracleScene(scene)
+    hits, _ = orc.trace_rays(rays)
+    t = np.where(hits["t"] < 1e30, hits["t"], 10.0).astype(np.float32)
+    thr = t * rng.uniform(0.5, 1.5, len(t)).astype(np.float32)
+    thr[::7] = t[::7]
+    thr[1::7] = np.nextafter(t[1::7], np.float32(np.inf))
+    thr[2::13] = -1.0
+    rays["t_max"] = thr
+    dev.upload_scene(scene)
+    g = dev.trace_shadow_rays(rays)
+    o = orc.shadow_rays(rays)
+    bad = g != o
+    if bad.any():
+        near = np.abs(t[bad].astype(np.float64) - thr[bad]) <= TIE_REL * np.abs(t[bad])
+        assert near.all() and bad.sum() <= len(rays) // 1000
+    assert 0.05 < g.mean() < 0.95
+
+
+def _render(dev, scene, cam, prm):
+    dev.upload_scene(scene)
+    node = PulsePathTracerNode(dev, cam.width, cam.height)
+    node.run(cam, prm)
+    return node.readback()
+
+
+def _check_image(g, o):
+    px = ~same_bits(g, o).all(axis=-1)
+    assert px.mean() <= 1e-3, f"{px.sum()} of {px.size} pixels differ"
+    return int(px.sum())
+
+
+@pytest.mark.parametrize("mode,nee,depth", [(N.MODE_REF_GGX, N.NEE_LAMBERT, 5), (N.MODE_REF_NEE_LAMBERT, N.NEE_LAMBERT, 5),
+                                            (N.MODE_GGX_NEE, N.NEE_LAMBERT, 4), (N.MODE_GGX_NEE, N.NEE_GGX, 4)])
+def test_wide_render_cornell(dev, cornell, mode, nee, depth):
+    scene, camf = cornell
+    cam = camf(96, 80)
+    prm = make_params(spp=2, max_depth=depth, mode=mode, nee_eval=nee)
+    g = _render(dev, scene, cam, prm)
+    o, _, _ = OracleScene(scene).render(cam, prm)
+    _check_image(g, o)
+
+
+@pytest.mark.parametrize("waves,lds,tps", [(0, 0, 1), (24, 24, 2), (16, 8, 1), (64, 16, 2)])
+def test_wide_render_field_persistent(dev, field, waves, lds, tps):
+    """One ray per lane and persistent launches (stack partly in device
+    memory) trace the same rays the same way: identical images."""
+    scene, camf = field
+    cam = camf(120, 72)
+    dev.set_option(N.OPT_PERSISTENT_WAVES, waves)
+    dev.set_option(N.OPT_LDS_STACK_ENTRIES, lds)
+    dev.set_option(N.OPT_TRIS_PER_STEP, tps)
+    try:
+        prm = make_params(spp=1, max_depth=4, mode=N.MODE_GGX_NEE, accumulation_count=3)
+        g = _render(dev, scene, cam, prm)
+        o, _, _ = OracleScene(scene).render(cam, prm)
+        _check_image(g, o)
+    finally:
+        dev.set_option(N.OPT_PERSISTENT_WAVES, 24)
+        dev.set_option(N.OPT_LDS_STACK_ENTRIES, 24)
+        dev.set_option(N.OPT_TRIS_PER_STEP, 2)
+
+
+def test_wide_full_frame(dev):
+    """The 1080p benchmark frame (64 subdivision-6 meshes, 800 instances)."""
+    scene, camf = P.icosphere_field()
+    cam = camf(1920, 1080)
+    prm = make_params(max_depth=4, mode=N.MODE_GGX_NEE, flags=N.FLAG_COLLECT_STATS)
+    g = _render(dev, scene, cam, prm)
+    assert dev.stats().error_flags == 0
+    o, _, _ = OracleScene(scene).render(cam, prm, threads=16)
+    _check_image(g, o)
+
+
+def test_wide_empty_scene(dev):
+    from pulse_amd.scene import SceneBuffers
+    bufs = {name: np.zeros(0, dt) for name, dt in N.BUFFER_DTYPES.items()}
+    bufs["tlas_nodes"] = np.zeros(1, N.TLAS_NODE_DTYPE)
+    scene = SceneBuffers(0, 0, bufs)
+    _, camf = P.cornell()
+    g = _render(dev, scene, camf(32, 32), make_params(max_depth=5))
+    assert (g[..., :3] == 0).all() and (g[..., 3] == 1).all()
+
+
+def test_wide_instance_update(dev):
+    """Moving instances rebuilds the wide TLAS: the image is the oracle's
+    for the moved scene, and the one a full upload of it gives."""
+    from test_instance_update import _make, _transforms
+    a = _make(_transforms(24, 1), keep=True)
+    moved = _transforms(24, 2)
+    for i in (0, 3, 5, 17, 23):
+        a.builder.set_instance_transform(i, moved[i])
+    b = a.builder.update_instances(a)
+    cam = P.make_camera(64, 48, eye=(0.0, 0.0, 40.0), target=(0.0, 0.0, 0.0))
+    prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
+    dev.upload_scene(a)
+    node = PulsePathTracerNode(dev, cam.width, cam.height)
+    node.run(cam, prm)
+    dev.update_instances(b)
+    node.run(cam, prm)
+    after = node.readback()
+    o, _, _ = OracleScene(b).render(cam, prm)
+    _check_image(after, o)
+    full = _render(dev, b, cam, prm)
+    assert same_bits(full, after).all()
