@@ -48,7 +48,7 @@ struct DevScene {
     const float *light_areas;
     const pulse_light_data_index *lights;
     const float *strength_cdf;          // light_emission_strength_cdf (may be short: checked on the host)
-    const float4 *wide_tlas;            // 4-wide TLAS, 8 float4 per node (null: not built)
+    const float4 *wide_tlas;            // 4-wide TLAS, 4 float4 per quantised node (null: not built)
     const float4 *wide_blas;            // all meshes' 4-wide BLASes
     const uint32_t *wide_root;          // per instance: first wide node of its BLAS
     uint32_t instance_count;

@@ -3,15 +3,16 @@
 //
 // Same interface and the same closest-hit / occlusion definitions as
 // Traversal (traverse.h), fewer and wider steps: one step fetches one
-// 112-byte node, tests its four child boxes, visits the nearest hit child
-// next and pushes the others (farthest first).  The leaves, the triangle
+// 64-byte quantised node, tests its four child boxes, visits the nearest hit
+// child next and pushes the others (farthest first).  The leaves, the triangle
 // slots, the triangle test, the per-instance object-space closest hit and its
 // world-t conversion are the reference's (utilities.wgsl:259-326); only the
 // order in which leaves are reached differs from the reference's binary
 // near-first order.  That order decides which of two hits with exactly equal
 // t is kept (the first found), and a box whose computed entry distance is
-// rounded above a hit inside it; the box test here is padded by 2^-20 of the
-// distances so that no box holding a hit below the current bound is skipped.
+// rounded above a hit inside it; the quantised boxes contain the exact ones
+// and the box test is padded by more than its rounding error, so no box
+// holding a hit below the current bound is skipped.
 #pragma once
 
 #include "pt_device.h"
@@ -38,17 +39,41 @@ __device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
     return w;
 }
 
-// Padded slab test of one child box: its entry distance, or kTFar when the
-// box is missed, behind the origin or at or beyond `bound`.
-__device__ __forceinline__ float wide_slab(const WideRay &w, float bound, float lx, float hx, float ly, float hy,
-                                           float lz, float hz) {
-    float x0 = (lx - w.o.x) * w.r.x, x1 = (hx - w.o.x) * w.r.x;
-    float y0 = (ly - w.o.y) * w.r.y, y1 = (hy - w.o.y) * w.r.y;
-    float z0 = (lz - w.o.z) * w.r.z, z1 = (hz - w.o.z) * w.r.z;
-    float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-    float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-    tmin = tmin - fabsf(tmin) * 0x1p-20f;
-    tmax = tmax + fabsf(tmax) * 0x1p-20f;
+// Ray-side constants of one quantised node: along each axis the distance to
+// plane q is q * (scale / d) + (origin - o) / d, one fma per plane.  The
+// rounding of those distances (the reciprocal, the products, the fma) is a
+// few 2^-24 of |origin - o| / |d| + 255 scale / |d| on that axis; the
+// constant term of the entry plane is lowered and that of the exit plane
+// raised by 2^-20 of it, so every computed slab contains the exact one.
+struct QuantFrame {
+    float srx, sry, srz;              // scale / d per axis
+    float blx, bhx, bly, bhy, blz, bhz;  // (origin - o) / d for the low / high planes, padded
+};
+__device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bl, float &bh) {
+    sr = s * r;
+    const float b = (org - o) * r;
+    const float pad = copysignf(0x1p-20f * (fabsf(b) + 255.0f * fabsf(sr)), r);
+    bl = b - pad;  // r > 0: the low plane is the entry
+    bh = b + pad;
+}
+__device__ __forceinline__ QuantFrame quant_frame(const WideRay &w, uint4 a) {
+    QuantFrame f;
+    quant_axis(__uint_as_float((a.w & 0xffu) << 23), __uint_as_float(a.x), w.o.x, w.r.x, f.srx, f.blx, f.bhx);
+    quant_axis(__uint_as_float(((a.w >> 8) & 0xffu) << 23), __uint_as_float(a.y), w.o.y, w.r.y, f.sry, f.bly, f.bhy);
+    quant_axis(__uint_as_float(((a.w >> 16) & 0xffu) << 23), __uint_as_float(a.z), w.o.z, w.r.z, f.srz, f.blz, f.bhz);
+    return f;
+}
+
+__device__ __forceinline__ float qbyte(uint32_t w, int k) { return static_cast<float>((w >> (8 * k)) & 0xffu); }
+
+// Slab test of child k (planes in bytes k of b and c): its entry distance,
+// or kTFar when the box is missed, behind the origin or at or beyond `bound`.
+__device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, uint4 b, uint4 c, int k) {
+    const float x0 = __builtin_fmaf(qbyte(b.x, k), f.srx, f.blx), x1 = __builtin_fmaf(qbyte(b.y, k), f.srx, f.bhx);
+    const float y0 = __builtin_fmaf(qbyte(b.z, k), f.sry, f.bly), y1 = __builtin_fmaf(qbyte(b.w, k), f.sry, f.bhy);
+    const float z0 = __builtin_fmaf(qbyte(c.x, k), f.srz, f.blz), z1 = __builtin_fmaf(qbyte(c.y, k), f.srz, f.bhz);
+    const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+    const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
     return (tmax >= tmin && tmin < bound && tmax > 0.0f) ? tmin : kTFar;
 }
 
@@ -177,14 +202,14 @@ struct WideTraversal {
             return true;
         }
         iter += 1;
-        const float4 *p = nodes + 8u * e;
-        const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-        const uint4 en = *reinterpret_cast<const uint4 *>(p + 6);
-        uint32_t e0 = en.x, e1 = en.y, e2 = en.z, e3 = en.w;
-        float d0 = e0 != kWideEmpty ? wide_slab(cur, cur_t, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x) : kTFar;
-        float d1 = e1 != kWideEmpty ? wide_slab(cur, cur_t, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y) : kTFar;
-        float d2 = e2 != kWideEmpty ? wide_slab(cur, cur_t, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z) : kTFar;
-        float d3 = e3 != kWideEmpty ? wide_slab(cur, cur_t, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w) : kTFar;
+        const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + 4u * e;
+        const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
+        const QuantFrame f = quant_frame(cur, qa);
+        uint32_t e0 = qc.z, e1 = qc.w, e2 = qd.x, e3 = qd.y;
+        float d0 = e0 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 0) : kTFar;
+        float d1 = e1 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 1) : kTFar;
+        float d2 = e2 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 2) : kTFar;
+        float d3 = e3 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 3) : kTFar;
         wide_cswap(d0, e0, d1, e1);
         wide_cswap(d2, e2, d3, e3);
         wide_cswap(d0, e0, d2, e2);
@@ -272,7 +297,7 @@ struct WideTraversal {
             h.u = 0.0f;
             h.v = 0.0f;
             cur_t = h.t;
-            nodes = sc.wide_blas + 8u * sc.wide_root[inst];
+            nodes = sc.wide_blas + 4u * sc.wide_root[inst];
             if (!in_blas) iter_tlas = iter;
             in_blas = true;
             sp_base = sp;

@@ -7,6 +7,7 @@
 #include "wide_bvh.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace pulse_host {
@@ -159,6 +160,63 @@ bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out) {
     }
     out.stack_need = need;
     return b.ok;
+}
+
+bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out) {
+    const size_t n = in.words.size() / kWideNodeWords;
+    const size_t base = out.size();
+    out.resize(base + n * kQuantNodeWords, 0u);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t *w = in.words.data() + i * kWideNodeWords;
+        uint32_t *q = out.data() + base + i * kQuantNodeWords;
+        bool valid[4];
+        for (int k = 0; k < 4; ++k) {
+            valid[k] = w[24 + k] != kWideEmpty;
+            q[10 + k] = w[24 + k];
+        }
+        uint32_t exps = 0;
+        for (int a = 0; a < 3; ++a) {
+            float clo[4], chi[4];
+            std::memcpy(clo, w + 8 * a, 16);
+            std::memcpy(chi, w + 8 * a + 4, 16);
+            float lo = 0.0f, hi = 0.0f;
+            bool any = false;
+            for (int k = 0; k < 4; ++k) {
+                if (!valid[k]) continue;
+                if (!std::isfinite(clo[k]) || !std::isfinite(chi[k]) || std::fabs(clo[k]) > 0x1p40f ||
+                    std::fabs(chi[k]) > 0x1p40f)
+                    return false;
+                lo = any ? std::min(lo, clo[k]) : clo[k];
+                hi = any ? std::max(hi, chi[k]) : chi[k];
+                any = true;
+            }
+            const double extent = static_cast<double>(hi) - lo;
+            int e = -126;
+            if (extent > 0.0) {
+                e = std::max(-126, static_cast<int>(std::ceil(std::log2(extent / 255.0))));
+                while (extent > 255.0 * std::ldexp(1.0, e)) ++e;
+            }
+            if (e > 127) return false;
+            const double sc = std::ldexp(1.0, e);
+            exps |= static_cast<uint32_t>(e + 127) << (8 * a);
+            std::memcpy(q + a, &lo, 4);
+            uint32_t qlo = 0, qhi = 0;
+            for (int k = 0; k < 4; ++k) {
+                if (!valid[k]) continue;
+                double l = std::floor((static_cast<double>(clo[k]) - lo) / sc);
+                double h = std::ceil((static_cast<double>(chi[k]) - lo) / sc);
+                while (l > 0.0 && lo + l * sc > clo[k]) l -= 1.0;
+                while (h < 255.0 && lo + h * sc < chi[k]) h += 1.0;
+                if (l < 0.0 || h > 255.0 || lo + l * sc > clo[k] || lo + h * sc < chi[k]) return false;
+                qlo |= static_cast<uint32_t>(l) << (8 * k);
+                qhi |= static_cast<uint32_t>(h) << (8 * k);
+            }
+            q[4 + 2 * a] = qlo;
+            q[5 + 2 * a] = qhi;
+        }
+        q[3] = exps;
+    }
+    return true;
 }
 
 }  // namespace pulse_host

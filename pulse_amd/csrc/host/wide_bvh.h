@@ -16,6 +16,18 @@
 //                              the first instance_indices entry (TLAS)
 // Binary leaves of more than 32 primitives become several leaf entries of at
 // most 32 (in order) under extra wide nodes with the leaf's box.
+//
+// The device reads the quantised form of the same nodes: 64 bytes, 16 words
+// (one 64-byte fetch of four uint4):
+//   words  0-2  node origin xyz (f32: the minimum corner of the children)
+//   word   3    per-axis scale exponents, biased by 127, in bytes 0-2
+//               (scale = 2^(byte - 127), a normal float)
+//   words  4-9  child planes as bytes (byte k = child k): min x, max x,
+//               min y, max y, min z, max z; plane = origin + byte * scale
+//   words 10-13 child entries (as above)
+//   words 14-15 unused (zero)
+// Quantised planes round outwards: every quantised box contains the child's
+// box.
 #pragma once
 
 #include <cstddef>
@@ -41,5 +53,12 @@ struct WideTree {
 // `nodes`) into `out`.  False if a leaf's first primitive is at or beyond
 // kWideFirstLimit (the entry cannot encode it).
 bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out);
+
+constexpr int kQuantNodeWords = 16;
+
+// Quantised copy of `in` (kQuantNodeWords per node, same node indices and
+// entries) appended to `out`.  False if a coordinate is not finite or beyond
+// 2^40 in magnitude.
+bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out);
 
 }  // namespace pulse_host

@@ -83,7 +83,8 @@ int bind_scene(pulse_ctx *ctx, SceneData *sd) {
     uint32_t st = stack_for(*sd, ctx->loop_variant);
     if (st == 0) {
         if (ctx->loop_variant == 2 && !sd->has_wide)
-            PULSE_FAIL(PULSE_E_UNSUPPORTED, "the scene has no 4-wide BVH (a leaf starts at slot 2^26 - 32 or later)");
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "the scene has no 4-wide BVH (a leaf starts at slot 2^26 - 32 or later, "
+                       "or a node coordinate is not finite or beyond 2^40)");
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the %s stack",
                    sd->tlas_depth, sd->blas_depth, ctx->loop_variant == 0 ? "nested loop's" : "single loop's");
     }
@@ -416,10 +417,11 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the 128-entry stack",
                    tlas_depth, blas_depth);
 
-    // --- 4-wide BVHs over the same leaves (traversal loop 2) ---
+    // --- 4-wide BVHs over the same leaves (traversal loop 2), quantised ---
     WideTree wtlas;
-    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas);
-    std::vector<uint32_t> wblas;
+    std::vector<uint32_t> qtlas, wblas;
+    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas) &&
+                    quantize_wide(wtlas, qtlas);
     std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
     uint32_t wide_blas_need = 0;
     {
@@ -432,15 +434,15 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
                 continue;
             }
             WideTree t;
-            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, t);
-            const uint32_t root = static_cast<uint32_t>(wblas.size() / kWideNodeWords);
-            wblas.insert(wblas.end(), t.words.begin(), t.words.end());
+            const uint32_t root = static_cast<uint32_t>(wblas.size() / kQuantNodeWords);
+            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, t) &&
+                       quantize_wide(t, wblas);
             wide_blas_need = std::max(wide_blas_need, t.stack_need);
             built.push_back({off, root});
             wroot[i] = root;
         }
     }
-    if (wblas.empty()) wblas.assign(kWideNodeWords, 0u);
+    if (wblas.empty()) wblas.assign(kQuantNodeWords, 0u);
     const uint32_t wide_need = wtlas.stack_need + wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
 
@@ -461,7 +463,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = up(B_LIGHTS, lights, d->light_indices.bytes)) ||
         (rc = up(B_SCDF, d->light_emission_strength_cdf.data, d->light_emission_strength_cdf.bytes)))
         return rc;
-    if (has_wide && ((rc = up(B_WTLAS, wtlas.words.data(), wtlas.words.size() * 4)) ||
+    if (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) ||
                      (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
                      (rc = up(B_WROOT, wroot.data(), wroot.size() * 4))))
         return rc;
@@ -546,13 +548,15 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
         return PULSE_OK;
     };
     WideTree wtlas;
-    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas);
+    std::vector<uint32_t> qtlas;
+    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas) &&
+                    quantize_wide(wtlas, qtlas);
     const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
     int rc;
     if ((rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) ||
         (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes)) ||
-        (has_wide && (rc = up(B_WTLAS, wtlas.words.data(), wtlas.words.size() * 4)))) {
+        (has_wide && (rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)))) {
         sd.valid = false;  // partially replaced: unusable until the next upload
         return rc;
     }

@@ -43,14 +43,17 @@ def field():
     return P.icosphere_field(n_meshes=8, subdiv=3, n_instances=60, n_emissive=6, box=30.0, radius=(1.0, 3.0))
 
 
-def _check_hits(g, o, n):
+def _check_hits(g, o, n, max_ties=None):
+    """Every difference is a t-tie; at most `max_ties` of them (0.1 % of the
+    rays by default)."""
     same = (same_bits(g["t"], o["t"]) & same_bits(g["u"], o["u"]) & same_bits(g["v"], o["v"])
             & (g["instance_index"] == o["instance_index"]) & (g["triangle_index"] == o["triangle_index"]))
     bad = ~same
     if bad.any():
         dt = np.abs(g["t"][bad].astype(np.float64) - o["t"][bad])
         assert (dt <= TIE_REL * np.abs(o["t"][bad])).all(), f"non-tie differences: {np.flatnonzero(bad)[:5]}"
-        assert bad.sum() <= max(1, n // 1000), f"{bad.sum()} t-ties of {n}"
+        limit = max(1, n // 1000) if max_ties is None else max_ties
+        assert bad.sum() <= limit, f"{bad.sum()} t-ties of {n}"
     return int(bad.sum())
 
 
@@ -83,7 +86,12 @@ def test_wide_axis_parallel_and_tmax(dev, cornell):
     dev.upload_scene(scene)
     g = dev.trace_rays(rays)
     o, _ = OracleScene(scene).trace_rays(rays)
-    _check_hits(g, o, n)
+    # axis-parallel rays in an axis-aligned box meet shared edges of
+    # coplanar and perpendicular faces: exact ties are common among them
+    h = n // 2
+    ties = _check_hits(g[:h], o[:h], h, max_ties=h // 20)
+    _check_hits(g[h:], o[h:], n - h)
+    print(f"axis-parallel t-ties: {ties} of {h}")
 
 
 def test_wide_matches_bruteforce(dev, field):
