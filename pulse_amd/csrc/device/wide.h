@@ -21,6 +21,10 @@
 namespace pt {
 
 constexpr uint32_t kWideEmpty = 0xffffffffu;
+constexpr int kMaxWideVisits = 1 << 22;  // per ray, over both levels
+constexpr uint32_t kRangeMask = (1u << 26) - 1u;
+constexpr uint32_t kRangeOne = 1u << 26;            // one primitive remaining
+constexpr uint32_t kRangeStep = 1u - (1u << 26);    // next + 1, remaining - 1
 
 // Origin, direction and clamped reciprocal direction (components below 2^-80
 // in magnitude count as 2^-80: the slab of a parallel axis stays finite).
@@ -89,16 +93,17 @@ __device__ __forceinline__ void wide_cswap(float &da, uint32_t &ea, float &db, u
 
 template <bool ANY_HIT, int TRIS_PER_STEP = 2>
 struct WideTraversal {
-    V3 wo, wd, world_r;  // world ray and its clamped reciprocal direction
-    float threshold, slack_thr, world_cull;
+    V3 wo, wd;         // world ray
+    float threshold, slack_thr, any_cull;  // any_cull: occlusion mode's world culling bound
     WideRay cur;       // ray of the current level
-    float cur_t;       // culling bound of the current level
-    const float4 *nodes;
+    uint32_t nbase;    // first node of the current level's tree (0 in the TLAS)
     bool in_blas;
     int sp, sp_base;
-    int iter, iter_tlas;  // node visits at the current level; the TLAS's while in a BLAS
-    uint32_t leaf_next, leaf_end, inst, idx_off;
-    uint32_t tri_next, tri_end;  // triangles of the BLAS leaf in progress
+    int iter;             // wide node visits of this ray (a guard: validated trees stay far below)
+    // primitive ranges in progress as next | remaining << 26 (one add steps
+    // both): instances of a TLAS leaf, triangles of a BLAS leaf
+    uint32_t leaf, tri;
+    uint32_t inst;
     bool want_advance;           // next: leave the BLAS and/or enter the next instance
     float safe_t;
     ObjHit h;
@@ -106,6 +111,10 @@ struct WideTraversal {
     float rec_t, rec_u, rec_v;
     uint32_t rec_inst, rec_tri;
     bool occluded;
+
+    // World-space culling bound: the closest hit so far, or the padded
+    // occlusion threshold.
+    __device__ __forceinline__ float wcull() const { return ANY_HIT ? any_cull : rec_t; }
 
     // false: nothing to traverse (empty TLAS); the result is already final.
     __device__ __forceinline__ bool begin(const DevScene &sc, V3 o, V3 d, float t_max, float thr) {
@@ -119,32 +128,24 @@ struct WideTraversal {
         if (root[3] == 0u && root[7] == 0u) return false;  // utilities.wgsl:129-131
         wo = o;
         wd = d;
-        world_r = wide_ray(o, d).r;
         threshold = thr;
-        world_cull = rec_t;
         slack_thr = 0.0f;
+        any_cull = 0.0f;
         if (ANY_HIT) {
             slack_thr = world_bound_slack(wo, threshold);
-            world_cull = threshold + slack_thr;
+            any_cull = threshold + slack_thr;
         }
-        cur.o = wo;
-        cur.d = wd;
-        cur.r = world_r;
-        cur_t = world_cull;
-        nodes = sc.wide_tlas;
+        cur = wide_ray(wo, wd);
+        nbase = 0u;
         entry = 0u;
         in_blas = false;
         sp = 0;
         sp_base = 0;
         iter = 0;
-        iter_tlas = 0;
-        leaf_next = 0;
-        leaf_end = 0;
-        tri_next = 0;
-        tri_end = 0;
+        leaf = 0;
+        tri = 0;
         want_advance = false;
         inst = 0;
-        idx_off = 0;
         safe_t = -1.0f;
         h.t = kTFar;
         h.found = false;
@@ -155,72 +156,78 @@ struct WideTraversal {
     }
 
     // One step; false when the traversal is complete (run_advance as in
-    // Traversal::step).
+    // Traversal::step).  A step tests the current interior node (if any) and
+    // then, when that leads to a leaf or the current entry is one, up to
+    // TRIS_PER_STEP of the leaf's triangles: a lane that reaches a leaf does
+    // not wait a step to start on it, so more lanes of a wave work in each
+    // of the two parts.
     __device__ __forceinline__ bool step(const DevScene &sc, LdsStack &stk, bool run_advance = true) {
         if (want_advance) {
             if (!run_advance) return true;
             want_advance = false;
             return advance(sc, stk);
         }
-        const uint32_t e = entry;
-        bool test_tri = in_blas && tri_next < tri_end;
-        if (!test_tri && (e >> 31)) {
+        if (!(in_blas && tri >= kRangeOne) && !(entry >> 31)) {
+            // interior wide node of the current level
+            if (iter >= kMaxWideVisits) {
+                atomicOr(sc.error_flags, kErrIterationCap);
+                return false;
+            }
+            iter += 1;
+            const float cur_t = in_blas ? h.t : wcull();  // culling bound of this level
+            const uint4 *p =
+                reinterpret_cast<const uint4 *>(in_blas ? sc.wide_blas : sc.wide_tlas) + 4u * (nbase + entry);
+            const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
+            const QuantFrame f = quant_frame(cur, qa);
+            uint32_t e0 = qc.z, e1 = qc.w, e2 = qd.x, e3 = qd.y;
+            float d0 = e0 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 0) : kTFar;
+            float d1 = e1 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 1) : kTFar;
+            float d2 = e2 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 2) : kTFar;
+            float d3 = e3 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 3) : kTFar;
+            wide_cswap(d0, e0, d1, e1);
+            wide_cswap(d2, e2, d3, e3);
+            wide_cswap(d0, e0, d2, e2);
+            wide_cswap(d1, e1, d3, e3);
+            wide_cswap(d1, e1, d2, e2);
+            if (d0 == kTFar) {
+                if (!pop(sc, stk)) return false;
+                if (want_advance) return true;
+            } else {
+                entry = e0;
+                push(sc, stk, d3, e3);
+                push(sc, stk, d2, e2);
+                push(sc, stk, d1, e1);
+            }
+        }
+        if (!(in_blas && tri >= kRangeOne)) {
+            const uint32_t e = entry;
+            if (!(e >> 31)) return true;  // an interior node: the next step
             const uint32_t first = e & ((1u << 26) - 1u);
             const uint32_t count = ((e >> 26) & 31u) + 1u;
             if (!in_blas) {
-                leaf_next = first;
-                leaf_end = first + count;
+                leaf = first | (count << 26);
                 inst = 0xffffffffu;
                 want_advance = true;
                 return true;
             }
-            tri_next = first;
-            tri_end = first + count;
-            test_tri = true;
+            tri = first | (count << 26);
         }
-        if (test_tri) {
-            mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
-            tri_next += 1;
-            if (TRIS_PER_STEP == 2 && tri_next < tri_end) {
-                mt(cur.o, cur.d, sc.tri_bvh, idx_off + tri_next, tri_next, h);
-                tri_next += 1;
-            }
-            if (tri_next < tri_end) return true;
-            if (ANY_HIT && h.found && h.t < safe_t) {
-                occluded = true;
-                return false;
-            }
-            cur_t = h.t;
-            return pop(sc, stk);
+        // BLAS leaf entries hold global slots; h.tri becomes the mesh's slot
+        // at the merge
+        uint32_t t0 = tri & kRangeMask;
+        mt(cur.o, cur.d, sc.tri_bvh, t0, t0, h);
+        tri += kRangeStep;
+        if (TRIS_PER_STEP == 2 && tri >= kRangeOne) {
+            t0 = tri & kRangeMask;
+            mt(cur.o, cur.d, sc.tri_bvh, t0, t0, h);
+            tri += kRangeStep;
         }
-
-        // interior wide node of the current level
-        if (iter >= kMaxIterations) {
-            atomicOr(sc.error_flags, kErrIterationCap);
-            if (!in_blas) return false;
-            want_advance = true;
-            return true;
+        if (tri >= kRangeOne) return true;
+        if (ANY_HIT && h.found && h.t < safe_t) {
+            occluded = true;
+            return false;
         }
-        iter += 1;
-        const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + 4u * e;
-        const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
-        const QuantFrame f = quant_frame(cur, qa);
-        uint32_t e0 = qc.z, e1 = qc.w, e2 = qd.x, e3 = qd.y;
-        float d0 = e0 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 0) : kTFar;
-        float d1 = e1 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 1) : kTFar;
-        float d2 = e2 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 2) : kTFar;
-        float d3 = e3 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 3) : kTFar;
-        wide_cswap(d0, e0, d1, e1);
-        wide_cswap(d2, e2, d3, e3);
-        wide_cswap(d0, e0, d2, e2);
-        wide_cswap(d1, e1, d3, e3);
-        wide_cswap(d1, e1, d2, e2);
-        if (d0 == kTFar) return pop(sc, stk);
-        entry = e0;
-        push(sc, stk, d3, e3);
-        push(sc, stk, d2, e2);
-        push(sc, stk, d1, e1);
-        return true;
+        return pop(sc, stk);
     }
 
     __device__ __forceinline__ void push(const DevScene &sc, LdsStack &stk, float d, uint32_t e) {
@@ -263,27 +270,25 @@ struct WideTraversal {
             } else if (new_t < rec_t) {
                 rec_t = new_t;
                 rec_inst = inst;
-                rec_tri = h.tri;
+                rec_tri = h.tri - mi.index_offset;
                 rec_u = h.u;
                 rec_v = h.v;
-                world_cull = rec_t;
             }
         }
-        if (leaf_next < leaf_end) {
-            inst = sc.instance_indices[leaf_next];
-            leaf_next += 1;
+        if (leaf >= kRangeOne) {
+            inst = sc.instance_indices[leaf & kRangeMask];
+            leaf += kRangeStep;
             const pulse_mesh_instance &mi = sc.instances[inst];
             float wm[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) wm[i] = mi.world_object[i];
-            idx_off = mi.index_offset;
             V3 oo = transform_position(wm, wo);
             V3 dd = transform_direction(wm, wd);
             const float dlen = length(dd);
             cur = wide_ray(oo, divs(dd, dlen));
             float start_t = kTFar;
-            if (world_cull < kTFar) {
-                float wb = world_cull + world_bound_slack(wo, world_cull);
+            if (wcull() < kTFar) {
+                float wb = wcull() + world_bound_slack(wo, wcull());
                 float ob = wb * dlen * (1.0f + 0x1p-10f);
                 if (ob < kTFar) start_t = ob;
             }
@@ -296,24 +301,17 @@ struct WideTraversal {
             h.tri = 0;
             h.u = 0.0f;
             h.v = 0.0f;
-            cur_t = h.t;
-            nodes = sc.wide_blas + 4u * sc.wide_root[inst];
-            if (!in_blas) iter_tlas = iter;
+            nbase = sc.wide_root[inst];
             in_blas = true;
             sp_base = sp;
-            iter = 0;
-            // (tri_next == tri_end here: every leaf runs to its end)
+            // (no triangles remain here: every leaf runs to its end)
             entry = 0u;
             return true;
         }
         // back to the TLAS
-        if (in_blas) iter = iter_tlas;
         in_blas = false;
-        cur.o = wo;
-        cur.d = wd;
-        cur.r = world_r;
-        cur_t = world_cull;
-        nodes = sc.wide_tlas;
+        cur = wide_ray(wo, wd);
+        nbase = 0u;
         sp_base = 0;
         if (sp == 0) return false;
         sp -= 1;

@@ -23,6 +23,7 @@ struct Item {
 struct Builder {
     const uint32_t *nodes;
     size_t n;
+    uint32_t first_offset;
     WideTree &out;
     bool ok = true;
 
@@ -57,8 +58,9 @@ struct Builder {
         w[24 + k] = entry;
     }
     uint32_t leaf_entry(uint32_t first, uint32_t count) {
-        if (first >= kWideFirstLimit) ok = false;
-        return kWideLeafBit | ((count - 1u) << 26) | (first & ((1u << 26) - 1u));
+        const uint64_t f = static_cast<uint64_t>(first) + first_offset;
+        if (f >= kWideFirstLimit) ok = false;
+        return kWideLeafBit | ((count - 1u) << 26) | (static_cast<uint32_t>(f) & ((1u << 26) - 1u));
     }
 
     // Node over a primitive range of more than 32 (chunks of 32 in order);
@@ -135,10 +137,10 @@ struct Builder {
 
 }  // namespace
 
-bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out) {
+bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideTree &out) {
     out.words.clear();
     out.stack_need = 0;
-    Builder b{nodes, n, out};
+    Builder b{nodes, n, first_offset, out};
     if (n == 0) {
         b.alloc();
         return true;

@@ -12,7 +12,8 @@
 //   bit 31 clear               interior: index of the child's wide node,
 //                              relative to the first node of its tree
 //   bit 31 set                 leaf: bits 26-30 hold count - 1 (1..32),
-//                              bits 0-25 the first triangle slot (BLAS) or
+//                              bits 0-25 the first triangle slot (BLAS:
+//                              the global slot, index_offset included) or
 //                              the first instance_indices entry (TLAS)
 // Binary leaves of more than 32 primitives become several leaf entries of at
 // most 32 (in order) under extra wide nodes with the leaf's box.
@@ -50,9 +51,9 @@ struct WideTree {
 // Collapses the binary tree whose nodes are `nodes[0..n)` (8 words each in
 // the reference's node layout: min xyz, first child / first primitive, max
 // xyz, primitive count; node 0 is the root, child indices are relative to
-// `nodes`) into `out`.  False if a leaf's first primitive is at or beyond
-// kWideFirstLimit (the entry cannot encode it).
-bool collapse_wide(const uint32_t *nodes, size_t n, WideTree &out);
+// `nodes`) into `out`; leaf entries hold first primitive + `first_offset`.
+// False if that is at or beyond kWideFirstLimit (the entry cannot encode it).
+bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideTree &out);
 
 constexpr int kQuantNodeWords = 16;
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -420,25 +421,27 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     // --- 4-wide BVHs over the same leaves (traversal loop 2), quantised ---
     WideTree wtlas;
     std::vector<uint32_t> qtlas, wblas;
-    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas) &&
+    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas) &&
                     quantize_wide(wtlas, qtlas);
     std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
     uint32_t wide_blas_need = 0;
     {
-        std::vector<std::pair<uint32_t, uint32_t>> built;  // node_offset -> first wide node
+        // one tree per (node_offset, index_offset): leaf entries hold global slots
+        std::vector<std::array<uint32_t, 3>> built;  // node_offset, index_offset, first wide node
         for (size_t i = 0; i < n_inst && has_wide; ++i) {
-            const uint32_t off = inst[i].node_offset;
-            auto it = std::find_if(built.begin(), built.end(), [&](const auto &p) { return p.first == off; });
+            const uint32_t off = inst[i].node_offset, ioff = inst[i].index_offset;
+            auto it = std::find_if(built.begin(), built.end(),
+                                   [&](const auto &p) { return p[0] == off && p[1] == ioff; });
             if (it != built.end()) {
-                wroot[i] = it->second;
+                wroot[i] = (*it)[2];
                 continue;
             }
             WideTree t;
             const uint32_t root = static_cast<uint32_t>(wblas.size() / kQuantNodeWords);
-            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, t) &&
+            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, ioff, t) &&
                        quantize_wide(t, wblas);
             wide_blas_need = std::max(wide_blas_need, t.stack_need);
-            built.push_back({off, root});
+            built.push_back({off, ioff, root});
             wroot[i] = root;
         }
     }
@@ -549,7 +552,7 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     };
     WideTree wtlas;
     std::vector<uint32_t> qtlas;
-    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, wtlas) &&
+    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas) &&
                     quantize_wide(wtlas, qtlas);
     const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
