@@ -340,6 +340,8 @@ def main() -> int:
     ap.add_argument("--persistent", type=int, default=None, help="persistent traversal waves per CU (0: off)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a persistent refill")
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
+    ap.add_argument("--lds", type=int, default=None, help="persistent: traversal stack entries kept in LDS")
+    ap.add_argument("--tps", type=int, default=None, help="persistent: leaf triangles tested per step (1 or 2)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
@@ -381,6 +383,10 @@ def main() -> int:
         dev.set_option(N.OPT_REFILL_LANES, args.refill)
     if args.gather is not None:
         dev.set_option(N.OPT_GATHER_LANES, args.gather)
+    if args.lds is not None:
+        dev.set_option(N.OPT_LDS_STACK_ENTRIES, args.lds)
+    if args.tps is not None:
+        dev.set_option(N.OPT_TRIS_PER_STEP, args.tps)
     t0 = time.perf_counter()
     dev.upload_scene(scene)
     dev.sync()

@@ -32,8 +32,10 @@ struct WideRay {
     V3 o, d, r;
 };
 
+// The hardware reciprocal (within 1 ulp): its error is covered by the
+// slab test's padding.
 __device__ __forceinline__ float wide_rcp(float v) {
-    return 1.0f / (fabsf(v) < 0x1p-80f ? copysignf(0x1p-80f, v) : v);
+    return __builtin_amdgcn_rcpf(fabsf(v) < 0x1p-80f ? copysignf(0x1p-80f, v) : v);
 }
 __device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
     WideRay w;
@@ -79,6 +81,35 @@ __device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, ui
     const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
     return (tmax >= tmin && tmin < bound && tmax > 0.0f) ? tmin : kTFar;
+}
+
+// Moeller-Trumbore exactly as mt() (pt_device.h) evaluates it, without its
+// early exits: every quantity is computed and the hit is taken when all of
+// mt()'s tests pass, so lanes of a wave stay together through the test.
+__device__ __forceinline__ void mt_select(V3 o, V3 d, const float4 *tri, uint32_t slot, ObjHit &h) {
+    const float4 A = tri[3 * slot];
+    const float4 B = tri[3 * slot + 1];
+    const float4 C = tri[3 * slot + 2];
+    const V3 p0 = mk(A.x, A.y, A.z);
+    const V3 e1 = mk(A.w, B.x, B.y);
+    const V3 e2 = mk(B.z, B.w, C.x);
+    const V3 hv = cross(d, e2);
+    const float a = dot(e1, hv);
+    const float f = 1.0f / a;
+    const V3 s = sub(o, p0);
+    const float u = f * dot(s, hv);
+    const V3 q = cross(s, e1);
+    const float v = f * dot(d, q);
+    const float t = f * dot(e2, q);
+    const bool hit = !(fabsf(a) < 0.0001f) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f) &&
+                     t > 0.001f && t < h.t;
+    if (hit) {
+        h.t = t;
+        h.tri = slot;
+        h.u = u;
+        h.v = v;
+        h.found = true;
+    }
 }
 
 __device__ __forceinline__ void wide_cswap(float &da, uint32_t &ea, float &db, uint32_t &eb) {
@@ -180,10 +211,14 @@ struct WideTraversal {
             const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
             const QuantFrame f = quant_frame(cur, qa);
             uint32_t e0 = qc.z, e1 = qc.w, e2 = qd.x, e3 = qd.y;
-            float d0 = e0 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 0) : kTFar;
-            float d1 = e1 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 1) : kTFar;
-            float d2 = e2 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 2) : kTFar;
-            float d3 = e3 != kWideEmpty ? quant_slab(f, cur_t, qb, qc, 3) : kTFar;
+            // all four boxes are tested (an empty position's planes are
+            // zero), then empty positions are dropped
+            float d0 = quant_slab(f, cur_t, qb, qc, 0), d1 = quant_slab(f, cur_t, qb, qc, 1);
+            float d2 = quant_slab(f, cur_t, qb, qc, 2), d3 = quant_slab(f, cur_t, qb, qc, 3);
+            d0 = e0 != kWideEmpty ? d0 : kTFar;
+            d1 = e1 != kWideEmpty ? d1 : kTFar;
+            d2 = e2 != kWideEmpty ? d2 : kTFar;
+            d3 = e3 != kWideEmpty ? d3 : kTFar;
             wide_cswap(d0, e0, d1, e1);
             wide_cswap(d2, e2, d3, e3);
             wide_cswap(d0, e0, d2, e2);
@@ -214,12 +249,10 @@ struct WideTraversal {
         }
         // BLAS leaf entries hold global slots; h.tri becomes the mesh's slot
         // at the merge
-        uint32_t t0 = tri & kRangeMask;
-        mt(cur.o, cur.d, sc.tri_bvh, t0, t0, h);
+        mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
         tri += kRangeStep;
         if (TRIS_PER_STEP == 2 && tri >= kRangeOne) {
-            t0 = tri & kRangeMask;
-            mt(cur.o, cur.d, sc.tri_bvh, t0, t0, h);
+            mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
             tri += kRangeStep;
         }
         if (tri >= kRangeOne) return true;
