@@ -355,13 +355,20 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                      /* persistent grid of n waves per CU whose lanes */
                                      /* take new rays as theirs finish               */
 #define PULSE_OPT_REFILL_LANES 5   /* persistent: idle lanes that trigger a refill   */
+                                   /* (0: the traversal loop's tuned value, 16 for   */
+                                   /* loop 1, 32 for loop 2)                         */
 #define PULSE_OPT_GATHER_LANES 6   /* persistent: lanes waiting for an instance change */
-                                   /* before the wave runs it (1: no waiting)         */
+                                   /* before the wave runs it (1: no waiting; 0: the */
+                                   /* loop's tuned value, 1 for loop 1, 64 for 2)    */
 #define PULSE_OPT_LDS_STACK_ENTRIES 7 /* persistent: stack entries per lane kept in   */
                                       /* LDS (0: all); deeper entries go to device    */
                                       /* memory, so more waves fit per CU             */
 #define PULSE_OPT_TRIS_PER_STEP 8  /* persistent: leaf triangles per traversal step  */
                                    /* (1 or 2; same order, same results)            */
+#define PULSE_OPT_ADVANCE_PERIOD 9 /* persistent: every n-th step a wave also runs   */
+                                   /* the instance changes its lanes wait for, however */
+                                   /* few (1: every step; 0: the loop's tuned value,  */
+                                   /* 1 for loop 1, 2 for loop 2)                     */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

@@ -49,6 +49,7 @@ OPT_REFILL_LANES = 5
 OPT_GATHER_LANES = 6
 OPT_LDS_STACK_ENTRIES = 7
 OPT_TRIS_PER_STEP = 8
+OPT_ADVANCE_PERIOD = 9
 
 
 class PulseError(RuntimeError):

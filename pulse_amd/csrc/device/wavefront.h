@@ -148,15 +148,17 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
 //
 // Instance changes (the longest code path of a step) are gathered: a lane
 // whose next operation is one waits until at least `gather` lanes of the
-// wave are waiting for one, or no lane of the wave has anything else to do.
+// wave are waiting for one, no lane of the wave has anything else to do, or
+// the wave's step count is a multiple of `period`.
 //
 // The top LDS_ENTRIES entries of each lane's stack live in LDS, the rest (up
 // to `capacity`) in `spill`, one column of capacity - LDS_ENTRIES words per
 // lane of the grid.  LOOP 1 walks the reference's binary BVHs, LOOP 2 the
 // 4-wide ones (wide.h).
 template <int LDS_ENTRIES, bool SHADOW, int TPS, int LOOP = 1>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill,
-                                                    uint32_t gather, int capacity, uint32_t *spill) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t gather, uint32_t period,
+              int capacity, uint32_t *spill) {
     __shared__ uint32_t smem[LDS_ENTRIES * 64];
     LdsStack stk{smem + threadIdx.x, 64, capacity};
     if (capacity > LDS_ENTRIES) {
@@ -170,6 +172,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     std::conditional_t<LOOP == 2, WideTraversal<SHADOW, TPS>, Traversal<SHADOW, TPS>> tr;
     bool alive = false;
     bool exhausted = false;  // wave-uniform: the queue has no rays left
+    uint32_t wave_step = 0;
     uint32_t slot = 0, pixel = 0;
     for (;;) {
         const uint64_t idle = __ballot(!alive);
@@ -205,7 +208,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             continue;
         }
         const uint64_t waiting = __ballot(alive && tr.want_advance);
-        const bool run_advance = waiting == act || static_cast<uint32_t>(__popcll(waiting)) >= gather;
+        wave_step += 1u;
+        const bool run_advance = waiting == act || static_cast<uint32_t>(__popcll(waiting)) >= gather ||
+                                 wave_step % period == 0u;
         if (alive && !tr.step(sc, stk, run_advance)) {
             alive = false;
             if (SHADOW) {

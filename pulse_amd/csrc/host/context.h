@@ -113,8 +113,13 @@ struct pulse_ctx {
     uint32_t stack_entries = 0;  // traversal stack capacity in use
     int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop, 2 4-wide BVH
     int persistent = 24;         // persistent traversal waves per CU (0: one ray per lane)
-    int refill = 16;             // idle lanes that make a persistent wave take new rays
-    int gather = 1;              // persistent: lanes gathered before an instance change runs
+    int refill = 0;              // idle lanes that make a persistent wave take new rays (0: tuned)
+    int gather = 0;              // persistent: lanes gathered before an instance change runs (0: tuned)
+    int advance_period = 0;      // persistent: instance changes run at least every n steps (0: tuned)
+    // the values in effect: explicit settings, else the traversal loop's tuned ones
+    int eff_refill() const { return refill ? refill : (loop_variant == 2 ? 32 : 16); }
+    int eff_gather() const { return gather ? gather : (loop_variant == 2 ? 64 : 1); }
+    int eff_period() const { return advance_period ? advance_period : (loop_variant == 2 ? 2 : 1); }
     int lds_entries = 24;        // persistent: stack entries kept in LDS (0: all); the rest spill
     int tris_per_step = 2;       // persistent: leaf triangles tested per traversal step (1 or 2)
     pulse_host::DevBuf spill;    // persistent: spilled stack entries

@@ -128,8 +128,9 @@ int persistent_lds(pulse_ctx *ctx) {
 template <int LDS, bool SHADOW, int TPS, int LOOP>
 void launch_queue_k(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, int q, int k, uint32_t *spill) {
     hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
-                       dev_scene(ctx), wb, q, k, static_cast<uint32_t>(ctx->refill),
-                       static_cast<uint32_t>(ctx->gather), static_cast<int>(ctx->stack_entries), spill);
+                       dev_scene(ctx), wb, q, k, static_cast<uint32_t>(ctx->eff_refill()),
+                       static_cast<uint32_t>(ctx->eff_gather()), static_cast<uint32_t>(ctx->eff_period()),
+                       static_cast<int>(ctx->stack_entries), spill);
 }
 template <int LDS, int LOOP>
 void launch_queue_loop(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k,
@@ -1080,7 +1081,7 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             ctx->persistent = static_cast<int>(value);
             return PULSE_OK;
         case PULSE_OPT_REFILL_LANES:
-            if (value < 1 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "refill lanes must be 1..64");
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "refill lanes must be 0..64");
             ctx->refill = static_cast<int>(value);
             return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP:
@@ -1092,8 +1093,12 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             ctx->lds_entries = static_cast<int>(value);
             return PULSE_OK;
         case PULSE_OPT_GATHER_LANES:
-            if (value < 1 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "gather lanes must be 1..64");
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "gather lanes must be 0..64");
             ctx->gather = static_cast<int>(value);
+            return PULSE_OK;
+        case PULSE_OPT_ADVANCE_PERIOD:
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "advance period must be 0..64");
+            ctx->advance_period = static_cast<int>(value);
             return PULSE_OK;
         default:
             PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
@@ -1105,8 +1110,9 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
     switch (option) {
         case PULSE_OPT_TRAVERSAL_LOOP: *value = ctx->loop_variant; return PULSE_OK;
         case PULSE_OPT_PERSISTENT_WAVES: *value = ctx->persistent; return PULSE_OK;
-        case PULSE_OPT_REFILL_LANES: *value = ctx->refill; return PULSE_OK;
-        case PULSE_OPT_GATHER_LANES: *value = ctx->gather; return PULSE_OK;
+        case PULSE_OPT_REFILL_LANES: *value = ctx->eff_refill(); return PULSE_OK;
+        case PULSE_OPT_GATHER_LANES: *value = ctx->eff_gather(); return PULSE_OK;
+        case PULSE_OPT_ADVANCE_PERIOD: *value = ctx->eff_period(); return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;

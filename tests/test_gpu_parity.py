@@ -15,7 +15,8 @@ from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT_PERSISTENT, DEFAULT_REFILL, DEFAULT_GATHER, DEFAULT_LDS = 24, 16, 1, 24
+DEFAULT_PERSISTENT, DEFAULT_LDS = 24, 24
+AUTO = 0  # refill, gather and advance period: the traversal loop's tuned values
 
 MODES = [(N.MODE_REF_GGX, N.NEE_LAMBERT, 5), (N.MODE_REF_NEE_LAMBERT, N.NEE_LAMBERT, 5),
          (N.MODE_GGX_NEE, N.NEE_LAMBERT, 4), (N.MODE_GGX_NEE, N.NEE_GGX, 4)]
@@ -200,10 +201,10 @@ def test_traversal_variants_bit_exact(dev, field, loop):
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
 
 
-@pytest.mark.parametrize("waves,refill,gather,lds,tps", [(0, 16, 1, 0, 1), (4, 1, 1, 0, 1), (16, 16, 1, 0, 2),
-                                                          (16, 64, 8, 24, 1), (64, 32, 64, 16, 2), (16, 8, 16, 8, 1),
-                                                          (8, 16, 32, 32, 1), (24, 16, 1, 24, 2)])
-def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds, tps):
+@pytest.mark.parametrize("waves,refill,gather,lds,tps,period", [
+    (0, 16, 1, 0, 1, 1), (4, 1, 1, 0, 1, 1), (16, 16, 1, 0, 2, 1), (16, 64, 8, 24, 1, 1), (64, 32, 64, 16, 2, 1),
+    (16, 8, 16, 8, 1, 3), (8, 16, 32, 32, 1, 1), (24, 16, 1, 24, 2, 1), (24, 32, 64, 24, 2, 2)])
+def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds, tps, period):
     """Persistent extend/shadow launches (lanes take new rays as theirs
     finish; instance changes gathered across lanes; the deep part of the
     stack in device memory) give the oracle's image for every grid size,
@@ -215,6 +216,7 @@ def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds, 
     dev.set_option(N.OPT_GATHER_LANES, gather)
     dev.set_option(N.OPT_LDS_STACK_ENTRIES, lds)
     dev.set_option(N.OPT_TRIS_PER_STEP, tps)
+    dev.set_option(N.OPT_ADVANCE_PERIOD, period)
     try:
         assert dev.get_option(N.OPT_PERSISTENT_WAVES) == waves
         cam = camf(96, 56)
@@ -227,8 +229,9 @@ def test_persistent_traversal_bit_exact(dev, field, waves, refill, gather, lds, 
             assert same_bits(node.readback(), oi).all(), (mode, waves, refill, gather)
     finally:
         dev.set_option(N.OPT_PERSISTENT_WAVES, DEFAULT_PERSISTENT)
-        dev.set_option(N.OPT_REFILL_LANES, DEFAULT_REFILL)
-        dev.set_option(N.OPT_GATHER_LANES, DEFAULT_GATHER)
+        dev.set_option(N.OPT_REFILL_LANES, AUTO)
+        dev.set_option(N.OPT_GATHER_LANES, AUTO)
+        dev.set_option(N.OPT_ADVANCE_PERIOD, AUTO)
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, DEFAULT_LDS)
         dev.set_option(N.OPT_TRIS_PER_STEP, 2)
 
