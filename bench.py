@@ -47,12 +47,24 @@ B_INSTANCE = 32 + 4 + 64 + 64 + 8
 B_TRIANGLE = 48
 B_RAY_EXTEND = 32 + 20     # ray in, hit out
 B_RAY_SHADOW = 48 + 16     # ray + threshold + colour in, colour out
+# The 4-wide traversal (loop 2): a node visit reads one 64 B quantised node
+# (the roots included); an entered instance reads its index, world_object,
+# object_world, wide root and index offset.
+B_WIDE_NODE = 64
+B_INSTANCE_WIDE = 4 + 64 + 64 + 4 + 4
 
 
 def bytes_per_ray(work: dict, io: int) -> float:
     r = max(work["rays"], 1)
     return (B_ROOT * r + B_NODE_PAIR * (work["tlas_interior"] + work["blas_interior"])
             + B_INSTANCE * work["instances_entered"] + B_TRIANGLE * work["tri_tests"]) / r + io
+
+
+def bytes_per_ray_wide(work, io: int) -> float:
+    """work = [rays, wide node visits, triangle tests, instances entered]
+    from the kernels' own census (PULSE_FLAG_COUNT_WORK)."""
+    r = max(int(work[0]), 1)
+    return (B_WIDE_NODE * work[1] + B_INSTANCE_WIDE * work[3] + B_TRIANGLE * work[2]) / r + io
 
 
 def cpu_baseline(scene, cam, prm, seconds: float):
@@ -211,7 +223,7 @@ def _pmc_run(counter: str, extra_args, timeout_s: float = 150.0):
         shutil.rmtree(out, ignore_errors=True)
 
 
-def pmc_traffic(max_depth: int) -> dict | None:
+def pmc_traffic(max_depth: int, loop: int) -> dict | None:
     """HBM bytes per dispatch of the traversal launches, from FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch, one counter per run) over one frame of this
     workload, each scaled by what the same counter reads for a plain 256 MiB
@@ -219,7 +231,7 @@ def pmc_traffic(max_depth: int) -> dict | None:
     by their order within the frame: extend (trace index 0 = primary, then
     bounces) and shadow launches alternate with shading."""
     frame_args = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-adjoint", "--no-batch", "--no-pmc",
-                  "--max-depth", str(max_depth)]
+                  "--max-depth", str(max_depth), "--loop", str(loop)]
     scale = {}
     per_kind = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -246,6 +258,28 @@ def pmc_traffic(max_depth: int) -> dict | None:
     for kind, v in per_kind.items():
         out[kind] = (v["FETCH_SIZE"] * scale["FETCH_SIZE"] + v["WRITE_SIZE"] * scale["WRITE_SIZE"]) * 1024.0
     return out
+
+
+def bench_loop1(dev, node, cam, args, frame: int, frames: int = 4) -> dict:
+    """The same frames with traversal loop 1 (the reference's binary BVHs and
+    visiting order: bit-exact with the oracle), timed the same way."""
+    from pulse_amd import _native as N
+    from pulse_amd.path_tracer import make_params
+
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+    flags = N.FLAG_STAGE_TIMING | N.FLAG_COLLECT_STATS
+    try:
+        node.run(cam, make_params(accumulation_count=frame, max_depth=args.max_depth, mode=N.MODE_GGX_NEE, flags=flags))
+        rays = ms = 0.0
+        for i in range(frames):
+            node.run(cam, make_params(accumulation_count=frame + 1 + i, max_depth=args.max_depth,
+                                      mode=N.MODE_GGX_NEE, flags=flags))
+            st = dev.stats()
+            rays += st.incoherent_rays
+            ms += st.ms_incoherent
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, args.loop)
+    return {"value": round(rays / (ms / 1e3) / 1e6, 2), "unit": "Mrays/s", "frames": frames}
 
 
 def bench_batch(frames: int = 2, check_views: int = 4) -> dict:
@@ -336,7 +370,8 @@ def main() -> int:
     ap.add_argument("--max-depth", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--loop", type=int, default=1, help="traversal loop variant (0 nested, 1 single loop)")
+    ap.add_argument("--loop", type=int, default=2,
+                    help="traversal loop (0 nested, 1 single loop: the reference's order; 2: 4-wide BVHs)")
     ap.add_argument("--persistent", type=int, default=None, help="persistent traversal waves per CU (0: off)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a persistent refill")
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
@@ -345,6 +380,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
+    ap.add_argument("--no-loop1", action="store_true", help="skip the reference-order (loop 1) comparison frames")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -419,6 +455,16 @@ def main() -> int:
     value = agg["incoherent_rays"] / (agg["ms_incoherent"] / 1e3) / 1e6
     all_rays = agg["extension_rays"] + agg["shadow_rays"]
     img = node.readback()
+    census = None
+    if args.loop == 2:
+        # one more frame (untimed) with the kernels' work census for the bytes model
+        node.run(cam, make_params(accumulation_count=frame, max_depth=args.max_depth, mode=N.MODE_GGX_NEE,
+                                  flags=N.FLAG_COUNT_WORK))
+        w = dev.stats().work
+        census = {c: [int(w[i][j]) for j in range(4)] for i, c in enumerate(("primary", "bounce", "shadow"))}
+    loop1 = None
+    if args.loop == 2 and not args.no_loop1:
+        loop1 = bench_loop1(dev, node, cam, args, frame + 1)
     instance_update = bench_instance_update(dev, scene, upload_s)
     print(f"bench: {args.steps} frames, {ms_per_step:.2f} ms/frame, {value:.1f} incoherent Mrays/s", file=sys.stderr,
           flush=True)
@@ -460,6 +506,10 @@ def main() -> int:
         "image_mean": float(np.mean(img[..., :3])),
         "instance_update": instance_update,
     }
+    if loop1 is not None:
+        result["reference_order_loop1"] = loop1
+    if census is not None:
+        result["work_census"] = census
 
     if not args.no_adjoint:
         result["fwd_adjoint"] = bench_adjoint(scene, camf)
@@ -473,11 +523,16 @@ def main() -> int:
         prm = make_params(accumulation_count=0, max_depth=args.max_depth, mode=N.MODE_GGX_NEE)
         base, work = cpu_baseline(scene, cam, prm, args.cpu_seconds)
         result["cpu_baseline"] = base
-        bpr = {"primary": bytes_per_ray(work["primary"], B_RAY_EXTEND),
-               "bounce": bytes_per_ray(work["bounce"], B_RAY_EXTEND),
-               "shadow": bytes_per_ray(work["shadow"], B_RAY_SHADOW)}
+        if census is not None:
+            bpr = {"primary": bytes_per_ray_wide(census["primary"], B_RAY_EXTEND),
+                   "bounce": bytes_per_ray_wide(census["bounce"], B_RAY_EXTEND),
+                   "shadow": bytes_per_ray_wide(census["shadow"], B_RAY_SHADOW)}
+        else:
+            bpr = {"primary": bytes_per_ray(work["primary"], B_RAY_EXTEND),
+                   "bounce": bytes_per_ray(work["bounce"], B_RAY_EXTEND),
+                   "shadow": bytes_per_ray(work["shadow"], B_RAY_SHADOW)}
         # dominant kernel by total stage time
-        persistent = result["persistent_waves_per_cu"] > 0 and args.loop == 1
+        persistent = result["persistent_waves_per_cu"] > 0 and args.loop != 0
         ext_name = "k_trace_queue<..., false>" if persistent else "k_extend"
         sh_name = "k_trace_queue<..., true>" if persistent else "k_shadow"
         cands = {
@@ -494,7 +549,7 @@ def main() -> int:
         traffic = None
         pmc = None
         if not args.no_pmc and persistent:
-            pmc = pmc_traffic(args.max_depth)
+            pmc = pmc_traffic(args.max_depth, args.loop)
             if pmc is not None:
                 traffic = round(pmc[kind])
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -199,6 +199,11 @@ typedef struct pulse_params {
                                       /* the moving sampled directions instead of */
                                       /* at fixed directions (the default)        */
 #define PULSE_FLAG_LIGHT_POWER 16u /* NEE picks the light by emission strength   */
+#define PULSE_FLAG_COUNT_WORK 32u  /* traversal loop 2: before each trace launch,  */
+                                   /* walk its queue once more and count node      */
+                                   /* visits, triangle tests and instances entered */
+                                   /* into pulse_stats.work (extra work: not for   */
+                                   /* timed frames)                                */
                                    /* (light_emission_strength_cdf, pdf = its    */
                                    /* step) as the reference intended            */
                                    /* (utilities.wgsl:501, 541-546), instead of   */
@@ -238,6 +243,9 @@ typedef struct pulse_stats {
     double ms_shadow;          /* occlusion stages                         */
     uint32_t extend_launches;  /* closest-hit stage launches               */
     uint32_t shadow_launches;  /* occlusion stage launches                 */
+    uint64_t work[3][4];       /* PULSE_FLAG_COUNT_WORK: per ray category   */
+                               /* (primary, bounce, shadow) rays, 4-wide    */
+                               /* node visits, triangle tests, instances    */
 } pulse_stats;
 
 /* ------------------------------------------------------------------------ */
@@ -342,8 +350,10 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 
 /* Tuning options of a context (defaults are the measured best on MI355X). */
 #define PULSE_OPT_TRAVERSAL_LOOP 1 /* 0: nested TLAS/BLAS loops as in the reference, */
-                                   /* 1: one loop over both levels (default)         */
-                                   /* 2: 4-wide BVHs collapsed at upload from the     */
+                                   /* 1: one loop over both levels: 0 and 1 give the */
+                                   /*    reference's results bit for bit              */
+                                   /* 2 (default where the scene has them, else 1):  */
+                                   /*    4-wide BVHs collapsed at upload from the     */
                                    /*    reference's BVHs: same leaves, fewer steps;  */
                                    /*    among hits with equal t (or within rounding  */
                                    /*    of each other) the first found is kept, so   */

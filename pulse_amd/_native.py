@@ -36,6 +36,7 @@ FLAG_STAGE_TIMING = 2
 FLAG_HORIZON_CLAMP = 4
 FLAG_ADJOINT_REPARAM = 8
 FLAG_LIGHT_POWER = 16
+FLAG_COUNT_WORK = 32
 PRESENT_RGBA32F = 0
 PRESENT_RGBA16F = 1
 PRESENT_RGBA8_SRGB = 2
@@ -125,6 +126,7 @@ class Stats(C.Structure):
         ("ms_shadow", C.c_double),
         ("extend_launches", C.c_uint32),
         ("shadow_launches", C.c_uint32),
+        ("work", (C.c_uint64 * 4) * 3),
     ]
 
 

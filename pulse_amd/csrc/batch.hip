@@ -23,6 +23,8 @@ struct pulse_batch {
     uint32_t traces = 0;        // extension traces per path
     uint32_t cstride = 0;
     uint32_t stack_need = 0;    // deepest TLAS + BLAS over the views' scenes
+    uint32_t wide_need = 0;     // the same for the 4-wide BVHs
+    bool has_wide = true;       // every view's scene has its 4-wide BVHs
     uint32_t total_slots = 0;
     uint32_t total_tiles = 0;
     uint32_t max_chunks = 0;    // sum over views of ceil(npix / chunk)
@@ -52,8 +54,10 @@ bool stack_allowed(uint32_t s, uint32_t loop) {
 }
 
 uint32_t resolve_stack(const pulse_batch *b, const pulse_batch_config &c) {
-    if (c.stack_entries == 0) return pick_stack(b->stack_need, static_cast<int>(c.traversal_loop));
-    if (c.stack_entries < b->stack_need || !stack_allowed(c.stack_entries, c.traversal_loop)) return 0;
+    if (c.traversal_loop == 2 && !b->has_wide) return 0;
+    const uint32_t need = c.traversal_loop == 2 ? b->wide_need : b->stack_need;
+    if (c.stack_entries == 0) return pick_stack(need, static_cast<int>(c.traversal_loop));
+    if (c.stack_entries < need || !stack_allowed(c.stack_entries, c.traversal_loop)) return 0;
     return c.stack_entries;
 }
 
@@ -67,14 +71,16 @@ int check_config(const pulse_batch *b, const pulse_batch_config &c) {
         PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: lds_stack_entries in 0..128 (got %u)", c.lds_stack_entries);
     if (c.persistent_waves_per_cu > 64)
         PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: persistent_waves_per_cu above 64 (got %u)", c.persistent_waves_per_cu);
-    if (c.traversal_loop > 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: traversal_loop must be 0 or 1");
+    if (c.traversal_loop > 2) PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: traversal_loop must be 0, 1 or 2");
+    if (c.traversal_loop == 2 && !b->has_wide)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "batch config: traversal_loop 2 needs 4-wide BVHs in every view's scene");
     if (c.regroup_every < 1 || c.regroup_every > 8) PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: regroup_every in 1..8");
     if (c.tile_width != 4 && c.tile_width != 8 && c.tile_width != 16 && c.tile_width != 32 && c.tile_width != 64)
         PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: tile_width must be 4, 8, 16, 32 or 64 (got %u)", c.tile_width);
     if (c.use_graph > 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: use_graph must be 0 or 1");
     if (resolve_stack(b, c) == 0)
         PULSE_FAIL(PULSE_E_INVALID_ARG, "batch config: stack_entries %u unusable (needs >= %u, a compiled size)",
-                   c.stack_entries, b->stack_need);
+                   c.stack_entries, c.traversal_loop == 2 ? b->wide_need : b->stack_need);
     return PULSE_OK;
 }
 
@@ -154,7 +160,7 @@ void launch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k)
 // Persistent single-loop traversal with per-lane refill: the compiled LDS
 // stack size at most the requested one (0: the whole stack).
 constexpr int kLdsSizes[] = {8, 16, 24, 32, 40, 48, 64, 128};
-bool refill_path(const pulse_batch *b) { return b->cfg.persistent_waves_per_cu > 0 && b->cfg.traversal_loop == 1; }
+bool refill_path(const pulse_batch *b) { return b->cfg.persistent_waves_per_cu > 0 && b->cfg.traversal_loop != 0; }
 int refill_lds(const pulse_batch *b) {
     int want = b->cfg.lds_stack_entries > 0 ? std::min<int>(b->cfg.lds_stack_entries, b->stack_entries)
                                             : static_cast<int>(b->stack_entries);
@@ -169,12 +175,19 @@ uint32_t trace_grid(const pulse_batch *b) {
         grid = std::min<uint32_t>(grid, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
     return std::max<uint32_t>(grid, 1);
 }
+template <int LDS, int LOOP>
+void launch_refill_loop(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+    if (shadow)
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, true, LOOP>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q,
+                           k);
+    else
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, false, LOOP>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a,
+                           q, k);
+}
 template <int LDS>
 void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
-    if (shadow)
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, true>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q, k);
-    else
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, false>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q, k);
+    if (b->cfg.traversal_loop == 2) launch_refill_loop<LDS, 2>(b, a, shadow, q, k);
+    else launch_refill_loop<LDS, 1>(b, a, shadow, q, k);
 }
 
 void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
@@ -198,7 +211,7 @@ void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int 
             case 64: launch_trace<64, 0>(b, a, shadow, q, k); break;
             default: launch_trace<128, 0>(b, a, shadow, q, k); break;
         }
-    } else {
+    } else if (b->cfg.traversal_loop == 1) {
         switch (b->stack_entries) {
             case 24: launch_trace<24, 1>(b, a, shadow, q, k); break;
             case 32: launch_trace<32, 1>(b, a, shadow, q, k); break;
@@ -206,6 +219,15 @@ void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int 
             case 48: launch_trace<48, 1>(b, a, shadow, q, k); break;
             case 64: launch_trace<64, 1>(b, a, shadow, q, k); break;
             default: launch_trace<128, 1>(b, a, shadow, q, k); break;
+        }
+    } else {
+        switch (b->stack_entries) {
+            case 24: launch_trace<24, 2>(b, a, shadow, q, k); break;
+            case 32: launch_trace<32, 2>(b, a, shadow, q, k); break;
+            case 40: launch_trace<40, 2>(b, a, shadow, q, k); break;
+            case 48: launch_trace<48, 2>(b, a, shadow, q, k); break;
+            case 64: launch_trace<64, 2>(b, a, shadow, q, k); break;
+            default: launch_trace<128, 2>(b, a, shadow, q, k); break;
         }
     }
 }
@@ -276,9 +298,13 @@ int refresh_scenes(pulse_batch *b) {
         const SceneData &sd = b->scenes[i]->data;
         if (sd.generation == b->scene_gen[i]) continue;
         if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_batch_render: view %u: its scene is not valid", i);
-        if (sd.stack_need > b->stack_entries)
+        const bool wide = b->cfg.traversal_loop == 2;
+        if (wide && !sd.has_wide)
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_batch_render: view %u: the updated scene has no 4-wide BVH", i);
+        const uint32_t need = wide ? sd.wide_need : sd.stack_need;
+        if (need > b->stack_entries)
             PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_batch_render: view %u: the updated scene needs %u stack entries, the "
-                       "batch has %u (set a larger stack_entries)", i, sd.stack_need, b->stack_entries);
+                       "batch has %u (set a larger stack_entries)", i, need, b->stack_entries);
         b->views[i].sc = dev_scene(sd, b->ctx->errors.as<uint32_t>());
         b->scene_gen[i] = sd.generation;
         changed = true;
@@ -315,7 +341,7 @@ int pulse_batch_default_config(pulse_batch_config *out) {
     out->persistent_waves_per_cu = 24;
     out->chunk_rays = 256;
     out->stack_entries = 0;
-    out->traversal_loop = 1;
+    out->traversal_loop = 2;
     out->regroup_every = 1;
     out->tile_width = 8;
     out->use_graph = 1;
@@ -375,6 +401,8 @@ int pulse_batch_create(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views
         b->scenes.push_back(v.scene);
         b->scene_gen.push_back(v.scene->data.generation);
         b->stack_need = std::max(b->stack_need, v.scene->data.stack_need);
+        b->wide_need = std::max(b->wide_need, v.scene->data.wide_need);
+        b->has_wide = b->has_wide && v.scene->data.has_wide;
     }
     if (slots > (1ull << 31)) {
         pulse_batch_destroy(b);
@@ -396,6 +424,7 @@ int pulse_batch_create(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views
     }
     pulse_batch_config c;
     pulse_batch_default_config(&c);
+    if (!b->has_wide) c.traversal_loop = 1;
     if ((rc = apply_config(b, c)) != PULSE_OK) return fail(rc);
     *out = b;
     return PULSE_OK;
@@ -522,7 +551,7 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
     if (rc == PULSE_OK) {
         std::vector<uint32_t> stack_opts = {0};
         for (uint32_t s : kStacks)
-            if (s > pick_stack(b->stack_need, 1) && stack_opts.size() < 2) stack_opts.push_back(s);
+            if (s > pick_stack(std::max(b->stack_need, b->wide_need), 1) && stack_opts.size() < 2) stack_opts.push_back(s);
         for (int pass = 0; pass < 2 && rc == PULSE_OK; ++pass) {
             for (int knob = 0; knob < 8 && rc == PULSE_OK; ++knob) {
                 std::vector<pulse_batch_config> cands;
@@ -549,7 +578,7 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
                         for (uint32_t r : {1u, 2u}) add([&](pulse_batch_config &c) { c.regroup_every = r; });
                         break;
                     case 3:
-                        for (uint32_t l : {0u, 1u})
+                        for (uint32_t l : {0u, 1u, 2u})
                             add([&](pulse_batch_config &c) {
                                 c.traversal_loop = l;
                                 c.stack_entries = 0;

@@ -26,6 +26,7 @@
 
 #include <type_traits>
 
+#include "queue.h"
 #include "traverse.h"
 
 namespace pt {
@@ -96,7 +97,9 @@ __device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
 // NEE samples).  Unbiased for the derivative of the expected image including
 // the effect of visibility changes, which the reparameterised form (DETACHED =
 // false, tangents through the moving directions) does not capture.
-template <int BLOCK, int STACK, bool DETACHED>
+// LOOP: the traversal of the forward pass being replayed (1: the binary
+// BVHs, 2: the 4-wide ones), so every replayed path is the forward's.
+template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
     using S = typename std::conditional<DETACHED, float, DA>::type;
     __shared__ uint32_t smem[STACK * BLOCK];
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             for (uint32_t depth = 0; depth < a.max_depth; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
                 uint32_t inst = 0, tri = 0;
-                trace_unified<false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
+                trace_any<LOOP, false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
                 if (!(t < kTFar)) break;
                 S tw, uu, vv;
                 if constexpr (DETACHED) {
@@ -175,7 +178,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
-                    bool occ = trace_unified<true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,
+                    bool occ = trace_any<LOOP, true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,
                                                    val(ns.threshold), stk);
                     if (!occ) {
                         C = add(C, mul(T, ns.contrib));

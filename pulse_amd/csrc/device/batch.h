@@ -27,6 +27,8 @@
 // kernels, in the same per-pixel order.
 #pragma once
 
+#include <type_traits>
+
 #include "pt_device.h"
 #include "queue.h"
 #include "shade.h"
@@ -228,8 +230,9 @@ __global__ void __launch_bounds__(64) k_b_trace(BatchArgs b, int q, int k) {
 // chunk's rays one by one as theirs finish (once at least `refill` lanes are
 // idle), and the wave takes the next chunk when the current one is drained.
 // The top LDS entries of each lane's stack are in LDS, the rest in `spill`.
-// Each ray is traversed exactly as by k_b_trace.
-template <int LDS, bool SHADOW>
+// Each ray is traversed exactly as by k_b_trace (LOOP 1: the binary BVHs,
+// LOOP 2: the 4-wide ones).
+template <int LDS, bool SHADOW, int LOOP = 1>
 __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k) {
     __shared__ uint32_t smem[LDS * 64];
     const uint32_t ci = SHADOW ? b.max_traces + 1u + static_cast<uint32_t>(k) : static_cast<uint32_t>(k);
@@ -242,7 +245,7 @@ __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k
         stk.spill_stride = gridDim.x * 64u;
     }
     const uint32_t lane = __lane_id();
-    Traversal<SHADOW> tr;
+    std::conditional_t<LOOP == 2, WideTraversal<SHADOW, 2>, Traversal<SHADOW>> tr;
     bool alive = false;
     uint32_t slot = 0, pixel = 0;
     // wave-uniform: the chunk being drained

@@ -223,6 +223,38 @@ k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t
     }
 }
 
+// Work census of one queue for the 4-wide traversal: walks every ray of
+// queue q at trace index k (the shadow queue if SHADOW) as the trace kernels
+// do, writes nothing back, and adds {rays, wide node visits, triangle tests,
+// instances entered} to out[0..3] (one atomic per counter per wave).
+template <int STACK, bool SHADOW>
+__global__ void __launch_bounds__(64) k_census(DevScene sc, WfBuffers wb, int q, int k, unsigned long long *out) {
+    __shared__ uint32_t smem[STACK * 64];
+    LdsStack stk{smem + threadIdx.x, 64, STACK};
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
+    uint32_t c[4] = {0u, 0u, 0u, 0u};
+    if (i < count) {
+        WideTraversal<SHADOW, 2> tr;
+        const float4 oa = SHADOW ? wb.sh_o[i] : wb.ray_o[q][i];
+        const float4 da = SHADOW ? wb.sh_d[i] : wb.ray_d[q][i];
+        if (tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, SHADOW ? da.w : 0.0f)) {
+            while (tr.step(sc, stk)) {
+            }
+            c[1] = static_cast<uint32_t>(tr.iter);
+            c[2] = tr.n_tri;
+            c[3] = tr.n_inst;
+        }
+        c[0] = 1u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t v = c[j];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (threadIdx.x == 0 && v) atomicAdd(&out[j], static_cast<unsigned long long>(v));
+    }
+}
+
 // G-buffer GI pass (gi.wgsl:56-126), the part before the first trace: per
 // pixel, from its G-buffer sample, a cosine scatter ray and the NEE shadow
 // ray.  Sample 0 seeds the RNG with pixel_index * 5817321; later samples go

@@ -139,6 +139,7 @@ struct WideTraversal {
     float safe_t;
     ObjHit h;
     uint32_t entry;              // current entry (wide node or leaf range)
+    uint32_t n_tri, n_inst;      // triangle tests, instances entered (work census; unread otherwise)
     float rec_t, rec_u, rec_v;
     uint32_t rec_inst, rec_tri;
     bool occluded;
@@ -173,6 +174,8 @@ struct WideTraversal {
         sp = 0;
         sp_base = 0;
         iter = 0;
+        n_tri = 0;
+        n_inst = 0;
         leaf = 0;
         tri = 0;
         want_advance = false;
@@ -251,9 +254,11 @@ struct WideTraversal {
         // at the merge
         mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
         tri += kRangeStep;
+        n_tri += 1;
         if (TRIS_PER_STEP == 2 && tri >= kRangeOne) {
             mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
             tri += kRangeStep;
+            n_tri += 1;
         }
         if (tri >= kRangeOne) return true;
         if (ANY_HIT && h.found && h.t < safe_t) {
@@ -311,6 +316,7 @@ struct WideTraversal {
         if (leaf >= kRangeOne) {
             inst = sc.instance_indices[leaf & kRangeMask];
             leaf += kRangeStep;
+            n_inst += 1;
             const pulse_mesh_instance &mi = sc.instances[inst];
             float wm[16];
 #pragma unroll

@@ -111,7 +111,9 @@ struct pulse_ctx {
     size_t wf_pixels = 0;
     uint64_t generation = 0;
     uint32_t stack_entries = 0;  // traversal stack capacity in use
-    int loop_variant = 1;        // 0 nested TLAS/BLAS loops, 1 single loop, 2 4-wide BVH
+    int loop_variant = 2;        // in effect: 0 nested TLAS/BLAS loops, 1 single loop, 2 4-wide BVHs
+    bool loop_explicit = false;  // set through PULSE_OPT_TRAVERSAL_LOOP; else 2 where the scene has
+                                 // 4-wide BVHs and 1 where it has not
     int persistent = 24;         // persistent traversal waves per CU (0: one ray per lane)
     int refill = 0;              // idle lanes that make a persistent wave take new rays (0: tuned)
     int gather = 0;              // persistent: lanes gathered before an instance change runs (0: tuned)
@@ -124,6 +126,7 @@ struct pulse_ctx {
     int tris_per_step = 2;       // persistent: leaf triangles tested per traversal step (1 or 2)
     pulse_host::DevBuf spill;    // persistent: spilled stack entries
     pulse_host::DevBuf loss_acc; // f64 accumulator of pulse_image_loss
+    pulse_host::DevBuf census;   // PULSE_FLAG_COUNT_WORK: u64 [3 categories][4 counters]
     pulse_stats stats{};
     hipEvent_t ev[3 * pt::kMaxTraces + 8] = {};
     bool events_created = false;

@@ -81,14 +81,16 @@ DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx
 
 // Makes `sd` the scene of the context's launches.
 int bind_scene(pulse_ctx *ctx, SceneData *sd) {
-    uint32_t st = stack_for(*sd, ctx->loop_variant);
+    const int lv = ctx->loop_explicit ? ctx->loop_variant : (sd->has_wide ? 2 : 1);
+    uint32_t st = stack_for(*sd, lv);
     if (st == 0) {
-        if (ctx->loop_variant == 2 && !sd->has_wide)
+        if (lv == 2 && !sd->has_wide)
             PULSE_FAIL(PULSE_E_UNSUPPORTED, "the scene has no 4-wide BVH (a leaf starts at slot 2^26 - 32 or later, "
                        "or a node coordinate is not finite or beyond 2^40)");
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the %s stack",
-                   sd->tlas_depth, sd->blas_depth, ctx->loop_variant == 0 ? "nested loop's" : "single loop's");
+                   sd->tlas_depth, sd->blas_depth, lv == 0 ? "nested loop's" : lv == 1 ? "single loop's" : "4-wide");
     }
+    ctx->loop_variant = lv;
     ctx->cur = sd;
     ctx->stack_entries = st;
     ctx->generation += 1;
@@ -175,6 +177,19 @@ int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow,
         default: launch_queue_lds<128>(ctx, wb, grid, shadow, q, k, spill); break;
     }
     return PULSE_OK;
+}
+// Work census (PULSE_FLAG_COUNT_WORK) of queue q / the shadow queue at k
+// into category `cat` of ctx->census.
+template <int STACK, int LOOP>
+void launch_census(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow, int q, int k, int cat) {
+    uint32_t grid = (cap + kTraceBlock - 1) / kTraceBlock;
+    unsigned long long *out = ctx->census.as<unsigned long long>() + 4 * cat;
+    if (shadow)
+        hipLaunchKernelGGL((k_census<STACK, true>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx), wb, q,
+                           k, out);
+    else
+        hipLaunchKernelGGL((k_census<STACK, false>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, dev_scene(ctx), wb,
+                           q, k, out);
 }
 template <int STACK, int LOOP>
 void launch_extend(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, int q, int k) {
@@ -577,15 +592,22 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
 
 namespace {
 constexpr int kAdjBlock = 128;
-template <int STACK>
-void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
+template <int STACK, int LOOP>
+void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
     uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
     if (a.detached)
-        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true, LOOP>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
                            dev_scene(ctx), a);
     else
-        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, false>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, false, LOOP>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
                            dev_scene(ctx), a);
+}
+// The forward pass's traversal: the 4-wide BVHs for loop 2, the binary ones
+// (same results as loops 0 and 1) otherwise.
+template <int STACK>
+void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
+    if (ctx->loop_variant == 2) launch_adjoint_loop<STACK, 2>(ctx, a, lds);
+    else launch_adjoint_loop<STACK, 1>(ctx, a, lds);
 }
 }  // namespace
 
@@ -827,6 +849,11 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
 
     const bool persistent = ctx->loop_variant != 0 && ctx->persistent > 0;
     if (persistent && (rc = ensure_spill(ctx, cap)) != PULSE_OK) return rc;
+    const bool census = (prm->flags & PULSE_FLAG_COUNT_WORK) && ctx->loop_variant == 2;
+    if (census) {
+        if ((rc = ctx->census.ensure(sizeof(uint64_t) * 12)) != PULSE_OK) return rc;
+        HIP_TRY(hipMemsetAsync(ctx->census.ptr, 0, sizeof(uint64_t) * 12, ctx->stream));
+    }
     ctx->stats = pulse_stats{};
     double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0;
     uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
@@ -839,6 +866,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
         hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, ctx->stream, wb, fa);
         int q = 0;
         for (uint32_t k = 0; k < traces; ++k) {
+            if (census) PULSE_STACK_DISPATCH(launch_census, ctx, wb, cap, false, q, static_cast<int>(k), k == 0 ? 0 : 1);
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
             if (persistent) {
                 if ((rc = launch_queue(ctx, wb, cap, false, q, static_cast<int>(k))) != PULSE_OK) return rc;
@@ -848,6 +876,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
             hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, q,
                                static_cast<int>(k));
+            if (census) PULSE_STACK_DISPATCH(launch_census, ctx, wb, cap, true, 0, static_cast<int>(k), 2);
             if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
             if (persistent) {
                 if ((rc = launch_queue(ctx, wb, cap, true, 0, static_cast<int>(k))) != PULSE_OK) return rc;
@@ -888,6 +917,11 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
                 ms_shadow += t_sh;
             }
         }
+    }
+    if (census) {
+        HIP_TRY(hipMemcpyAsync(ctx->stats.work, ctx->census.ptr, sizeof(uint64_t) * 12, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     ctx->stats.primary_rays = ext_primary;
     ctx->stats.extension_rays = ext_total;
@@ -1073,6 +1107,7 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             uint32_t st = ctx->has_scene() ? stack_for(*ctx->cur, static_cast<int>(value)) : 1;
             if (st == 0) PULSE_FAIL(PULSE_E_UNSUPPORTED, "no stack size for this variant and scene");
             ctx->loop_variant = static_cast<int>(value);
+            ctx->loop_explicit = true;
             if (ctx->has_scene()) ctx->stack_entries = st;
             return PULSE_OK;
         }

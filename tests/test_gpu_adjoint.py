@@ -35,6 +35,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def dev():
     d = Device(0)
+    d.set_option(N.OPT_TRAVERSAL_LOOP, 1)  # the reference's traversal order: bit-exact vs the oracle
     yield d
     d.close()
 
@@ -50,10 +51,21 @@ def _cornell_case(spp=2, depth=3, size=48):
 MODES = [pytest.param(0, True, id="detached"), pytest.param(N.FLAG_ADJOINT_REPARAM, False, id="reparam")]
 
 
+@pytest.mark.parametrize("loop", [1, 2])
 @pytest.mark.parametrize("extra_flags,frozen", MODES)
-def test_replay_matches_forward_and_oracle_gradient(dev, extra_flags, frozen):
+def test_replay_matches_forward_and_oracle_gradient(dev, extra_flags, frozen, loop):
+    """The replay walks the forward pass's BVHs (loop 2: the 4-wide ones), so
+    it reproduces the forward image bit for bit either way."""
     scene, cam, prm = _cornell_case()
     prm.flags |= extra_flags
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        _replay_case(dev, scene, cam, prm, frozen)
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
+def _replay_case(dev, scene, cam, prm, frozen):
     dev.upload_scene(scene)
     nm = len(scene.buffers["materials"])
     adj = PulseAdjoint(dev, cam.width, cam.height, nm)

@@ -34,7 +34,10 @@ def views(dev):
     return [(s_c, cam_c(64, 48)), (s_1, cam_f1(37, 29)), (s_2, cam_f2(96, 64)), (s_c, cam_c(17, 80))]
 
 
-def _sequential(dev, views, params, acc_list):
+def _sequential(dev, views, params, acc_list, loop=2):
+    """pulse_trace of each view alone with the batch's traversal loop (0 and
+    1 give the same results)."""
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
     out = []
     for sc, cam in views:
         dev.bind_scene(sc)
@@ -62,6 +65,7 @@ def test_batch_equals_sequential(dev, views, pkw):
     prm = make_params(**pkw)
     ref = _sequential(dev, views, prm, [0])
     b = Batch(dev, views, prm)
+    assert b.config().traversal_loop == 2
     b.render(0)
     for i, (img, r) in enumerate(zip(b.images(), ref)):
         assert same_bits(img, r).all(), f"view {i}"
@@ -70,6 +74,8 @@ def test_batch_equals_sequential(dev, views, pkw):
 
 CONFIGS = [
     dict(persistent_waves_per_cu=0, chunk_rays=64),
+    dict(persistent_waves_per_cu=24, traversal_loop=1),
+    dict(persistent_waves_per_cu=0, chunk_rays=128, traversal_loop=1),
     dict(persistent_waves_per_cu=8, chunk_rays=128, traversal_loop=0),
     dict(persistent_waves_per_cu=16, chunk_rays=64, regroup_every=2),
     dict(persistent_waves_per_cu=0, chunk_rays=256, regroup_every=3, tile_width=16),
@@ -84,12 +90,12 @@ CONFIGS = [
 @pytest.mark.parametrize("cfg", CONFIGS, ids=[str(i) for i in range(len(CONFIGS))])
 def test_launch_shapes_keep_images(dev, views, cfg):
     prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
-    ref = _sequential(dev, views, prm, [0])
     b = Batch(dev, views, prm)
     b.set_config(**cfg)
     got = b.config().as_dict()
     for k, v in cfg.items():
         assert got[k] == v
+    ref = _sequential(dev, views, prm, [0], loop=got["traversal_loop"])
     b.render(0)
     for i, (img, r) in enumerate(zip(b.images(), ref)):
         assert same_bits(img, r).all(), f"view {i} with {cfg}"
@@ -141,10 +147,10 @@ def test_set_camera_and_stats(dev, views):
 
 def test_autotune_keeps_images(dev, views):
     prm = make_params(spp=1, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
-    ref = _sequential(dev, views, prm, [0])
     b = Batch(dev, views, prm)
     cfg, ms = b.autotune(reps=1)
     assert ms > 0 and cfg.chunk_rays in (64, 128, 256, 512, 1024, 2048, 4096) and cfg.stack_entries >= 24
+    ref = _sequential(dev, views, prm, [0], loop=cfg.traversal_loop)
     b.render(0)
     for i, (img, r) in enumerate(zip(b.images(), ref)):
         assert same_bits(img, r).all(), f"view {i} after autotune {cfg.as_dict()}"

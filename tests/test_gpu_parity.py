@@ -25,6 +25,7 @@ MODES = [(N.MODE_REF_GGX, N.NEE_LAMBERT, 5), (N.MODE_REF_NEE_LAMBERT, N.NEE_LAMB
 @pytest.fixture(scope="module")
 def dev():
     d = Device(0)
+    d.set_option(N.OPT_TRAVERSAL_LOOP, 1)  # the reference's traversal order: bit-exact vs the oracle
     yield d
     d.close()
 

@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def dev():
     d = Device(0)
+    d.set_option(N.OPT_TRAVERSAL_LOOP, 1)  # the reference's traversal order: bit-exact vs the oracle
     yield d
     d.close()
 
