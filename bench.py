@@ -230,7 +230,7 @@ def pmc_traffic(max_depth: int, loop: int) -> dict | None:
     device copy (bytes moved / counter reading).  Dispatches are told apart
     by their order within the frame: extend (trace index 0 = primary, then
     bounces) and shadow launches alternate with shading."""
-    frame_args = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-adjoint", "--no-batch", "--no-pmc",
+    frame_args = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-adjoint", "--no-batch", "--no-pmc", "--no-loop1",
                   "--max-depth", str(max_depth), "--loop", str(loop)]
     scale = {}
     per_kind = {}
@@ -249,8 +249,9 @@ def pmc_traffic(max_depth: int, loop: int) -> dict | None:
         sh = [float(r["Counter_Value"]) for r in rows if "k_trace_queue" in r["Kernel_Name"] and "true" in r["Kernel_Name"]]
         if not ext or not sh:
             return None
-        # the measured frame is the last max_depth extend / shadow dispatches
-        ext, sh = ext[-max_depth:], sh[-max_depth:]
+        # the measured frame is the first max_depth extend / shadow dispatches
+        # (a work-census frame may follow it)
+        ext, sh = ext[:max_depth], sh[:max_depth]
         per_kind.setdefault("extend_primary", {})[counter] = ext[0]
         per_kind.setdefault("extend_bounce", {})[counter] = sum(ext[1:]) / max(len(ext) - 1, 1)
         per_kind.setdefault("shadow", {})[counter] = sum(sh) / len(sh)
