@@ -300,6 +300,20 @@ int pulse_scene_create(pulse_ctx *ctx, const pulse_scene_desc *desc, pulse_scene
  * their BLAS and the light table must be unchanged (checked; otherwise
  * PULSE_E_LAYOUT: upload the scene).  Synchronises the context's stream. */
 int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse_scene_desc *desc);
+/* Instance motion without the host: `object_world` holds n = instance_count
+ * new object-to-world matrices (16 floats each, column-major, host memory,
+ * the MeshInstance field of scene/mod.rs:460-466).  On the device: the
+ * matrices and their inverses (cofactor expansion in double, as
+ * pulse_sb_build) go into the instance records, each instance's world box is
+ * recomputed from its BLAS root (scene/mod.rs:624-680), and the 4-wide TLAS
+ * is refit over the new boxes (the tree of the last build, new bounds) --
+ * the part of the reference's per-frame prepare_mesh_instances / build_tlas
+ * (scene/mod.rs:561-697, tlas.rs:29-136) that motion changes.  The
+ * reference's binary TLAS is not rebuilt: afterwards only traversal loop 2
+ * runs on the scene (loops 0 and 1 return PULSE_E_UNSUPPORTED until
+ * pulse_scene_update_instances or pulse_scene_upload).  Requires the scene's
+ * 4-wide BVHs.  Synchronises the context's stream. */
+int pulse_scene_refit_instances(pulse_ctx *ctx, pulse_scene *scene, const float *object_world, uint32_t n);
 int pulse_scene_destroy(pulse_scene *scene);
 int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
 

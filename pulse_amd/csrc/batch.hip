@@ -296,6 +296,7 @@ int refresh_scenes(pulse_batch *b) {
     bool changed = false;
     for (uint32_t i = 0; i < b->n_views; ++i) {
         const SceneData &sd = b->scenes[i]->data;
+        if (int rc = check_traversal(sd, static_cast<int>(b->cfg.traversal_loop))) return rc;
         if (sd.generation == b->scene_gen[i]) continue;
         if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_batch_render: view %u: its scene is not valid", i);
         const bool wide = b->cfg.traversal_loop == 2;

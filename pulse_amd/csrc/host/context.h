@@ -67,6 +67,9 @@ struct SceneData {
     uint32_t wide_need = 0;      // the same bound for the 4-wide BVHs
     uint32_t wide_blas_need = 0; // deepest 4-wide BLAS part of wide_need
     bool has_wide = false;       // the 4-wide BVHs were built
+    uint32_t n_wide_tlas = 0;    // quantised 4-wide TLAS nodes
+    bool binary_stale = false;   // a device refit moved instances: the binary TLAS is out of date
+    DevBuf refit[5];             // refit scratch: matrices, instance boxes, child boxes, node boxes, error word
     uint32_t tlas_depth = 0, blas_depth = 0;
     uint32_t n_materials = 0;
     uint32_t n_strength_cdf = 0;  // entries of light_emission_strength_cdf
@@ -81,6 +84,7 @@ struct SceneData {
     std::vector<pulse_light_data_index> light_indices;
     void release() {
         for (auto &b : buf) b.release();
+        for (auto &b : refit) b.release();
         valid = false;
     }
 };
@@ -90,6 +94,13 @@ int scene_upload(SceneData &s, const pulse_scene_desc *d);
 // Replaces the instances, TLAS and instance indices of an uploaded scene
 // (same meshes, same lights); validation failures leave `s` unchanged.
 int scene_update_instances(SceneData &s, const pulse_scene_desc *d);
+// Moves the instances of an uploaded scene on the device (refit.h): new
+// object_world matrices (16 floats each, column-major, host memory), their
+// inverses, world boxes and a refit of the 4-wide TLAS.
+int scene_refit_instances(SceneData &s, const float *object_world, uint32_t n, hipStream_t stream);
+// PULSE_E_UNSUPPORTED if traversal loop `loop` cannot run on `s` (the
+// binary TLAS is stale after a device refit).
+int check_traversal(const SceneData &s, int loop);
 // Device view of a scene; `errors` is the context's device error word.
 pt::DevScene dev_scene(const SceneData &s, uint32_t *errors);
 // Smallest compiled stack size that holds `need` entries (0: none).

@@ -11,6 +11,7 @@
 
 #include "device/adjoint.h"
 #include "device/present.h"
+#include "device/refit.h"
 #include "device/wavefront.h"
 #include "host/common.h"
 #include "host/context.h"
@@ -489,6 +490,8 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
     sd.wide_blas_need = wide_blas_need;
+    sd.n_wide_tlas = static_cast<uint32_t>(qtlas.size() / kQuantNodeWords);
+    sd.binary_stale = false;
     sd.uniform = d->uniform;
     sd.n_strength_cdf = static_cast<uint32_t>(n_scdf);
     sd.stack_need = need;
@@ -581,9 +584,59 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     }
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
+    sd.n_wide_tlas = static_cast<uint32_t>(qtlas.size() / kQuantNodeWords);
+    sd.binary_stale = false;
     sd.tlas_depth = tlas_depth;
     sd.stack_need = need;
     sd.fast_div = fast_div;
+    sd.generation += 1;
+    return PULSE_OK;
+}
+
+int check_traversal(const SceneData &s, int loop) {
+    if (loop != 2 && s.binary_stale)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "traversal loop %d: the scene's binary TLAS is out of date after a device "
+                   "refit (pulse_scene_refit_instances); use loop 2 or pulse_scene_update_instances", loop);
+    return PULSE_OK;
+}
+
+int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, hipStream_t stream) {
+    if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_scene_refit_instances: no scene uploaded");
+    if (!sd.has_wide)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_scene_refit_instances: the scene has no 4-wide BVHs to refit");
+    if (n != sd.uniform.instance_count || 4ull * n != sd.inst_ranges.size())
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: %u matrices for %u instances", n,
+                   sd.uniform.instance_count);
+    if (n == 0) return PULSE_OK;
+    if (!object_world) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: null matrices");
+    const uint32_t nodes = sd.n_wide_tlas;
+    int rc;
+    if ((rc = sd.refit[0].ensure(sizeof(float) * 16 * n)) || (rc = sd.refit[1].ensure(sizeof(float) * 6 * n)) ||
+        (rc = sd.refit[2].ensure(sizeof(float) * 24 * nodes)) || (rc = sd.refit[3].ensure(sizeof(float) * 6 * nodes)) ||
+        (rc = sd.refit[4].ensure(sizeof(uint32_t))))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(stream));  // launches in flight read the current instances
+    HIP_TRY(hipMemcpy(sd.refit[0].ptr, object_world, sizeof(float) * 16 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(sd.refit[4].ptr, 0, sizeof(uint32_t), stream));
+    uint32_t *err = sd.refit[4].as<uint32_t>();
+    hipLaunchKernelGGL(k_refit_instances, dim3((n + 63) / 64), dim3(64), 0, stream, sd.refit[0].as<float>(),
+                       sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.buf[B_BLAS].as<float4>(),
+                       sd.refit[1].as<float>(), n, err);
+    hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(1), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(), nodes,
+                       sd.buf[B_INST_IDX].as<uint32_t>(), sd.refit[1].as<float>(), sd.refit[2].as<float>(),
+                       sd.refit[3].as<float>());
+    hipLaunchKernelGGL(k_refit_quantize, dim3((nodes + 63) / 64), dim3(64), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(),
+                       nodes, sd.refit[2].as<float>(), err);
+    HIP_TRY(hipGetLastError());
+    uint32_t flags = 0;
+    HIP_TRY(hipMemcpyAsync(&flags, err, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (flags) {
+        sd.valid = false;  // partly rewritten: unusable until the next upload
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: a singular transform or a world box beyond "
+                   "2^40 (the scene must be uploaded again)");
+    }
+    sd.binary_stale = true;
     sd.generation += 1;
     return PULSE_OK;
 }
@@ -752,6 +805,20 @@ int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse
     return PULSE_OK;
 }
 
+int pulse_scene_refit_instances(pulse_ctx *ctx, pulse_scene *scene, const float *object_world, uint32_t n) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: null ctx");
+    if (scene && scene->ctx != ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: scene of another context");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SceneData *sd = scene ? &scene->data : &ctx->own;
+    if ((rc = scene_refit_instances(*sd, object_world, n, ctx->stream)) != PULSE_OK) {
+        if (ctx->cur == sd && !sd->valid) ctx->cur = nullptr;
+        return rc;
+    }
+    if (ctx->cur == sd) ctx->generation += 1;
+    return PULSE_OK;
+}
+
 int pulse_scene_create(pulse_ctx *ctx, const pulse_scene_desc *desc, pulse_scene **out) {
     if (!ctx || !desc || !out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_create: null argument");
     int rc = set_device(ctx);
@@ -790,6 +857,7 @@ int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene) {
 int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uint32_t n) {
     if (!ctx || (n && (!rays || !hits))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_rays: bad argument");
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_rays: no scene uploaded");
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     int rc = set_device(ctx);
     if (rc) return rc;
     if (n == 0) return PULSE_OK;
@@ -801,6 +869,7 @@ int pulse_trace_rays(pulse_ctx *ctx, const pulse_ray *rays, pulse_hit *hits, uin
 int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occluded, uint32_t n) {
     if (!ctx || (n && (!rays || !occluded))) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_shadow_rays: bad argument");
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_shadow_rays: no scene uploaded");
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     int rc = set_device(ctx);
     if (rc) return rc;
     if (n == 0) return PULSE_OK;
@@ -812,6 +881,7 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
 int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
     if (!ctx || !cam || !prm || !accum_rgba) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: null argument");
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace: no scene uploaded");
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: empty image");
     if (prm->mode > PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown mode %u", prm->mode);
     if (prm->nee_eval > PULSE_NEE_GGX) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown nee_eval %u", prm->nee_eval);
@@ -944,6 +1014,7 @@ int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width,
     if (!ctx || !gb || !prm || !gi_rgba || !gb->position || !gb->normal || !gb->base_color || !gb->emissive)
         PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_gbuffer: null argument");
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace_gbuffer: no scene uploaded");
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     if (width == 0 || height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_gbuffer: empty image");
     if (prm->max_depth > static_cast<uint32_t>(kMaxTraces - 1))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_gbuffer: max_depth %u above %d", prm->max_depth, kMaxTraces - 1);
@@ -1043,6 +1114,7 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
                   const float *target, float *grad, float *replay_rgba) {
     if (!ctx || !cam || !prm || !image || !target || !grad) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: null argument");
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs PULSE_MODE_GGX_NEE");
     if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVertices);

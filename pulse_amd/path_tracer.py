@@ -79,6 +79,15 @@ class Device:
         N.check(N.lib().pulse_scene_update_instances(self._ctx, target.handle if target is not None else None,
                                                      C.byref(d)))
 
+    def refit_instances(self, object_world: np.ndarray, target: "DeviceScene" = None) -> None:
+        """Instance motion on the device (pulse_scene_refit_instances): new
+        object-to-world matrices, (n, 4, 4) column-major as the instance
+        records store them (or (n, 16)); the 4-wide TLAS is refit, after
+        which only traversal loop 2 runs on the scene."""
+        m = np.ascontiguousarray(object_world, dtype=np.float32).reshape(-1, 16)
+        N.check(N.lib().pulse_scene_refit_instances(self._ctx, target.handle if target is not None else None,
+                                                    m.ctypes.data, len(m)))
+
     def bind_scene(self, scene: "DeviceScene" = None) -> None:
         """Makes `scene` (None: the uploaded scene) the one pulse_trace uses."""
         N.check(N.lib().pulse_scene_bind(self._ctx, scene.handle if scene is not None else None))

@@ -1,0 +1,116 @@
+"""Instance motion on the device (pulse_scene_refit_instances): new
+transforms, their inverses and the 4-wide TLAS refit on the GPU.  A render
+after the refit equals the oracle's render of the same scene rebuilt on the
+host (pulse_sb_update_instances) up to t-ties -- the refit tree differs from
+a rebuilt one, the closest hits do not -- and the instance records are the
+host builder's (a different inverse would move every object-space ray and
+show in every pixel).  The reference's binary TLAS is stale afterwards, so
+traversal loops 0/1 are refused until a host update."""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleScene, same_bits
+from pulse_amd import _native as N
+from pulse_amd import procedural as P
+from pulse_amd.path_tracer import Batch, Device, PulsePathTracerNode, make_params
+
+from test_instance_update import _make, _transforms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = Device(0)
+    d.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+    yield d
+    d.close()
+
+
+def _moved(idx=(0, 3, 5, 17, 23)):
+    a = _make(_transforms(24, 1), keep=True)
+    moved = _transforms(24, 2)
+    for i in idx:
+        a.builder.set_instance_transform(i, moved[i])
+    return a, a.builder.update_instances(a)
+
+
+def _cam():
+    return P.make_camera(64, 48, eye=(0.0, 0.0, 40.0), target=(0.0, 0.0, 0.0))
+
+
+def _render(dev, cam, prm):
+    node = PulsePathTracerNode(dev, cam.width, cam.height)
+    node.run(cam, prm)
+    return node.readback()
+
+
+def _few_differ(g, o):
+    px = ~same_bits(g, o).all(axis=-1)
+    assert px.mean() <= 1e-3, f"{px.sum()} of {px.size} pixels differ"
+
+
+@pytest.mark.parametrize("idx", [(0, 3, 5, 17, 23), tuple(range(24))], ids=["five", "all"])
+def test_refit_equals_host_rebuild_and_oracle(dev, idx):
+    a, b = _moved(idx)
+    cam = _cam()
+    prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
+    dev.upload_scene(a)
+    before = _render(dev, cam, prm)
+    dev.refit_instances(b.buffers["instances"]["object_world"])
+    after = _render(dev, cam, prm)
+    assert not same_bits(before, after).all()
+    o, _, _ = OracleScene(b).render(cam, prm)
+    _few_differ(after, o)
+    dev.update_instances(b)  # the host's rebuilt TLAS: another tree, the same hits
+    _few_differ(_render(dev, cam, prm), after)
+
+
+def test_refit_stales_the_binary_tlas(dev):
+    a, b = _moved()
+    cam = _cam()
+    prm = make_params(spp=1, max_depth=3, mode=N.MODE_GGX_NEE)
+    dev.upload_scene(a)
+    dev.refit_instances(b.buffers["instances"]["object_world"])
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+    try:
+        with pytest.raises(N.PulseError, match="UNSUPPORTED"):
+            _render(dev, cam, prm)
+        dev.update_instances(b)  # the reference's rebuild: loop 1 runs again, bit-exact
+        o, _, _ = OracleScene(b).render(cam, prm)
+        assert same_bits(_render(dev, cam, prm), o).all()
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+
+
+def test_refit_rejects_bad_input(dev):
+    a, b = _moved()
+    dev.upload_scene(a)
+    m = b.buffers["instances"]["object_world"]
+    with pytest.raises(N.PulseError, match="matrices"):
+        dev.refit_instances(m[:-1])
+    bad = m.copy()
+    bad[4] = 0.0  # singular
+    with pytest.raises(N.PulseError, match="singular"):
+        dev.refit_instances(bad)
+    with pytest.raises(N.PulseError):
+        _render(dev, _cam(), make_params(max_depth=2))  # partly rewritten: upload again
+    dev.upload_scene(a)
+    _render(dev, _cam(), make_params(max_depth=2))
+
+
+def test_batch_sees_the_refit(dev):
+    a, b = _moved()
+    cam = _cam()
+    prm = make_params(spp=1, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=1)
+    sa = dev.create_scene(a)
+    batch = Batch(dev, [(sa, cam)], prm)
+    batch.render(0)
+    first = batch.images()[0]
+    dev.refit_instances(b.buffers["instances"]["object_world"], target=sa)
+    batch.render(0)
+    got = batch.images()[0]
+    assert not same_bits(first, got).all()
+    dev.bind_scene(sa)
+    assert same_bits(got, _render(dev, cam, prm)).all()
+    batch.close()
