@@ -62,11 +62,12 @@ __device__ inline bool invert4_dev(const float *mf, float *outf) {
 }
 
 // One thread per instance: the new object_world, its inverse, and the world
-// box of the instance's BLAS root (its 8 corners transformed as the host
-// builder does, scene/mod.rs:624-680) into box[6 i .. 6 i + 5].
+// box of the instance's BLAS cut boxes (cut_range[2 i], [2 i + 1] into
+// cut_boxes: the corners transformed as the host's instance_world_box does,
+// widened by 2^-20 of their magnitude) into box[6 i .. 6 i + 5].
 __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse_mesh_instance *inst,
-                                                        const float4 *blas_nodes, float *box, uint32_t n,
-                                                        uint32_t *err) {
+                                                        const float *cut_boxes, const uint32_t *cut_range,
+                                                        float *box, uint32_t n, uint32_t *err) {
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= n) return;
     float m[16], wo[16];
@@ -80,24 +81,27 @@ __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse
         mi.object_world[j] = m[j];
         mi.world_object[j] = wo[j];
     }
-    const float4 lo = blas_nodes[2u * mi.node_offset], hi = blas_nodes[2u * mi.node_offset + 1u];
     float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
     float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
-    for (int c = 0; c < 8; ++c) {
-        const float px = (c & 1) ? hi.x : lo.x, py = (c & 2) ? hi.y : lo.y, pz = (c & 4) ? hi.z : lo.z;
-        const float hx = m[0] * px + m[4] * py + m[8] * pz + m[12] * 1.0f;
-        const float hy = m[1] * px + m[5] * py + m[9] * pz + m[13] * 1.0f;
-        const float hz = m[2] * px + m[6] * py + m[10] * pz + m[14] * 1.0f;
-        const float hw = m[3] * px + m[7] * py + m[11] * pz + m[15] * 1.0f;
-        const float w[3] = {hx / hw, hy / hw, hz / hw};
-        for (int a = 0; a < 3; ++a) {
-            mn[a] = fminf(mn[a], w[a]);
-            mx[a] = fmaxf(mx[a], w[a]);
+    const uint32_t first = cut_range[2u * i], count = cut_range[2u * i + 1u];
+    for (uint32_t j = first; j < first + count; ++j) {
+        const float *b = cut_boxes + 6u * j;
+        for (int c = 0; c < 8; ++c) {
+            const float px = (c & 1) ? b[3] : b[0], py = (c & 2) ? b[4] : b[1], pz = (c & 4) ? b[5] : b[2];
+            const float hx = m[0] * px + m[4] * py + m[8] * pz + m[12] * 1.0f;
+            const float hy = m[1] * px + m[5] * py + m[9] * pz + m[13] * 1.0f;
+            const float hz = m[2] * px + m[6] * py + m[10] * pz + m[14] * 1.0f;
+            const float hw = m[3] * px + m[7] * py + m[11] * pz + m[15] * 1.0f;
+            const float w[3] = {hx / hw, hy / hw, hz / hw};
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = fminf(mn[a], w[a]);
+                mx[a] = fmaxf(mx[a], w[a]);
+            }
         }
     }
     for (int a = 0; a < 3; ++a) {
-        box[6u * i + a] = mn[a];
-        box[6u * i + 3u + a] = mx[a];
+        box[6u * i + a] = mn[a] - fabsf(mn[a]) * 0x1p-20f;
+        box[6u * i + 3u + a] = mx[a] + fabsf(mx[a]) * 0x1p-20f;
     }
 }
 

@@ -164,6 +164,83 @@ bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideT
     return b.ok;
 }
 
+void blas_cut_boxes(const uint32_t *nodes, int depth, std::vector<float> &out) {
+    std::vector<std::pair<uint32_t, int>> st{{0u, 0}};
+    while (!st.empty()) {
+        auto [i, d] = st.back();
+        st.pop_back();
+        const uint32_t *w = nodes + 8 * static_cast<size_t>(i);
+        if (w[7] == 0 && d < depth && w[3] != 0) {
+            st.push_back({w[3] + 1u, d + 1});
+            st.push_back({w[3], d + 1});
+            continue;
+        }
+        float b[6];
+        std::memcpy(b, w, 12);
+        std::memcpy(b + 3, w + 4, 12);
+        out.insert(out.end(), b, b + 6);
+    }
+}
+
+void instance_world_box(const float *m, const float *boxes, size_t n, float out[6]) {
+    float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (size_t j = 0; j < n; ++j) {
+        const float *b = boxes + 6 * j;
+        for (int c = 0; c < 8; ++c) {
+            const float px = (c & 1) ? b[3] : b[0], py = (c & 2) ? b[4] : b[1], pz = (c & 4) ? b[5] : b[2];
+            const float hx = m[0] * px + m[4] * py + m[8] * pz + m[12] * 1.0f;
+            const float hy = m[1] * px + m[5] * py + m[9] * pz + m[13] * 1.0f;
+            const float hz = m[2] * px + m[6] * py + m[10] * pz + m[14] * 1.0f;
+            const float hw = m[3] * px + m[7] * py + m[11] * pz + m[15] * 1.0f;
+            const float w[3] = {hx / hw, hy / hw, hz / hw};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], w[a]);
+                hi[a] = std::max(hi[a], w[a]);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        out[a] = lo[a] - std::fabs(lo[a]) * 0x1p-20f;
+        out[3 + a] = hi[a] + std::fabs(hi[a]) * 0x1p-20f;
+    }
+}
+
+void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box) {
+    const size_t n = t.words.size() / kWideNodeWords;
+    std::vector<float> nb(6 * n);
+    for (size_t i = n; i-- > 0;) {
+        uint32_t *w = t.words.data() + i * kWideNodeWords;
+        float box[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
+                        -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t e = w[24 + k];
+            if (e == kWideEmpty) continue;
+            float cb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
+                           -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+            if (e & kWideLeafBit) {
+                const uint32_t first = e & ((1u << 26) - 1u), count = ((e >> 26) & 31u) + 1u;
+                for (uint32_t j = first; j < first + count; ++j) {
+                    const float *b = inst_box + 6 * static_cast<size_t>(iidx[j]);
+                    for (int a = 0; a < 3; ++a) {
+                        cb[a] = std::min(cb[a], b[a]);
+                        cb[3 + a] = std::max(cb[3 + a], b[3 + a]);
+                    }
+                }
+            } else {
+                std::memcpy(cb, &nb[6 * static_cast<size_t>(e)], sizeof(cb));
+            }
+            for (int a = 0; a < 3; ++a) {
+                std::memcpy(w + 8 * a + k, &cb[a], 4);
+                std::memcpy(w + 8 * a + 4 + k, &cb[3 + a], 4);
+                box[a] = std::min(box[a], cb[a]);
+                box[3 + a] = std::max(box[3 + a], cb[3 + a]);
+            }
+        }
+        std::memcpy(&nb[6 * i], box, sizeof(box));
+    }
+}
+
 bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out) {
     const size_t n = in.words.size() / kWideNodeWords;
     const size_t base = out.size();

@@ -62,4 +62,25 @@ constexpr int kQuantNodeWords = 16;
 // 2^40 in magnitude.
 bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out);
 
+// Tighter world boxes for the TLAS leaves of the 4-wide traversal.  The
+// reference bounds an instance by the 8 transformed corners of its BLAS
+// root box (scene/mod.rs:624-680), which for a rotated mesh is up to 1.7x
+// wider per axis than the mesh; bounding the transformed corners of the
+// boxes of the BLAS cut at depth kCutDepth (every leaf lies under one of
+// them) is far closer.  The binary TLAS keeps the reference's boxes.
+constexpr int kCutDepth = 6;
+// The boxes (min xyz, max xyz) of the BLAS rooted at `nodes` (8 words per
+// node, child indices relative to it) cut at depth `depth` (leaves above it
+// count as they are), appended to `out`.
+void blas_cut_boxes(const uint32_t *nodes, int depth, std::vector<float> &out);
+// World box of an instance: the corners of `n` object boxes through the
+// column-major object_world `m` (transform_position with the divide by w,
+// in f32 as the host builder computes instance boxes), widened outwards by
+// 2^-20 of the coordinates' magnitude against that rounding.
+void instance_world_box(const float *m, const float *boxes, size_t n, float out[6]);
+// Recomputes the f32 child boxes of a 4-wide TLAS (`t`, collapsed from the
+// binary TLAS) from per-instance world boxes inst_box[6 * instance] over
+// instance_indices `iidx`, deepest nodes first.
+void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box);
+
 }  // namespace pulse_host

@@ -436,10 +436,32 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
                    tlas_depth, blas_depth);
 
     // --- 4-wide BVHs over the same leaves (traversal loop 2), quantised ---
+    // instance boxes for its TLAS: the transformed BLAS cut boxes
+    std::vector<float> cut_boxes, ibox(6 * std::max<size_t>(n_inst, 1));
+    std::vector<uint32_t> cut_range(2 * std::max<size_t>(n_inst, 1), 0u);
+    {
+        std::vector<std::array<uint32_t, 3>> seen;  // node_offset, first box, count
+        for (size_t i = 0; i < n_inst; ++i) {
+            const uint32_t off = inst[i].node_offset;
+            auto it = std::find_if(seen.begin(), seen.end(), [&](const auto &p) { return p[0] == off; });
+            if (it == seen.end()) {
+                const uint32_t first = static_cast<uint32_t>(cut_boxes.size() / 6);
+                blas_cut_boxes(reinterpret_cast<const uint32_t *>(blas + off), kCutDepth, cut_boxes);
+                seen.push_back({off, first, static_cast<uint32_t>(cut_boxes.size() / 6) - first});
+                it = seen.end() - 1;
+            }
+            cut_range[2 * i] = (*it)[1];
+            cut_range[2 * i + 1] = (*it)[2];
+            instance_world_box(inst[i].object_world, &cut_boxes[6 * static_cast<size_t>((*it)[1])], (*it)[2],
+                               &ibox[6 * i]);
+        }
+    }
     WideTree wtlas;
     std::vector<uint32_t> qtlas, wblas;
-    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas) &&
-                    quantize_wide(wtlas, qtlas);
+    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas);
+    if (has_wide && !empty_tlas) refit_wide_tlas(wtlas, iidx, ibox.data());
+    has_wide = has_wide && quantize_wide(wtlas, qtlas);
+    if (cut_boxes.empty()) cut_boxes.assign(6, 0.0f);
     std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
     uint32_t wide_blas_need = 0;
     {
@@ -487,6 +509,19 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
                      (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
                      (rc = up(B_WROOT, wroot.data(), wroot.size() * 4))))
         return rc;
+    {
+        auto up_refit = [&](int b, const void *src, size_t bytes) -> int {
+            int r = sd.refit[b].ensure(bytes);
+            if (r) return r;
+            HIP_TRY(hipMemcpy(sd.refit[b].ptr, src, bytes, hipMemcpyHostToDevice));
+            return PULSE_OK;
+        };
+        if ((rc = up_refit(5, cut_boxes.data(), cut_boxes.size() * 4)) ||
+            (rc = up_refit(6, cut_range.data(), cut_range.size() * 4)))
+            return rc;
+    }
+    sd.cut_boxes = std::move(cut_boxes);
+    sd.cut_range = std::move(cut_range);
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
     sd.wide_blas_need = wide_blas_need;
@@ -571,8 +606,15 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     };
     WideTree wtlas;
     std::vector<uint32_t> qtlas;
-    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas) &&
-                    quantize_wide(wtlas, qtlas);
+    bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas);
+    if (has_wide && !empty_tlas) {
+        std::vector<float> ibox(6 * n_inst);
+        for (size_t i = 0; i < n_inst; ++i)
+            instance_world_box(inst[i].object_world, &sd.cut_boxes[6 * static_cast<size_t>(sd.cut_range[2 * i])],
+                               sd.cut_range[2 * i + 1], &ibox[6 * i]);
+        refit_wide_tlas(wtlas, iidx, ibox.data());
+    }
+    has_wide = has_wide && quantize_wide(wtlas, qtlas);
     const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
     int rc;
@@ -620,8 +662,8 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
     HIP_TRY(hipMemsetAsync(sd.refit[4].ptr, 0, sizeof(uint32_t), stream));
     uint32_t *err = sd.refit[4].as<uint32_t>();
     hipLaunchKernelGGL(k_refit_instances, dim3((n + 63) / 64), dim3(64), 0, stream, sd.refit[0].as<float>(),
-                       sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.buf[B_BLAS].as<float4>(),
-                       sd.refit[1].as<float>(), n, err);
+                       sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.refit[5].as<float>(),
+                       sd.refit[6].as<uint32_t>(), sd.refit[1].as<float>(), n, err);
     hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(1), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(), nodes,
                        sd.buf[B_INST_IDX].as<uint32_t>(), sd.refit[1].as<float>(), sd.refit[2].as<float>(),
                        sd.refit[3].as<float>());

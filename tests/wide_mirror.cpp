@@ -62,11 +62,28 @@ struct Layout {
     uint32_t need = 0;
 };
 
+// The TLAS's instance boxes as the library makes them at upload: each
+// instance's BLAS cut at kCutDepth, transformed.
+std::vector<float> instance_boxes(const Scene &s) {
+    std::vector<float> ib(6 * std::max<size_t>(s.n_instances, 1));
+    for (size_t i = 0; i < s.n_instances; ++i) {
+        std::vector<float> cut;
+        pulse_host::blas_cut_boxes(reinterpret_cast<const uint32_t *>(s.blas_nodes + s.instances[i].node_offset),
+                                   pulse_host::kCutDepth, cut);
+        pulse_host::instance_world_box(s.instances[i].object_world, cut.data(), cut.size() / 6, &ib[6 * i]);
+    }
+    return ib;
+}
+
 bool build(const Scene &s, Layout &L) {
     WideTree t;
-    if (!pulse_host::collapse_wide(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u, t) ||
-        !pulse_host::quantize_wide(t, L.tlas))
-        return false;
+    if (!pulse_host::collapse_wide(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u, t)) return false;
+    const bool empty = s.tlas_nodes[0].a_or_first_instance == 0u && s.tlas_nodes[0].instance_count == 0u;
+    if (!empty) {
+        const std::vector<float> ib = instance_boxes(s);
+        pulse_host::refit_wide_tlas(t, s.instance_indices, ib.data());
+    }
+    if (!pulse_host::quantize_wide(t, L.tlas)) return false;
     uint32_t blas_need = 0;
     std::vector<std::array<uint32_t, 3>> built;
     L.root.assign(s.n_instances, 0);
@@ -228,10 +245,27 @@ int wide_mirror_trace(const pulse_scene_desc *d, const pulse_ray *rays, pulse_hi
 int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
     const Scene s = view(d);
     out[0] = out[1] = out[2] = 0;
-    auto check_tree = [&](const uint32_t *bin, size_t nb, uint32_t off) {
+    auto check_tree = [&](const uint32_t *bin, size_t nb, uint32_t off, bool tlas) {
         WideTree t;
         std::vector<uint32_t> q;
-        if (!pulse_host::collapse_wide(bin, nb, off, t) || !pulse_host::quantize_wide(t, q)) return false;
+        if (!pulse_host::collapse_wide(bin, nb, off, t)) return false;
+        if (tlas && !(nb == 1 && bin[3] == 0 && bin[7] == 0)) {
+            const std::vector<float> ib = instance_boxes(s);
+            pulse_host::refit_wide_tlas(t, s.instance_indices, ib.data());
+            for (size_t i = 0; i < s.n_instances; ++i) {
+                const float *b = &ib[6 * i];
+                // every triangle vertex of the instance, transformed, lies in its box
+                const pulse_mesh_instance &mi = s.instances[i];
+                for (uint32_t k = 0; k < mi.triangle_count; k += 7) {
+                    const pulse_primitive &pr = s.primitives[mi.triangle_offset + k];
+                    for (const float *v : {pr.p_first, pr.p_second, pr.p_third}) {
+                        const V3<float> w = transform_position<float>(mi.object_world, load3<float>(v));
+                        if (w.x < b[0] || w.y < b[1] || w.z < b[2] || w.x > b[3] || w.y > b[4] || w.z > b[5]) ++out[1];
+                    }
+                }
+            }
+        }
+        if (!pulse_host::quantize_wide(t, q)) return false;
         const size_t nodes = t.words.size() / pulse_host::kWideNodeWords;
         std::vector<std::pair<uint32_t, uint32_t>> wide_leaves;
         for (size_t i = 0; i < nodes; ++i) {
@@ -273,7 +307,7 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
         out[0] += wide_leaves.size();
         return true;
     };
-    if (!check_tree(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u)) return -1;
+    if (!check_tree(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u, true)) return -1;
     std::vector<std::pair<uint32_t, uint32_t>> seen;
     for (size_t i = 0; i < s.n_instances; ++i) {
         const uint32_t off = s.instances[i].node_offset, ioff = s.instances[i].index_offset;
@@ -283,7 +317,7 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
         size_t end = s.n_blas;
         for (size_t j = 0; j < s.n_instances; ++j)
             if (s.instances[j].node_offset > off) end = std::min<size_t>(end, s.instances[j].node_offset);
-        if (!check_tree(reinterpret_cast<const uint32_t *>(s.blas_nodes + off), end - off, ioff)) return -1;
+        if (!check_tree(reinterpret_cast<const uint32_t *>(s.blas_nodes + off), end - off, ioff, false)) return -1;
     }
     return 0;
 }
