@@ -61,30 +61,35 @@ __device__ inline bool invert4_dev(const float *mf, float *outf) {
     return true;
 }
 
-// One thread per instance: the new object_world, its inverse, and the world
-// box of the instance's BLAS cut boxes (cut_range[2 i], [2 i + 1] into
-// cut_boxes: the corners transformed as the host's instance_world_box does,
-// widened by 2^-20 of their magnitude) into box[6 i .. 6 i + 5].
+// One wave per instance (block i): lane 0 writes the new object_world and
+// its inverse; the lanes share the instance's BLAS cut boxes (cut_range[2 i],
+// [2 i + 1] into cut_boxes), transform their corners as the host's
+// instance_world_box does and reduce them to the world box, widened by 2^-20
+// of its magnitude, in box[6 i .. 6 i + 5].
 __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse_mesh_instance *inst,
                                                         const float *cut_boxes, const uint32_t *cut_range,
                                                         float *box, uint32_t n, uint32_t *err) {
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t i = blockIdx.x;
     if (i >= n) return;
-    float m[16], wo[16];
+    const uint32_t lane = threadIdx.x;
+    float m[16];
     for (int j = 0; j < 16; ++j) m[j] = mats[16u * i + j];
-    if (!invert4_dev(m, wo)) {
-        atomicOr(err, kErrRefit);
-        return;
-    }
-    pulse_mesh_instance &mi = inst[i];
-    for (int j = 0; j < 16; ++j) {
-        mi.object_world[j] = m[j];
-        mi.world_object[j] = wo[j];
+    if (lane == 0u) {
+        float wo[16];
+        if (invert4_dev(m, wo)) {
+            pulse_mesh_instance &mi = inst[i];
+            for (int j = 0; j < 16; ++j) {
+                mi.object_world[j] = m[j];
+                mi.world_object[j] = wo[j];
+            }
+        } else {
+            atomicOr(err, kErrRefit);
+        }
     }
     float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
     float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
     const uint32_t first = cut_range[2u * i], count = cut_range[2u * i + 1u];
-    for (uint32_t j = first; j < first + count; ++j) {
+    for (uint32_t j = first + lane; j < first + count; j += 64u) {
         const float *b = cut_boxes + 6u * j;
         for (int c = 0; c < 8; ++c) {
             const float px = (c & 1) ? b[3] : b[0], py = (c & 2) ? b[4] : b[1], pz = (c & 4) ? b[5] : b[2];
@@ -99,10 +104,16 @@ __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse
             }
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        box[6u * i + a] = mn[a] - fabsf(mn[a]) * 0x1p-20f;
-        box[6u * i + 3u + a] = mx[a] + fabsf(mx[a]) * 0x1p-20f;
-    }
+    for (int a = 0; a < 3; ++a)
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+        }
+    if (lane == 0u)
+        for (int a = 0; a < 3; ++a) {
+            box[6u * i + a] = mn[a] - fabsf(mn[a]) * 0x1p-20f;
+            box[6u * i + 3u + a] = mx[a] + fabsf(mx[a]) * 0x1p-20f;
+        }
 }
 
 // One thread: the boxes of every child of every wide TLAS node, deepest

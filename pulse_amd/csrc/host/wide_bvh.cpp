@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace pulse_host {
 namespace {
@@ -204,6 +205,23 @@ void instance_world_box(const float *m, const float *boxes, size_t n, float out[
         out[a] = lo[a] - std::fabs(lo[a]) * 0x1p-20f;
         out[3 + a] = hi[a] + std::fabs(hi[a]) * 0x1p-20f;
     }
+}
+
+void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxes, const uint32_t *cut_range,
+                          size_t n, float *out, int threads) {
+    auto run = [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i)
+            instance_world_box(m + m_stride * i, cut_boxes + 6 * static_cast<size_t>(cut_range[2 * i]),
+                               cut_range[2 * i + 1], out + 6 * i);
+    };
+    const size_t t = std::max<size_t>(1, std::min<size_t>(static_cast<size_t>(std::max(threads, 1)), n / 64));
+    if (t <= 1) {
+        run(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t k = 0; k < t; ++k) pool.emplace_back(run, n * k / t, n * (k + 1) / t);
+    for (auto &th : pool) th.join();
 }
 
 void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box) {

@@ -67,8 +67,9 @@ bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out);
 // root box (scene/mod.rs:624-680), which for a rotated mesh is up to 1.7x
 // wider per axis than the mesh; bounding the transformed corners of the
 // boxes of the BLAS cut at depth kCutDepth (every leaf lies under one of
-// them) is far closer.  The binary TLAS keeps the reference's boxes.
-constexpr int kCutDepth = 6;
+// them; up to 256 boxes) is far closer.  The binary TLAS keeps the
+// reference's boxes.
+constexpr int kCutDepth = 8;
 // The boxes (min xyz, max xyz) of the BLAS rooted at `nodes` (8 words per
 // node, child indices relative to it) cut at depth `depth` (leaves above it
 // count as they are), appended to `out`.
@@ -78,6 +79,11 @@ void blas_cut_boxes(const uint32_t *nodes, int depth, std::vector<float> &out);
 // in f32 as the host builder computes instance boxes), widened outwards by
 // 2^-20 of the coordinates' magnitude against that rounding.
 void instance_world_box(const float *m, const float *boxes, size_t n, float out[6]);
+// instance_world_box for n instances, instance i from matrix m + 16 i (or
+// the object_world of pulse_mesh_instance records, stride 40 floats) and its
+// cut range, on up to `threads` threads.
+void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxes, const uint32_t *cut_range,
+                          size_t n, float *out, int threads);
 // Recomputes the f32 child boxes of a 4-wide TLAS (`t`, collapsed from the
 // binary TLAS) from per-instance world boxes inst_box[6 * instance] over
 // instance_indices `iidx`, deepest nodes first.

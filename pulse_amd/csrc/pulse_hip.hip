@@ -272,6 +272,8 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
 
 namespace pulse_host {
 
+constexpr int kHostThreads = 8;  // host threads for the per-instance TLAS boxes
+
 // A depth-first traversal that pushes at most one far child per level never
 // holds more entries than the depth of the node it is at.
 uint32_t pick_stack(uint32_t need, int loop_variant) {
@@ -452,9 +454,10 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
             }
             cut_range[2 * i] = (*it)[1];
             cut_range[2 * i + 1] = (*it)[2];
-            instance_world_box(inst[i].object_world, &cut_boxes[6 * static_cast<size_t>((*it)[1])], (*it)[2],
-                               &ibox[6 * i]);
         }
+        if (n_inst)
+            instance_world_boxes(inst[0].object_world, sizeof(pulse_mesh_instance) / sizeof(float), cut_boxes.data(),
+                                 cut_range.data(), n_inst, ibox.data(), kHostThreads);
     }
     WideTree wtlas;
     std::vector<uint32_t> qtlas, wblas;
@@ -609,9 +612,8 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     bool has_wide = sd.has_wide && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas);
     if (has_wide && !empty_tlas) {
         std::vector<float> ibox(6 * n_inst);
-        for (size_t i = 0; i < n_inst; ++i)
-            instance_world_box(inst[i].object_world, &sd.cut_boxes[6 * static_cast<size_t>(sd.cut_range[2 * i])],
-                               sd.cut_range[2 * i + 1], &ibox[6 * i]);
+        instance_world_boxes(inst[0].object_world, sizeof(pulse_mesh_instance) / sizeof(float), sd.cut_boxes.data(),
+                             sd.cut_range.data(), n_inst, ibox.data(), kHostThreads);
         refit_wide_tlas(wtlas, iidx, ibox.data());
     }
     has_wide = has_wide && quantize_wide(wtlas, qtlas);
@@ -661,7 +663,7 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
     HIP_TRY(hipMemcpy(sd.refit[0].ptr, object_world, sizeof(float) * 16 * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemsetAsync(sd.refit[4].ptr, 0, sizeof(uint32_t), stream));
     uint32_t *err = sd.refit[4].as<uint32_t>();
-    hipLaunchKernelGGL(k_refit_instances, dim3((n + 63) / 64), dim3(64), 0, stream, sd.refit[0].as<float>(),
+    hipLaunchKernelGGL(k_refit_instances, dim3(n), dim3(64), 0, stream, sd.refit[0].as<float>(),
                        sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.refit[5].as<float>(),
                        sd.refit[6].as<uint32_t>(), sd.refit[1].as<float>(), n, err);
     hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(1), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(), nodes,
