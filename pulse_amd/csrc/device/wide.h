@@ -51,35 +51,55 @@ __device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
 // few 2^-24 of |origin - o| / |d| + 255 scale / |d| on that axis; the
 // constant term of the entry plane is lowered and that of the exit plane
 // raised by 2^-20 of it, so every computed slab contains the exact one.
+// Which plane of an axis is the entry is fixed by the sign of d on that
+// axis (the low plane when d >= 0), so the entry and exit plane bytes are
+// picked once per node instead of ordering each child's pair of distances:
+// both give the same distances (fma rounding is monotone and the padded
+// entry term never exceeds the exit term).
 struct QuantFrame {
-    float srx, sry, srz;              // scale / d per axis
-    float blx, bhx, bly, bhy, blz, bhz;  // (origin - o) / d for the low / high planes, padded
+    float srx, sry, srz;                 // scale / d per axis
+    float bnx, bfx, bny, bfy, bnz, bfz;  // (origin - o) / d for the entry / exit planes, padded
 };
-__device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bl, float &bh) {
+__device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bn, float &bf) {
     sr = s * r;
     const float b = (org - o) * r;
-    const float pad = copysignf(0x1p-20f * (fabsf(b) + 255.0f * fabsf(sr)), r);
-    bl = b - pad;  // r > 0: the low plane is the entry
-    bh = b + pad;
+    const float pad = 0x1p-20f * __builtin_fmaf(255.0f, fabsf(sr), fabsf(b));
+    bn = b - pad;
+    bf = b + pad;
 }
 __device__ __forceinline__ QuantFrame quant_frame(const WideRay &w, uint4 a) {
     QuantFrame f;
-    quant_axis(__uint_as_float((a.w & 0xffu) << 23), __uint_as_float(a.x), w.o.x, w.r.x, f.srx, f.blx, f.bhx);
-    quant_axis(__uint_as_float(((a.w >> 8) & 0xffu) << 23), __uint_as_float(a.y), w.o.y, w.r.y, f.sry, f.bly, f.bhy);
-    quant_axis(__uint_as_float(((a.w >> 16) & 0xffu) << 23), __uint_as_float(a.z), w.o.z, w.r.z, f.srz, f.blz, f.bhz);
+    quant_axis(__uint_as_float((a.w & 0xffu) << 23), __uint_as_float(a.x), w.o.x, w.r.x, f.srx, f.bnx, f.bfx);
+    quant_axis(__uint_as_float(((a.w >> 8) & 0xffu) << 23), __uint_as_float(a.y), w.o.y, w.r.y, f.sry, f.bny, f.bfy);
+    quant_axis(__uint_as_float(((a.w >> 16) & 0xffu) << 23), __uint_as_float(a.z), w.o.z, w.r.z, f.srz, f.bnz, f.bfz);
     return f;
+}
+
+// Plane bytes of the four children: entry (n*) and exit (f*) plane per axis.
+struct QuantPlanes {
+    uint32_t nx, fx, ny, fy, nz, fz;
+};
+__device__ __forceinline__ QuantPlanes quant_planes(const WideRay &w, uint4 b, uint4 c) {
+    QuantPlanes p;
+    const bool sx = w.r.x < 0.0f, sy = w.r.y < 0.0f, sz = w.r.z < 0.0f;
+    p.nx = sx ? b.y : b.x;
+    p.fx = sx ? b.x : b.y;
+    p.ny = sy ? b.w : b.z;
+    p.fy = sy ? b.z : b.w;
+    p.nz = sz ? c.y : c.x;
+    p.fz = sz ? c.x : c.y;
+    return p;
 }
 
 __device__ __forceinline__ float qbyte(uint32_t w, int k) { return static_cast<float>((w >> (8 * k)) & 0xffu); }
 
-// Slab test of child k (planes in bytes k of b and c): its entry distance,
-// or kTFar when the box is missed, behind the origin or at or beyond `bound`.
-__device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, uint4 b, uint4 c, int k) {
-    const float x0 = __builtin_fmaf(qbyte(b.x, k), f.srx, f.blx), x1 = __builtin_fmaf(qbyte(b.y, k), f.srx, f.bhx);
-    const float y0 = __builtin_fmaf(qbyte(b.z, k), f.sry, f.bly), y1 = __builtin_fmaf(qbyte(b.w, k), f.sry, f.bhy);
-    const float z0 = __builtin_fmaf(qbyte(c.x, k), f.srz, f.blz), z1 = __builtin_fmaf(qbyte(c.y, k), f.srz, f.bhz);
-    const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-    const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+// Slab test of child k (plane bytes k of p): its entry distance, or kTFar
+// when the box is missed, behind the origin or at or beyond `bound`.
+__device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, const QuantPlanes &p, int k) {
+    const float tmin = fmaxf(fmaxf(__builtin_fmaf(qbyte(p.nx, k), f.srx, f.bnx), __builtin_fmaf(qbyte(p.ny, k), f.sry, f.bny)),
+                             __builtin_fmaf(qbyte(p.nz, k), f.srz, f.bnz));
+    const float tmax = fminf(fminf(__builtin_fmaf(qbyte(p.fx, k), f.srx, f.bfx), __builtin_fmaf(qbyte(p.fy, k), f.sry, f.bfy)),
+                             __builtin_fmaf(qbyte(p.fz, k), f.srz, f.bfz));
     return (tmax >= tmin && tmin < bound && tmax > 0.0f) ? tmin : kTFar;
 }
 
@@ -213,11 +233,12 @@ struct WideTraversal {
                 reinterpret_cast<const uint4 *>(in_blas ? sc.wide_blas : sc.wide_tlas) + 4u * (nbase + entry);
             const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
             const QuantFrame f = quant_frame(cur, qa);
+            const QuantPlanes pl = quant_planes(cur, qb, qc);
             uint32_t e0 = qc.z, e1 = qc.w, e2 = qd.x, e3 = qd.y;
             // all four boxes are tested (an empty position's planes are
             // zero), then empty positions are dropped
-            float d0 = quant_slab(f, cur_t, qb, qc, 0), d1 = quant_slab(f, cur_t, qb, qc, 1);
-            float d2 = quant_slab(f, cur_t, qb, qc, 2), d3 = quant_slab(f, cur_t, qb, qc, 3);
+            float d0 = quant_slab(f, cur_t, pl, 0), d1 = quant_slab(f, cur_t, pl, 1);
+            float d2 = quant_slab(f, cur_t, pl, 2), d3 = quant_slab(f, cur_t, pl, 3);
             d0 = e0 != kWideEmpty ? d0 : kTFar;
             d1 = e1 != kWideEmpty ? d1 : kTFar;
             d2 = e2 != kWideEmpty ? d2 : kTFar;
