@@ -51,6 +51,7 @@ struct DevScene {
     const float4 *wide_tlas;            // 4-wide TLAS, 4 float4 per quantised node (null: not built)
     const float4 *wide_blas;            // all meshes' 4-wide BLASes
     const uint32_t *wide_root;          // per instance: first wide node of its BLAS
+    const float4 *wide_tri;             // tri_bvh rows, refined BLAS leaves reordered (wide.h: mt_select)
     uint32_t instance_count;
     uint32_t light_count;
     uint32_t *error_flags;

@@ -4,15 +4,19 @@
 // Same interface and the same closest-hit / occlusion definitions as
 // Traversal (traverse.h), fewer and wider steps: one step fetches one
 // 64-byte quantised node, tests its four child boxes, visits the nearest hit
-// child next and pushes the others (farthest first).  The leaves, the triangle
-// slots, the triangle test, the per-instance object-space closest hit and its
-// world-t conversion are the reference's (utilities.wgsl:259-326); only the
-// order in which leaves are reached differs from the reference's binary
-// near-first order.  That order decides which of two hits with exactly equal
-// t is kept (the first found), and a box whose computed entry distance is
-// rounded above a hit inside it; the quantised boxes contain the exact ones
-// and the box test is padded by more than its rounding error, so no box
-// holding a hit below the current bound is skipped.
+// child next and pushes the others (farthest first).  The triangle test, the
+// per-instance object-space closest hit and its world-t conversion are the
+// reference's (utilities.wgsl:259-326).  The reference's BLAS leaves of more
+// than two triangles are split at upload into leaves of at most two with
+// tighter boxes (host/wide_bvh.h: refine_leaves), and of two hits at exactly
+// equal t in one reference leaf the lower slot is kept, as the reference's
+// slot-order walk of that leaf keeps it; what differs from the reference is
+// the order in which its leaves are reached.  That order decides which of two
+// hits with exactly equal t in different reference leaves is kept (the first
+// found), and a box whose computed entry distance is rounded above a hit
+// inside it; the quantised boxes contain the exact ones and the box test is
+// padded by more than its rounding error, so no box holding a hit below the
+// current bound is skipped.
 #pragma once
 
 #include "pt_device.h"
@@ -106,10 +110,19 @@ __device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, co
 // Moeller-Trumbore exactly as mt() (pt_device.h) evaluates it, without its
 // early exits: every quantity is computed and the hit is taken when all of
 // mt()'s tests pass, so lanes of a wave stay together through the test.
+// `tri` rows are wide_tri's: the third float4 holds e2.z, the triangle's
+// tri_bvh slot and the end slot of the reference's leaf holding it.  h.tri
+// receives the tri_bvh slot.  With TIE_IN_LEAF a hit at exactly the current
+// t also replaces the current one when both lie in one reference leaf and
+// its slot is the lower: the reference tests a leaf in slot order and keeps
+// the first of equal hits, and a refined leaf's triangles come in another
+// order.
+template <bool TIE_IN_LEAF>
 __device__ __forceinline__ void mt_select(V3 o, V3 d, const float4 *tri, uint32_t slot, ObjHit &h) {
     const float4 A = tri[3 * slot];
     const float4 B = tri[3 * slot + 1];
     const float4 C = tri[3 * slot + 2];
+    const uint32_t ref = __float_as_uint(C.y), leaf_end = __float_as_uint(C.z);
     const V3 p0 = mk(A.x, A.y, A.z);
     const V3 e1 = mk(A.w, B.x, B.y);
     const V3 e2 = mk(B.z, B.w, C.x);
@@ -121,11 +134,12 @@ __device__ __forceinline__ void mt_select(V3 o, V3 d, const float4 *tri, uint32_
     const V3 q = cross(s, e1);
     const float v = f * dot(d, q);
     const float t = f * dot(e2, q);
+    const bool nearer = t < h.t || (TIE_IN_LEAF && t == h.t && ref < h.tri && h.tri < leaf_end);
     const bool hit = !(fabsf(a) < 0.0001f) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f) &&
-                     t > 0.001f && t < h.t;
+                     t > 0.001f && nearer;
     if (hit) {
         h.t = t;
-        h.tri = slot;
+        h.tri = ref;
         h.u = u;
         h.v = v;
         h.found = true;
@@ -271,13 +285,13 @@ struct WideTraversal {
             }
             tri = first | (count << 26);
         }
-        // BLAS leaf entries hold global slots; h.tri becomes the mesh's slot
-        // at the merge
-        mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
+        // BLAS leaf entries hold global slots of wide_tri; h.tri (a global
+        // tri_bvh slot) becomes the mesh's slot at the merge
+        mt_select<!ANY_HIT>(cur.o, cur.d, sc.wide_tri, tri & kRangeMask, h);
         tri += kRangeStep;
         n_tri += 1;
         if (TRIS_PER_STEP == 2 && tri >= kRangeOne) {
-            mt_select(cur.o, cur.d, sc.tri_bvh, tri & kRangeMask, h);
+            mt_select<!ANY_HIT>(cur.o, cur.d, sc.wide_tri, tri & kRangeMask, h);
             tri += kRangeStep;
             n_tri += 1;
         }

@@ -46,7 +46,7 @@ struct DevBuf {
 
 enum SceneBuf {
     B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
-    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_WTLAS, B_WBLAS, B_WROOT, B_COUNT
+    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_WTLAS, B_WBLAS, B_WROOT, B_WTRI, B_COUNT
 };
 
 // Queue counter words of the single-view path: extension and shadow queue

@@ -316,4 +316,175 @@ bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out) {
     return true;
 }
 
+namespace {
+
+struct TriBox {
+    float lo[3], hi[3];
+    double c[3];  // centroid
+    uint32_t slot;
+};
+
+float round_down(double x) {
+    float f = static_cast<float>(x);
+    return static_cast<double>(f) > x ? std::nextafter(f, -INFINITY) : f;
+}
+float round_up(double x) {
+    float f = static_cast<float>(x);
+    return static_cast<double>(f) < x ? std::nextafter(f, INFINITY) : f;
+}
+
+TriBox tri_box(const float *tri, uint32_t slot) {
+    const float *t = tri + 12 * static_cast<size_t>(slot);
+    TriBox b{};
+    b.slot = slot;
+    double lo[3], hi[3], ext = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double v0 = t[a], v1 = v0 + static_cast<double>(t[3 + a]), v2 = v0 + static_cast<double>(t[6 + a]);
+        lo[a] = std::min(v0, std::min(v1, v2));
+        hi[a] = std::max(v0, std::max(v1, v2));
+        b.c[a] = (v0 + v1 + v2) / 3.0;
+        ext = std::max(ext, hi[a] - lo[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        const double pad = std::ldexp(ext, -8) + std::ldexp(std::max(std::fabs(lo[a]), std::fabs(hi[a])), -20);
+        b.lo[a] = round_down(lo[a] - pad);
+        b.hi[a] = round_up(hi[a] + pad);
+    }
+    return b;
+}
+
+double half_area(const float lo[3], const float hi[3]) {
+    const double dx = static_cast<double>(hi[0]) - lo[0], dy = static_cast<double>(hi[1]) - lo[1],
+                 dz = static_cast<double>(hi[2]) - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Fills node `at` of `out` over items[0..cnt), whose triangles take the
+// slots first_offset + pos .. + cnt in the order of the leaves.
+void build_sub(std::vector<uint32_t> &out, uint32_t at, TriBox *items, uint32_t cnt, uint32_t pos,
+               uint32_t max_leaf, uint32_t first_offset, uint32_t *perm) {
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = items[0].lo[a];
+        hi[a] = items[0].hi[a];
+        for (uint32_t k = 1; k < cnt; ++k) {
+            lo[a] = std::min(lo[a], items[k].lo[a]);
+            hi[a] = std::max(hi[a], items[k].hi[a]);
+        }
+    }
+    {
+        uint32_t *w = out.data() + 8 * static_cast<size_t>(at);
+        std::memcpy(w, lo, 12);
+        std::memcpy(w + 4, hi, 12);
+    }
+    if (cnt <= max_leaf) {
+        out[8 * static_cast<size_t>(at) + 3] = pos;
+        out[8 * static_cast<size_t>(at) + 7] = cnt;
+        for (uint32_t k = 0; k < cnt; ++k) perm[static_cast<size_t>(first_offset) + pos + k] = items[k].slot;
+        return;
+    }
+    // least area-weighted cost over the centroid orders of the three axes
+    int best_axis = 0;
+    uint32_t best_k = cnt / 2;
+    double best = INFINITY;
+    std::vector<double> right(cnt);
+    for (int a = 0; a < 3; ++a) {
+        std::sort(items, items + cnt, [a](const TriBox &x, const TriBox &y) {
+            return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.slot < y.slot);
+        });
+        float rl[3], rh[3];
+        for (uint32_t k = cnt; k-- > 1;) {
+            for (int b = 0; b < 3; ++b) {
+                rl[b] = k == cnt - 1 ? items[k].lo[b] : std::min(rl[b], items[k].lo[b]);
+                rh[b] = k == cnt - 1 ? items[k].hi[b] : std::max(rh[b], items[k].hi[b]);
+            }
+            right[k] = half_area(rl, rh) * (cnt - k);
+        }
+        float ll[3], lh[3];
+        for (uint32_t k = 1; k < cnt; ++k) {
+            for (int b = 0; b < 3; ++b) {
+                ll[b] = k == 1 ? items[0].lo[b] : std::min(ll[b], items[k - 1].lo[b]);
+                lh[b] = k == 1 ? items[0].hi[b] : std::max(lh[b], items[k - 1].hi[b]);
+            }
+            const double cost = half_area(ll, lh) * k + right[k];
+            if (cost < best) {
+                best = cost;
+                best_axis = a;
+                best_k = k;
+            }
+        }
+    }
+    std::sort(items, items + cnt, [best_axis](const TriBox &x, const TriBox &y) {
+        return x.c[best_axis] < y.c[best_axis] || (x.c[best_axis] == y.c[best_axis] && x.slot < y.slot);
+    });
+    const uint32_t child = static_cast<uint32_t>(out.size() / 8);
+    out.resize(out.size() + 16, 0u);
+    out[8 * static_cast<size_t>(at) + 3] = child;
+    out[8 * static_cast<size_t>(at) + 7] = 0u;
+    build_sub(out, child, items, best_k, pos, max_leaf, first_offset, perm);
+    build_sub(out, child + 1, items + best_k, cnt - best_k, pos + best_k, max_leaf, first_offset, perm);
+}
+
+}  // namespace
+
+void count_leaf_refs(const uint32_t *nodes, size_t n, uint32_t first_offset, std::vector<uint8_t> &refs,
+                     std::vector<uint32_t> *leaf_end) {
+    if (n == 0) return;
+    std::vector<uint32_t> st{0u};
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const uint32_t *w = nodes + 8 * static_cast<size_t>(i);
+        if (w[7] == 0) {
+            if (w[3] != 0 && static_cast<size_t>(w[3]) + 1 < n) {
+                st.push_back(w[3]);
+                st.push_back(w[3] + 1u);
+            }
+            continue;
+        }
+        const size_t end = static_cast<size_t>(first_offset) + w[3] + w[7];
+        for (uint32_t k = 0; k < w[7]; ++k) {
+            const size_t g = static_cast<size_t>(first_offset) + w[3] + k;
+            if (g < refs.size() && refs[g] < 255) ++refs[g];
+            if (leaf_end && g < leaf_end->size()) (*leaf_end)[g] = static_cast<uint32_t>(end);
+        }
+    }
+}
+
+void refine_leaves(const uint32_t *nodes, size_t n, uint32_t first_offset, const float *tri, uint32_t max_leaf,
+                   const std::vector<uint8_t> &refs, std::vector<uint32_t> &out, uint32_t *perm) {
+    out.clear();
+    if (n == 0) return;
+    size_t top = 0;
+    std::vector<uint32_t> leaves, st{0u};
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        top = std::max<size_t>(top, i);
+        const uint32_t *w = nodes + 8 * static_cast<size_t>(i);
+        if (w[7] == 0) {
+            if (w[3] != 0 && static_cast<size_t>(w[3]) + 1 < n) {
+                st.push_back(w[3]);
+                st.push_back(w[3] + 1u);
+            }
+        } else {
+            leaves.push_back(i);
+        }
+    }
+    out.assign(nodes, nodes + 8 * (top + 1));
+    std::vector<TriBox> items;
+    for (uint32_t leaf : leaves) {
+        const uint32_t first = nodes[8 * static_cast<size_t>(leaf) + 3], count = nodes[8 * static_cast<size_t>(leaf) + 7];
+        if (count <= max_leaf || count > kRefineMaxLeaf) continue;
+        const size_t g0 = static_cast<size_t>(first_offset) + first;
+        if (g0 + count > refs.size()) continue;
+        bool own = true;
+        for (uint32_t k = 0; k < count && own; ++k) own = refs[g0 + k] == 1;
+        if (!own) continue;
+        items.clear();
+        for (uint32_t k = 0; k < count; ++k) items.push_back(tri_box(tri, static_cast<uint32_t>(g0 + k)));
+        build_sub(out, leaf, items.data(), count, first, max_leaf, first_offset, perm);
+    }
+}
+
 }  // namespace pulse_host

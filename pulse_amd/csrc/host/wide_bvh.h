@@ -89,4 +89,40 @@ void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxe
 // instance_indices `iidx`, deepest nodes first.
 void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box);
 
+// Leaf refinement of the 4-wide BLASes.  The reference's builder stops
+// splitting at 8 triangles or fewer (scene/blas.rs:97); the 4-wide
+// traversal tests up to two triangles per step, so every binary leaf of
+// more than `max_leaf` (and at most kRefineMaxLeaf) triangles becomes a
+// binary subtree whose leaves hold at most `max_leaf`, split where the
+// surface-area cost (area x triangles, summed over the two sides) over the
+// triangles' centroids sorted along one axis is least.  A triangle's box is
+// that of p0, p0 + e1, p0 + e2 (the triangle test's own operands) widened
+// by 2^-8 of its longest side plus 2^-20 of the coordinates' magnitude,
+// so that the points the triangle test accepts just outside the exact
+// triangle stay inside.  The triangles of a refined leaf are reordered
+// within the leaf's slot range: refinement yields, per global slot g, the
+// slot whose triangle sits at g (the identity outside refined leaves).
+constexpr uint32_t kRefineMaxLeaf = 64;
+constexpr uint32_t kRefineLeaf = 2;  // the upload's max_leaf: a traversal step tests up to two triangles
+
+// Adds, per global slot, the number of leaves of the binary tree `nodes`
+// (collapse_wide's form; `n` nodes) that hold it: refs[first_offset +
+// first + k] (saturating at 255).  A leaf whose slots other leaves share is
+// never refined (its reordering would move their triangles).  With
+// `leaf_end`, also stores there the global end slot of the (last) leaf
+// holding each slot: of two hits at equal t in one binary leaf the
+// reference keeps the lower slot (it tests a leaf in slot order), which
+// the 4-wide traversal reproduces from this.
+void count_leaf_refs(const uint32_t *nodes, size_t n, uint32_t first_offset, std::vector<uint8_t> &refs,
+                     std::vector<uint32_t> *leaf_end = nullptr);
+
+// Copy of the binary tree `nodes` (node 0 the root, `n` nodes) into `out`
+// (8 words per node) with each refinable leaf turned into an interior node
+// over its appended subtree; `tri` holds 12 floats per global slot (p0, e1,
+// e2, 3 unused) and `perm` (one entry per global slot, set to the identity
+// beforehand) receives the reordering of the refined leaves' slots.  Leaves
+// that any slot of theirs shares with another leaf (refs > 1) stay whole.
+void refine_leaves(const uint32_t *nodes, size_t n, uint32_t first_offset, const float *tri, uint32_t max_leaf,
+                   const std::vector<uint8_t> &refs, std::vector<uint32_t> &out, uint32_t *perm);
+
 }  // namespace pulse_host

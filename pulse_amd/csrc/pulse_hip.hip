@@ -5,8 +5,10 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "device/adjoint.h"
@@ -272,7 +274,7 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
 
 namespace pulse_host {
 
-constexpr int kHostThreads = 8;  // host threads for the per-instance TLAS boxes
+constexpr int kHostThreads = 8;  // host threads for the per-instance TLAS boxes and the wide BLASes
 
 // A depth-first traversal that pushes at most one far child per level never
 // holds more entries than the depth of the node it is at.
@@ -311,6 +313,7 @@ DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     s.wide_tlas = sd.has_wide ? sd.buf[B_WTLAS].as<float4>() : nullptr;
     s.wide_blas = sd.has_wide ? sd.buf[B_WBLAS].as<float4>() : nullptr;
     s.wide_root = sd.has_wide ? sd.buf[B_WROOT].as<uint32_t>() : nullptr;
+    s.wide_tri = sd.has_wide ? sd.buf[B_WTRI].as<float4>() : nullptr;
     s.instance_count = sd.uniform.instance_count;
     s.light_count = sd.uniform.light_count;
     s.error_flags = errors;
@@ -467,27 +470,66 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     if (cut_boxes.empty()) cut_boxes.assign(6, 0.0f);
     std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
     uint32_t wide_blas_need = 0;
-    {
+    // the refined leaves' triangle order: wperm[g] = tri_bvh slot at wide slot g
+    std::vector<uint32_t> wperm(std::max<size_t>(n_tidx, 1)), leaf_end(std::max<size_t>(n_tidx, 1), 0u);
+    for (size_t g = 0; g < wperm.size(); ++g) wperm[g] = static_cast<uint32_t>(g);
+    if (has_wide) {
         // one tree per (node_offset, index_offset): leaf entries hold global slots
-        std::vector<std::array<uint32_t, 3>> built;  // node_offset, index_offset, first wide node
-        for (size_t i = 0; i < n_inst && has_wide; ++i) {
-            const uint32_t off = inst[i].node_offset, ioff = inst[i].index_offset;
-            auto it = std::find_if(built.begin(), built.end(),
-                                   [&](const auto &p) { return p[0] == off && p[1] == ioff; });
-            if (it != built.end()) {
-                wroot[i] = (*it)[2];
-                continue;
-            }
-            WideTree t;
-            const uint32_t root = static_cast<uint32_t>(wblas.size() / kQuantNodeWords);
-            has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(blas + off), n_blas - off, ioff, t) &&
-                       quantize_wide(t, wblas);
-            wide_blas_need = std::max(wide_blas_need, t.stack_need);
-            built.push_back({off, ioff, root});
-            wroot[i] = root;
+        std::vector<std::array<uint32_t, 2>> trees;
+        std::vector<uint32_t> tree_of(n_inst);
+        for (size_t i = 0; i < n_inst; ++i) {
+            const std::array<uint32_t, 2> key{inst[i].node_offset, inst[i].index_offset};
+            auto it = std::find(trees.begin(), trees.end(), key);
+            tree_of[i] = static_cast<uint32_t>(it - trees.begin());
+            if (it == trees.end()) trees.push_back(key);
         }
+        std::vector<uint8_t> refs(n_tidx, 0);
+        for (const auto &t : trees)
+            count_leaf_refs(reinterpret_cast<const uint32_t *>(blas + t[0]), n_blas - t[0], t[1], refs, &leaf_end);
+        std::vector<std::vector<uint32_t>> q(trees.size());
+        std::vector<uint32_t> t_need(trees.size(), 0);
+        std::vector<char> t_ok(trees.size(), 0);
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            std::vector<uint32_t> nodes;
+            for (size_t k; (k = next.fetch_add(1)) < trees.size();) {
+                refine_leaves(reinterpret_cast<const uint32_t *>(blas + trees[k][0]), n_blas - trees[k][0], trees[k][1],
+                              reinterpret_cast<const float *>(tri_bvh.data()), kRefineLeaf, refs, nodes, wperm.data());
+                WideTree w;
+                t_ok[k] = collapse_wide(nodes.data(), nodes.size() / 8, trees[k][1], w) && quantize_wide(w, q[k]);
+                t_need[k] = w.stack_need;
+            }
+        };
+        std::vector<std::thread> pool;
+        const size_t nth = std::min<size_t>(kHostThreads, trees.size());
+        for (size_t k = 1; k < nth; ++k) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        std::vector<uint32_t> root(trees.size());
+        for (size_t k = 0; k < trees.size() && has_wide; ++k) {
+            has_wide = t_ok[k] != 0;
+            root[k] = static_cast<uint32_t>(wblas.size() / kQuantNodeWords);
+            wblas.insert(wblas.end(), q[k].begin(), q[k].end());
+            wide_blas_need = std::max(wide_blas_need, t_need[k]);
+        }
+        for (size_t i = 0; i < n_inst && has_wide; ++i) wroot[i] = root[tree_of[i]];
     }
     if (wblas.empty()) wblas.assign(kQuantNodeWords, 0u);
+    // wide_tri: tri_bvh's rows in the refined order, each with its tri_bvh
+    // slot and its reference leaf's end slot in the third row
+    std::vector<float4> wtri;
+    if (has_wide) {
+        wtri = tri_bvh;
+        for (size_t g = 0; g < n_tidx; ++g) {
+            const size_t p = wperm[g];
+            wtri[3 * g] = tri_bvh[3 * p];
+            wtri[3 * g + 1] = tri_bvh[3 * p + 1];
+            uint32_t bits[2] = {static_cast<uint32_t>(p), leaf_end[p]};
+            float f[2];
+            std::memcpy(f, bits, 8);
+            wtri[3 * g + 2] = make_float4(tri_bvh[3 * p + 2].x, f[0], f[1], 0.0f);
+        }
+    }
     const uint32_t wide_need = wtlas.stack_need + wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
 
@@ -510,7 +552,8 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         return rc;
     if (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) ||
                      (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
-                     (rc = up(B_WROOT, wroot.data(), wroot.size() * 4))))
+                     (rc = up(B_WROOT, wroot.data(), wroot.size() * 4)) ||
+                     (rc = up(B_WTRI, wtri.data(), wtri.size() * sizeof(float4)))))
         return rc;
     {
         auto up_refit = [&](int b, const void *src, size_t bytes) -> int {

@@ -26,7 +26,8 @@ float u2f(uint32_t u) {
 float rcp_clamped(float v) { return 1.0f / (std::fabs(v) < 0x1p-80f ? std::copysign(0x1p-80f, v) : v); }
 
 struct Frame {
-    float sr[3], bl[3], bh[3];
+    float sr[3], bn[3], bf[3];
+    bool neg[3];
 };
 
 // As quant_frame / quant_slab (device/wide.h).
@@ -37,20 +38,21 @@ Frame frame(const uint32_t *q, V3<float> o, V3<float> r) {
         const float s = u2f(((q[3] >> (8 * a)) & 0xffu) << 23);
         f.sr[a] = s * rc[a];
         const float b = (u2f(q[a]) - oc[a]) * rc[a];
-        const float pad = std::copysign(0x1p-20f * (std::fabs(b) + 255.0f * std::fabs(f.sr[a])), rc[a]);
-        f.bl[a] = b - pad;
-        f.bh[a] = b + pad;
+        const float pad = 0x1p-20f * std::fma(255.0f, std::fabs(f.sr[a]), std::fabs(b));
+        f.bn[a] = b - pad;
+        f.bf[a] = b + pad;
+        f.neg[a] = rc[a] < 0.0f;
     }
     return f;
 }
 float slab(const Frame &f, float bound, const uint32_t *q, int k) {
     float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
-        const float ql = static_cast<float>((q[4 + 2 * a] >> (8 * k)) & 0xffu);
-        const float qh = static_cast<float>((q[5 + 2 * a] >> (8 * k)) & 0xffu);
-        const float t0 = std::fma(ql, f.sr[a], f.bl[a]), t1 = std::fma(qh, f.sr[a], f.bh[a]);
-        lo[a] = std::fmin(t0, t1);
-        hi[a] = std::fmax(t0, t1);
+        // entry plane: the low one unless the direction is negative
+        const float qn = static_cast<float>((q[(f.neg[a] ? 5 : 4) + 2 * a] >> (8 * k)) & 0xffu);
+        const float qf = static_cast<float>((q[(f.neg[a] ? 4 : 5) + 2 * a] >> (8 * k)) & 0xffu);
+        lo[a] = std::fma(qn, f.sr[a], f.bn[a]);
+        hi[a] = std::fma(qf, f.sr[a], f.bf[a]);
     }
     const float tmin = std::fmax(std::fmax(lo[0], lo[1]), lo[2]);
     const float tmax = std::fmin(std::fmin(hi[0], hi[1]), hi[2]);
@@ -59,8 +61,49 @@ float slab(const Frame &f, float bound, const uint32_t *q, int k) {
 
 struct Layout {
     std::vector<uint32_t> tlas, blas, root;
+    std::vector<uint32_t> perm;      // wide slot -> triangle slot (the refined leaves' order)
+    std::vector<uint32_t> leaf_end;  // triangle slot -> end slot of its binary leaf
     uint32_t need = 0;
 };
+
+// Per triangle slot: p0, e1, e2 in f32 (the kernels' triangle rows, 12
+// floats a slot), and the refinement's leaf reference counts.
+std::vector<float> tri_rows(const Scene &s, size_t n_slots) {
+    std::vector<float> rows(12 * std::max<size_t>(n_slots, 1), 0.0f);
+    for (size_t i = 0; i < s.n_instances; ++i) {
+        const pulse_mesh_instance &mi = s.instances[i];
+        for (uint32_t k = 0; k < mi.triangle_count; ++k) {
+            const size_t g = mi.index_offset + k;
+            const pulse_primitive &p = s.primitives[mi.triangle_offset + s.triangle_indices[g]];
+            float *r = &rows[12 * g];
+            for (int a = 0; a < 3; ++a) {
+                r[a] = p.p_first[a];
+                r[3 + a] = p.p_second[a] - p.p_first[a];
+                r[6 + a] = p.p_third[a] - p.p_first[a];
+            }
+        }
+    }
+    return rows;
+}
+size_t slot_count(const Scene &s) {
+    size_t n = 0;
+    for (size_t i = 0; i < s.n_instances; ++i)
+        n = std::max<size_t>(n, static_cast<size_t>(s.instances[i].index_offset) + s.instances[i].triangle_count);
+    return n;
+}
+std::vector<uint8_t> leaf_refs(const Scene &s, size_t n_slots, std::vector<uint32_t> *leaf_end = nullptr) {
+    std::vector<uint8_t> refs(n_slots, 0);
+    if (leaf_end) leaf_end->assign(n_slots, 0u);
+    std::vector<std::pair<uint32_t, uint32_t>> seen;
+    for (size_t i = 0; i < s.n_instances; ++i) {
+        const uint32_t off = s.instances[i].node_offset, ioff = s.instances[i].index_offset;
+        if (std::find(seen.begin(), seen.end(), std::make_pair(off, ioff)) != seen.end()) continue;
+        seen.push_back({off, ioff});
+        pulse_host::count_leaf_refs(reinterpret_cast<const uint32_t *>(s.blas_nodes + off), s.n_blas - off, ioff, refs,
+                                    leaf_end);
+    }
+    return refs;
+}
 
 // The TLAS's instance boxes as the library makes them at upload: each
 // instance's BLAS cut at kCutDepth, transformed.
@@ -87,6 +130,11 @@ bool build(const Scene &s, Layout &L) {
     uint32_t blas_need = 0;
     std::vector<std::array<uint32_t, 3>> built;
     L.root.assign(s.n_instances, 0);
+    const size_t n_slots = slot_count(s);
+    const std::vector<float> rows = tri_rows(s, n_slots);
+    const std::vector<uint8_t> refs = leaf_refs(s, n_slots, &L.leaf_end);
+    L.perm.resize(n_slots);
+    for (size_t g = 0; g < n_slots; ++g) L.perm[g] = static_cast<uint32_t>(g);
     for (size_t i = 0; i < s.n_instances; ++i) {
         const uint32_t off = s.instances[i].node_offset, ioff = s.instances[i].index_offset;
         bool found = false;
@@ -98,8 +146,10 @@ bool build(const Scene &s, Layout &L) {
         if (found) continue;
         WideTree w;
         const uint32_t r = static_cast<uint32_t>(L.blas.size() / kQuantNodeWords);
-        if (!pulse_host::collapse_wide(reinterpret_cast<const uint32_t *>(s.blas_nodes + off), s.n_blas - off, ioff, w) ||
-            !pulse_host::quantize_wide(w, L.blas))
+        std::vector<uint32_t> nodes;
+        pulse_host::refine_leaves(reinterpret_cast<const uint32_t *>(s.blas_nodes + off), s.n_blas - off, ioff,
+                                  rows.data(), pulse_host::kRefineLeaf, refs, nodes, L.perm.data());
+        if (!pulse_host::collapse_wide(nodes.data(), nodes.size() / 8, ioff, w) || !pulse_host::quantize_wide(w, L.blas))
             return false;
         blas_need = std::max(blas_need, w.stack_need);
         built.push_back({off, ioff, r});
@@ -172,8 +222,20 @@ void trace(const Scene &s, const Layout &L, Ray<float> &ray, uint64_t &visits, u
             const uint32_t *nodes = L.blas.data() + static_cast<size_t>(kQuantNodeWords) * L.root[inst];
             walk(nodes, ro.origin, r, [&] { return ro.t; },
                  [&](uint32_t f, uint32_t n) {
-                     // leaf entries hold global slots: the mesh's slot is the record
-                     for (uint32_t k = f; k < f + n; ++k) ray_triangle_intersect(s, ro, k - mi.index_offset, inst, c);
+                     // leaf entries hold global wide slots: the mesh's slot is the record;
+                     // of hits at equal t in one binary leaf the lower slot is kept (the
+                     // reference tests a leaf in slot order)
+                     for (uint32_t k = f; k < f + n; ++k) {
+                         const uint32_t local = L.perm[k] - mi.index_offset;
+                         if (local < ro.triangle_index && ro.triangle_index + mi.index_offset < L.leaf_end[L.perm[k]]) {
+                             const float saved = ro.t, above = std::nextafter(saved, INFINITY);
+                             ro.t = above;
+                             ray_triangle_intersect(s, ro, local, inst, c);
+                             if (ro.t == above) ro.t = saved;
+                         } else {
+                             ray_triangle_intersect(s, ro, local, inst, c);
+                         }
+                     }
                  },
                  stk, visits, max_sp);
             if (ro.t == start) continue;  // no hit below the bound: as a miss
@@ -245,10 +307,16 @@ int wide_mirror_trace(const pulse_scene_desc *d, const pulse_ray *rays, pulse_hi
 int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
     const Scene s = view(d);
     out[0] = out[1] = out[2] = 0;
+    const size_t n_slots = slot_count(s);
+    const std::vector<float> rows = tri_rows(s, n_slots);
+    const std::vector<uint8_t> refs = leaf_refs(s, n_slots);
+    std::vector<uint32_t> perm(n_slots);
+    for (size_t g = 0; g < n_slots; ++g) perm[g] = static_cast<uint32_t>(g);
     auto check_tree = [&](const uint32_t *bin, size_t nb, uint32_t off, bool tlas) {
         WideTree t;
-        std::vector<uint32_t> q;
-        if (!pulse_host::collapse_wide(bin, nb, off, t)) return false;
+        std::vector<uint32_t> q, refined;
+        if (!tlas) pulse_host::refine_leaves(bin, nb, off, rows.data(), pulse_host::kRefineLeaf, refs, refined, perm.data());
+        if (!pulse_host::collapse_wide(tlas ? bin : refined.data(), tlas ? nb : refined.size() / 8, off, t)) return false;
         if (tlas && !(nb == 1 && bin[3] == 0 && bin[7] == 0)) {
             const std::vector<float> ib = instance_boxes(s);
             pulse_host::refit_wide_tlas(t, s.instance_indices, ib.data());
@@ -275,7 +343,25 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
                 const uint32_t e = w[24 + k];
                 if (e == kWideEmpty) continue;
                 if (qq[10 + k] != e) ++out[1];
-                if (e >> 31) wide_leaves.push_back({e & ((1u << 26) - 1u), ((e >> 26) & 31u) + 1u});
+                if (e >> 31) {
+                    const uint32_t first = e & ((1u << 26) - 1u), count = ((e >> 26) & 31u) + 1u;
+                    wide_leaves.push_back({first, count});
+                    // a BLAS leaf's f32 box holds its triangles (p0, p0 + e1, p0 + e2 in
+                    // double; the reference's own leaf boxes hold p1 and p2, which lie
+                    // within 2^-24 |e| of those)
+                    for (uint32_t j = first; j < first + count && !tlas; ++j) {
+                        const float *r = &rows[12 * static_cast<size_t>(perm[j])];
+                        for (int a = 0; a < 3; ++a) {
+                            float lo, hi;
+                            std::memcpy(&lo, w + 8 * a + k, 4);
+                            std::memcpy(&hi, w + 8 * a + 4 + k, 4);
+                            const double v0 = r[a], v1 = v0 + static_cast<double>(r[3 + a]), v2 = v0 + static_cast<double>(r[6 + a]);
+                            const double tol = std::ldexp(std::fabs(r[3 + a]) + std::fabs(r[6 + a]), -23);
+                            if (std::min(v0, std::min(v1, v2)) < lo - tol || std::max(v0, std::max(v1, v2)) > hi + tol)
+                                ++out[1];
+                        }
+                    }
+                }
                 for (int a = 0; a < 3; ++a) {
                     float lo, hi;
                     std::memcpy(&lo, w + 8 * a + k, 4);
@@ -295,8 +381,16 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
         }
         if (nb == 1 && bin[7] == 0 && bin[3] == 0) bin_leaves.clear();  // empty tree
         std::vector<uint32_t> covered_b, covered_w;
-        for (auto &l : bin_leaves)
+        for (auto &l : bin_leaves) {
             for (uint32_t j = 0; j < l.second; ++j) covered_b.push_back(l.first + j);
+            // the refinement reorders slots within the leaf only
+            if (!tlas) {
+                std::vector<uint32_t> moved(perm.begin() + l.first, perm.begin() + l.first + l.second);
+                std::sort(moved.begin(), moved.end());
+                for (uint32_t j = 0; j < l.second; ++j)
+                    if (moved[j] != l.first + j) ++out[2];
+            }
+        }
         for (auto &l : wide_leaves) {
             if (l.second > 32) ++out[2];
             for (uint32_t j = 0; j < l.second; ++j) covered_w.push_back(l.first + j);
