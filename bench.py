@@ -8,8 +8,10 @@ per pixel per step, 4 bounces (max_depth 4) with next event estimation
 shade, shadow), accumulate, with the scene already resident in HBM.
 
 value = incoherent Mrays/s: extension rays at depth >= 1 plus shadow rays,
-divided by the summed time of those stages (hipEvents around each stage on
-the library's stream, exact counts from the queue lengths).
+divided by the wall time of those stages (hipEvents on the library's two
+streams: the shadow rays of bounce k trace beside bounce k+1's extension
+rays, and the time counted is from each shade stage's end until both
+launches it feeds are done; exact counts from the queue lengths).
 
 Prints one JSON line.  --gpus N > 1 is not supported on this build.
 """
@@ -387,6 +389,8 @@ def main() -> int:
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
     ap.add_argument("--lds", type=int, default=None, help="persistent: traversal stack entries kept in LDS")
     ap.add_argument("--tps", type=int, default=None, help="persistent: leaf triangles tested per step (1 or 2)")
+    ap.add_argument("--concurrent", type=int, default=None, choices=(0, 1),
+                    help="persistent: shadow rays of bounce k beside bounce k+1's extension rays (default 1)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
@@ -433,6 +437,8 @@ def main() -> int:
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, args.lds)
     if args.tps is not None:
         dev.set_option(N.OPT_TRIS_PER_STEP, args.tps)
+    if args.concurrent is not None:
+        dev.set_option(N.OPT_CONCURRENT_SHADOW, args.concurrent)
     t0 = time.perf_counter()
     dev.upload_scene(scene)
     dev.sync()
@@ -503,13 +509,15 @@ def main() -> int:
         "rays_per_step": {k: agg[k] / args.steps for k in ["primary_rays", "extension_rays", "shadow_rays",
                                                             "incoherent_rays"]},
         "stage_ms_per_step": {k: round(agg[k] / args.steps, 3) for k in ["ms_total", "ms_primary_extend",
-                                                                         "ms_extend_bounce", "ms_shadow", "ms_shade"]},
+                                                                         "ms_incoherent", "ms_extend_bounce",
+                                                                         "ms_shadow", "ms_shade"]},
         "all_rays_mrays_per_s": round(all_rays / wall / 1e6, 2),
         "host_build_s": round(host_build_s, 2),
         "upload_s": round(upload_s, 2),
         "traversal_loop": args.loop,
         "persistent_waves_per_cu": dev.get_option(N.OPT_PERSISTENT_WAVES),
         "refill_lanes": dev.get_option(N.OPT_REFILL_LANES),
+        "concurrent_shadow": dev.get_option(N.OPT_CONCURRENT_SHADOW),
         "gather_lanes": dev.get_option(N.OPT_GATHER_LANES),
         "stack_entries": dev.get_option(N.OPT_STACK_ENTRIES),
         "fast_division": dev.get_option(N.OPT_FAST_DIVISION),

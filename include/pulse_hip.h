@@ -237,10 +237,18 @@ typedef struct pulse_stats {
     uint32_t stack_entries;    /* traversal stack depth used by the kernels */
     double ms_total;           /* generate .. accumulate, all samples      */
     double ms_primary_extend;  /* closest-hit stage at trace index 0       */
-    double ms_incoherent;      /* ms_extend_bounce + ms_shadow             */
-    double ms_shade;           /* generate + shade + accumulate stages     */
-    double ms_extend_bounce;   /* closest-hit stages at trace index >= 1   */
-    double ms_shadow;          /* occlusion stages                         */
+    double ms_incoherent;      /* wall time of the closest-hit stages at   */
+                               /* trace index >= 1 and the occlusion       */
+                               /* stages: their sum, or with concurrent    */
+                               /* shadow launches the spans from each      */
+                               /* shade stage's end until both launches it */
+                               /* feeds are done                           */
+    double ms_shade;           /* ms_total - primary - incoherent:         */
+                               /* generate + shade + accumulate stages     */
+    double ms_extend_bounce;   /* closest-hit launches at trace index >= 1 */
+                               /* (each launch's own duration)             */
+    double ms_shadow;          /* occlusion launches (own durations; they  */
+                               /* overlap the former when concurrent)      */
     uint32_t extend_launches;  /* closest-hit stage launches               */
     uint32_t shadow_launches;  /* occlusion stage launches                 */
     uint64_t work[3][4];       /* PULSE_FLAG_COUNT_WORK: per ray category   */
@@ -393,6 +401,11 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                    /* the instance changes its lanes wait for, however */
                                    /* few (1: every step; 0: the loop's tuned value,  */
                                    /* 1 for loop 1, 2 for loop 2)                     */
+#define PULSE_OPT_CONCURRENT_SHADOW 10 /* persistent: 1 (default) = the shadow rays  */
+                                       /* of bounce k trace on a second stream while */
+                                       /* bounce k+1's extension rays trace (the two */
+                                       /* launches fill each other's tails); 0 = one */
+                                       /* after the other.  Same results either way  */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

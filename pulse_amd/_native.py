@@ -51,6 +51,7 @@ OPT_GATHER_LANES = 6
 OPT_LDS_STACK_ENTRIES = 7
 OPT_TRIS_PER_STEP = 8
 OPT_ADVANCE_PERIOD = 9
+OPT_CONCURRENT_SHADOW = 10
 
 
 class PulseError(RuntimeError):

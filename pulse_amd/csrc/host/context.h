@@ -146,6 +146,14 @@ struct pulse_ctx {
     pulse_stats stats{};
     hipEvent_t ev[3 * pt::kMaxTraces + 8] = {};
     bool events_created = false;
+    // concurrent shadow launches (PULSE_OPT_CONCURRENT_SHADOW): a second
+    // stream, the fork/join events that order it against `stream` (which
+    // stays the one every call completes on), and the timing events
+    int concurrent_shadow = 1;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork_ev[pt::kMaxTraces] = {}, join_ev[pt::kMaxTraces] = {};
+    hipEvent_t ev2[6 * pt::kMaxTraces + 4] = {};
+    bool events2_created = false;
     bool has_scene() const { return cur && cur->valid; }
 };
 

@@ -131,53 +131,82 @@ int persistent_lds(pulse_ctx *ctx) {
     return got;
 }
 template <int LDS, bool SHADOW, int TPS, int LOOP>
-void launch_queue_k(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, int q, int k, uint32_t *spill) {
-    hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS, LOOP>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream,
+void launch_queue_k(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, int q, int k, uint32_t *spill,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_trace_queue<LDS, SHADOW, TPS, LOOP>), dim3(grid), dim3(kTraceBlock), 0, st,
                        dev_scene(ctx), wb, q, k, static_cast<uint32_t>(ctx->eff_refill()),
                        static_cast<uint32_t>(ctx->eff_gather()), static_cast<uint32_t>(ctx->eff_period()),
                        static_cast<int>(ctx->stack_entries), spill);
 }
 template <int LDS, int LOOP>
 void launch_queue_loop(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k,
-                       uint32_t *spill) {
+                       uint32_t *spill, hipStream_t st) {
     if (ctx->tris_per_step == 2) {
-        if (shadow) launch_queue_k<LDS, true, 2, LOOP>(ctx, wb, grid, q, k, spill);
-        else launch_queue_k<LDS, false, 2, LOOP>(ctx, wb, grid, q, k, spill);
+        if (shadow) launch_queue_k<LDS, true, 2, LOOP>(ctx, wb, grid, q, k, spill, st);
+        else launch_queue_k<LDS, false, 2, LOOP>(ctx, wb, grid, q, k, spill, st);
     } else {
-        if (shadow) launch_queue_k<LDS, true, 1, LOOP>(ctx, wb, grid, q, k, spill);
-        else launch_queue_k<LDS, false, 1, LOOP>(ctx, wb, grid, q, k, spill);
+        if (shadow) launch_queue_k<LDS, true, 1, LOOP>(ctx, wb, grid, q, k, spill, st);
+        else launch_queue_k<LDS, false, 1, LOOP>(ctx, wb, grid, q, k, spill, st);
     }
 }
 template <int LDS>
-void launch_queue_lds(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k, uint32_t *spill) {
-    if (ctx->loop_variant == 2) launch_queue_loop<LDS, 2>(ctx, wb, grid, shadow, q, k, spill);
-    else launch_queue_loop<LDS, 1>(ctx, wb, grid, shadow, q, k, spill);
+void launch_queue_lds(pulse_ctx *ctx, const WfBuffers &wb, uint32_t grid, bool shadow, int q, int k, uint32_t *spill,
+                      hipStream_t st) {
+    if (ctx->loop_variant == 2) launch_queue_loop<LDS, 2>(ctx, wb, grid, shadow, q, k, spill, st);
+    else launch_queue_loop<LDS, 1>(ctx, wb, grid, shadow, q, k, spill, st);
 }
 // Spill area of the persistent launches over queues of up to `cap` rays:
-// the stack entries below the LDS part, for every lane of the grid.  Grows
-// only after the stream has drained (no launch in flight uses it).
-int ensure_spill(pulse_ctx *ctx, uint32_t cap) {
+// the stack entries below the LDS part, for every lane of the grid, twice
+// (two launches may run at once).  Grows only after the stream has drained
+// (no launch in flight uses it: the second stream's work is joined into the
+// first before any call returns).
+size_t spill_part_words(pulse_ctx *ctx, uint32_t cap) {
     const int lds = persistent_lds(ctx);
-    if (static_cast<int>(ctx->stack_entries) <= lds) return PULSE_OK;
-    size_t bytes = sizeof(uint32_t) * persistent_grid(ctx, cap) * kTraceBlock * (ctx->stack_entries - lds);
-    if (bytes <= ctx->spill.cap && ctx->spill.ptr) return PULSE_OK;
+    if (static_cast<int>(ctx->stack_entries) <= lds) return 0;
+    return static_cast<size_t>(persistent_grid(ctx, cap)) * kTraceBlock * (ctx->stack_entries - lds);
+}
+int ensure_spill(pulse_ctx *ctx, uint32_t cap) {
+    const size_t bytes = 2 * sizeof(uint32_t) * spill_part_words(ctx, cap);
+    if (bytes == 0 || (bytes <= ctx->spill.cap && ctx->spill.ptr)) return PULSE_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return ctx->spill.ensure(bytes);
 }
-// Persistent extend/shadow (ensure_spill first).
-int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow, int q, int k) {
+// Persistent extend/shadow on stream `st` (default: the context's) with
+// spill part `part` (0 or 1: launches that may run at once use different
+// parts); ensure_spill first.
+int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow, int q, int k,
+                 hipStream_t st = nullptr, int part = 0) {
     const uint32_t grid = persistent_grid(ctx, cap);
     const int lds = persistent_lds(ctx);
-    uint32_t *spill = static_cast<int>(ctx->stack_entries) > lds ? ctx->spill.as<uint32_t>() : nullptr;
+    if (!st) st = ctx->stream;
+    uint32_t *spill = static_cast<int>(ctx->stack_entries) > lds
+                          ? ctx->spill.as<uint32_t>() + static_cast<size_t>(part) * spill_part_words(ctx, cap)
+                          : nullptr;
     switch (lds) {
-        case 8: launch_queue_lds<8>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 16: launch_queue_lds<16>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 24: launch_queue_lds<24>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 32: launch_queue_lds<32>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 40: launch_queue_lds<40>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 48: launch_queue_lds<48>(ctx, wb, grid, shadow, q, k, spill); break;
-        case 64: launch_queue_lds<64>(ctx, wb, grid, shadow, q, k, spill); break;
-        default: launch_queue_lds<128>(ctx, wb, grid, shadow, q, k, spill); break;
+        case 8: launch_queue_lds<8>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 16: launch_queue_lds<16>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 24: launch_queue_lds<24>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 32: launch_queue_lds<32>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 40: launch_queue_lds<40>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 48: launch_queue_lds<48>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        case 64: launch_queue_lds<64>(ctx, wb, grid, shadow, q, k, spill, st); break;
+        default: launch_queue_lds<128>(ctx, wb, grid, shadow, q, k, spill, st); break;
+    }
+    return PULSE_OK;
+}
+// The second stream and its events (PULSE_OPT_CONCURRENT_SHADOW), made on
+// first use; default priority like the context's stream.
+int ensure_stream2(pulse_ctx *ctx, bool timing) {
+    if (!ctx->stream2) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        for (int i = 0; i < pt::kMaxTraces; ++i) {
+            HIP_TRY(hipEventCreateWithFlags(&ctx->fork_ev[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+        }
+    }
+    if (timing && !ctx->events2_created) {
+        for (auto &e : ctx->ev2) HIP_TRY(hipEventCreate(&e));
+        ctx->events2_created = true;
     }
     return PULSE_OK;
 }
@@ -791,6 +820,16 @@ int pulse_destroy(pulse_ctx *ctx) {
     ctx->spill.release();
     if (ctx->events_created)
         for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->events2_created)
+        for (auto &e : ctx->ev2) (void)hipEventDestroy(e);
+    if (ctx->stream2) {
+        (void)hipStreamSynchronize(ctx->stream2);
+        for (int i = 0; i < pt::kMaxTraces; ++i) {
+            (void)hipEventDestroy(ctx->fork_ev[i]);
+            (void)hipEventDestroy(ctx->join_ev[i]);
+        }
+        (void)hipStreamDestroy(ctx->stream2);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PULSE_OK;
@@ -1011,13 +1050,81 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
         if ((rc = ctx->census.ensure(sizeof(uint64_t) * 12)) != PULSE_OK) return rc;
         HIP_TRY(hipMemsetAsync(ctx->census.ptr, 0, sizeof(uint64_t) * 12, ctx->stream));
     }
+    // shadow rays of bounce k on the second stream beside bounce k+1's
+    // extension rays: both queues come from shade(k) and neither launch
+    // touches what the other writes (hits vs path colours); shade(k+1) waits
+    // for both, since it rewrites the shadow queue and the path colours
+    const bool concurrent = persistent && ctx->concurrent_shadow && !census;
+    if (concurrent && (rc = ensure_stream2(ctx, timing)) != PULSE_OK) return rc;
     ctx->stats = pulse_stats{};
-    double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0;
+    double ms_primary = 0, ms_ext_bounce = 0, ms_shadow = 0, ms_total = 0, ms_incoherent = 0;
     uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
     std::vector<uint32_t> counts(2 * (kMaxTraces + 1));
     for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
         fa.accumulation_count = prm->accumulation_count + s;
         HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * kCounterWords, ctx->stream));
+        if (concurrent) {
+            // timing events (ctx->ev2): [0] start; extend k: [1 + 2k], [2 + 2k];
+            // after shade k: [1 + 2T + k]; both of bounce k's launches done:
+            // [1 + 3T + k]; shadow k (second stream): [1 + 4T + k], [1 + 5T + k];
+            // end [1 + 6T]
+            const int T = static_cast<int>(traces);
+            hipEvent_t *E = ctx->ev2;
+            hipStream_t A = ctx->stream, B = ctx->stream2;
+            if (timing) HIP_TRY(hipEventRecord(E[0], A));
+            hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, A, wb, fa);
+            int q = 0;
+            if (timing) HIP_TRY(hipEventRecord(E[1], A));
+            if ((rc = launch_queue(ctx, wb, cap, false, q, 0, A, 0)) != PULSE_OK) return rc;
+            if (timing) HIP_TRY(hipEventRecord(E[2], A));
+            for (int k = 0; k < T; ++k) {
+                hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, A, dev_scene(ctx), wb, fa, q, k);
+                if (timing) HIP_TRY(hipEventRecord(E[1 + 2 * T + k], A));
+                q ^= 1;
+                HIP_TRY(hipEventRecord(ctx->fork_ev[k], A));
+                HIP_TRY(hipStreamWaitEvent(B, ctx->fork_ev[k], 0));
+                if (timing) HIP_TRY(hipEventRecord(E[1 + 4 * T + k], B));
+                if ((rc = launch_queue(ctx, wb, cap, true, 0, k, B, 1)) != PULSE_OK) return rc;
+                if (timing) HIP_TRY(hipEventRecord(E[1 + 5 * T + k], B));
+                HIP_TRY(hipEventRecord(ctx->join_ev[k], B));
+                if (k + 1 < T) {
+                    if (timing) HIP_TRY(hipEventRecord(E[1 + 2 * (k + 1)], A));
+                    if ((rc = launch_queue(ctx, wb, cap, false, q, k + 1, A, 0)) != PULSE_OK) return rc;
+                    if (timing) HIP_TRY(hipEventRecord(E[2 + 2 * (k + 1)], A));
+                }
+                HIP_TRY(hipStreamWaitEvent(A, ctx->join_ev[k], 0));
+                if (timing) HIP_TRY(hipEventRecord(E[1 + 3 * T + k], A));
+            }
+            hipLaunchKernelGGL(k_accumulate, dim3(shade_grid), dim3(kShadeBlock), 0, A, wb,
+                               reinterpret_cast<float4 *>(accum_rgba), cap, fa.accumulation_count);
+            if (timing) HIP_TRY(hipEventRecord(E[1 + 6 * T], A));
+            HIP_TRY(hipGetLastError());
+            if (timing || (prm->flags & PULSE_FLAG_COLLECT_STATS)) {
+                HIP_TRY(hipMemcpyAsync(counts.data(), wb.ext_count, sizeof(uint32_t) * counts.size(),
+                                       hipMemcpyDeviceToHost, A));
+                HIP_TRY(hipStreamSynchronize(A));
+                for (uint32_t k = 0; k < traces; ++k) {
+                    ext_total += counts[k];
+                    sh_total += counts[kMaxTraces + 1 + k];
+                }
+                ext_primary += counts[0];
+            }
+            if (timing) {
+                auto el = [&](int a, int b) {
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, E[a], E[b]);
+                    return static_cast<double>(ms);
+                };
+                ms_total += el(0, 1 + 6 * T);
+                ms_primary += el(1, 2);
+                for (int k = 0; k < T; ++k) {
+                    ms_incoherent += el(1 + 2 * T + k, 1 + 3 * T + k);
+                    ms_shadow += el(1 + 4 * T + k, 1 + 5 * T + k);
+                    if (k > 0) ms_ext_bounce += el(1 + 2 * k, 2 + 2 * k);
+                }
+            }
+            continue;
+        }
         int e = 0;
         if (timing) HIP_TRY(hipEventRecord(ctx->ev[e++], ctx->stream));
         hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, ctx->stream, wb, fa);
@@ -1072,6 +1179,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
                 if (k == 0) ms_primary += t_ext;
                 else ms_ext_bounce += t_ext;
                 ms_shadow += t_sh;
+                ms_incoherent += (k == 0 ? 0.0 : t_ext) + t_sh;
             }
         }
     }
@@ -1087,10 +1195,10 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.stack_entries = ctx->stack_entries;
     ctx->stats.ms_total = ms_total;
     ctx->stats.ms_primary_extend = ms_primary;
-    ctx->stats.ms_incoherent = ms_ext_bounce + ms_shadow;
+    ctx->stats.ms_incoherent = ms_incoherent;
     ctx->stats.ms_extend_bounce = ms_ext_bounce;
     ctx->stats.ms_shadow = ms_shadow;
-    ctx->stats.ms_shade = ms_total - ms_primary - ms_ext_bounce - ms_shadow;
+    ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
     return PULSE_OK;
@@ -1294,6 +1402,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "advance period must be 0..64");
             ctx->advance_period = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_CONCURRENT_SHADOW:
+            if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "concurrent shadow must be 0 or 1");
+            ctx->concurrent_shadow = static_cast<int>(value);
+            return PULSE_OK;
         default:
             PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
     }
@@ -1307,6 +1419,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_REFILL_LANES: *value = ctx->eff_refill(); return PULSE_OK;
         case PULSE_OPT_GATHER_LANES: *value = ctx->eff_gather(); return PULSE_OK;
         case PULSE_OPT_ADVANCE_PERIOD: *value = ctx->eff_period(); return PULSE_OK;
+        case PULSE_OPT_CONCURRENT_SHADOW: *value = ctx->concurrent_shadow; return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
