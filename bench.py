@@ -578,6 +578,13 @@ def main() -> int:
         if pmc is not None:
             result["roofline"]["hbm_bytes_per_launch"] = {k: round(v) for k, v in pmc.items() if k != "scale"}
             result["roofline"]["pmc_scale"] = pmc["scale"]
+        # the bounce extension and shadow launches share the GPU (concurrent
+        # streams): their algorithmic bytes over the wall time they take together
+        inc_bytes = ((agg["extension_rays"] - agg["primary_rays"]) * bpr["bounce"] + agg["shadow_rays"] * bpr["shadow"])
+        inc_rate = inc_bytes / (agg["ms_incoherent"] / 1e3) / 1e9
+        result["roofline"]["incoherent_stages"] = {"achieved": round(inc_rate, 1), "unit": "GB/s",
+                                                   "frac": round(inc_rate / HBM_PEAK_GBS, 4),
+                                                   "ms_per_step": round(agg["ms_incoherent"] / args.steps, 3)}
     print(json.dumps(result))
     dev.close()
     return 0

@@ -38,6 +38,10 @@ struct pulse_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
+    // shadow launches on the context's second stream beside the next
+    // bounce's extension launches (the context's PULSE_OPT_CONCURRENT_SHADOW
+    // when the configuration was applied; persistent traversal only)
+    bool concurrent = false;
 };
 
 namespace {
@@ -149,12 +153,12 @@ int layout_views(pulse_batch *b) {
 
 uint32_t trace_grid(const pulse_batch *b);
 template <int STACK, int LOOP>
-void launch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+void launch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
     const uint32_t grid = trace_grid(b);
     if (shadow)
-        hipLaunchKernelGGL((k_b_trace<STACK, LOOP, true>), dim3(grid), dim3(64), 0, b->ctx->stream, a, q, k);
+        hipLaunchKernelGGL((k_b_trace<STACK, LOOP, true>), dim3(grid), dim3(64), 0, st, a, q, k);
     else
-        hipLaunchKernelGGL((k_b_trace<STACK, LOOP, false>), dim3(grid), dim3(64), 0, b->ctx->stream, a, q, k);
+        hipLaunchKernelGGL((k_b_trace<STACK, LOOP, false>), dim3(grid), dim3(64), 0, st, a, q, k);
 }
 
 // Persistent single-loop traversal with per-lane refill: the compiled LDS
@@ -175,59 +179,63 @@ uint32_t trace_grid(const pulse_batch *b) {
         grid = std::min<uint32_t>(grid, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
     return std::max<uint32_t>(grid, 1);
 }
+// Spilled stack words of one persistent launch (0: the whole stack in LDS
+// or not persistent); concurrent launches take one part each.
+size_t spill_part_words(const pulse_batch *b) {
+    if (!refill_path(b) || static_cast<int>(b->stack_entries) <= refill_lds(b)) return 0;
+    return static_cast<size_t>(trace_grid(b)) * 64u * (b->stack_entries - static_cast<uint32_t>(refill_lds(b)));
+}
 template <int LDS, int LOOP>
-void launch_refill_loop(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+void launch_refill_loop(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
     if (shadow)
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, true, LOOP>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a, q,
-                           k);
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, true, LOOP>), dim3(trace_grid(b)), dim3(64), 0, st, a, q, k);
     else
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, false, LOOP>), dim3(trace_grid(b)), dim3(64), 0, b->ctx->stream, a,
-                           q, k);
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, false, LOOP>), dim3(trace_grid(b)), dim3(64), 0, st, a, q, k);
 }
 template <int LDS>
-void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
-    if (b->cfg.traversal_loop == 2) launch_refill_loop<LDS, 2>(b, a, shadow, q, k);
-    else launch_refill_loop<LDS, 1>(b, a, shadow, q, k);
+void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
+    if (b->cfg.traversal_loop == 2) launch_refill_loop<LDS, 2>(b, a, shadow, q, k, st);
+    else launch_refill_loop<LDS, 1>(b, a, shadow, q, k, st);
 }
 
-void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k) {
+void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
     if (refill_path(b)) {
         switch (refill_lds(b)) {
-            case 8: launch_refill<8>(b, a, shadow, q, k); break;
-            case 16: launch_refill<16>(b, a, shadow, q, k); break;
-            case 24: launch_refill<24>(b, a, shadow, q, k); break;
-            case 32: launch_refill<32>(b, a, shadow, q, k); break;
-            case 40: launch_refill<40>(b, a, shadow, q, k); break;
-            case 48: launch_refill<48>(b, a, shadow, q, k); break;
-            case 64: launch_refill<64>(b, a, shadow, q, k); break;
-            default: launch_refill<128>(b, a, shadow, q, k); break;
+            case 8: launch_refill<8>(b, a, shadow, q, k, st); break;
+            case 16: launch_refill<16>(b, a, shadow, q, k, st); break;
+            case 24: launch_refill<24>(b, a, shadow, q, k, st); break;
+            case 32: launch_refill<32>(b, a, shadow, q, k, st); break;
+            case 40: launch_refill<40>(b, a, shadow, q, k, st); break;
+            case 48: launch_refill<48>(b, a, shadow, q, k, st); break;
+            case 64: launch_refill<64>(b, a, shadow, q, k, st); break;
+            default: launch_refill<128>(b, a, shadow, q, k, st); break;
         }
         return;
     }
     if (b->cfg.traversal_loop == 0) {
         switch (b->stack_entries) {
-            case 32: launch_trace<32, 0>(b, a, shadow, q, k); break;
-            case 48: launch_trace<48, 0>(b, a, shadow, q, k); break;
-            case 64: launch_trace<64, 0>(b, a, shadow, q, k); break;
-            default: launch_trace<128, 0>(b, a, shadow, q, k); break;
+            case 32: launch_trace<32, 0>(b, a, shadow, q, k, st); break;
+            case 48: launch_trace<48, 0>(b, a, shadow, q, k, st); break;
+            case 64: launch_trace<64, 0>(b, a, shadow, q, k, st); break;
+            default: launch_trace<128, 0>(b, a, shadow, q, k, st); break;
         }
     } else if (b->cfg.traversal_loop == 1) {
         switch (b->stack_entries) {
-            case 24: launch_trace<24, 1>(b, a, shadow, q, k); break;
-            case 32: launch_trace<32, 1>(b, a, shadow, q, k); break;
-            case 40: launch_trace<40, 1>(b, a, shadow, q, k); break;
-            case 48: launch_trace<48, 1>(b, a, shadow, q, k); break;
-            case 64: launch_trace<64, 1>(b, a, shadow, q, k); break;
-            default: launch_trace<128, 1>(b, a, shadow, q, k); break;
+            case 24: launch_trace<24, 1>(b, a, shadow, q, k, st); break;
+            case 32: launch_trace<32, 1>(b, a, shadow, q, k, st); break;
+            case 40: launch_trace<40, 1>(b, a, shadow, q, k, st); break;
+            case 48: launch_trace<48, 1>(b, a, shadow, q, k, st); break;
+            case 64: launch_trace<64, 1>(b, a, shadow, q, k, st); break;
+            default: launch_trace<128, 1>(b, a, shadow, q, k, st); break;
         }
     } else {
         switch (b->stack_entries) {
-            case 24: launch_trace<24, 2>(b, a, shadow, q, k); break;
-            case 32: launch_trace<32, 2>(b, a, shadow, q, k); break;
-            case 40: launch_trace<40, 2>(b, a, shadow, q, k); break;
-            case 48: launch_trace<48, 2>(b, a, shadow, q, k); break;
-            case 64: launch_trace<64, 2>(b, a, shadow, q, k); break;
-            default: launch_trace<128, 2>(b, a, shadow, q, k); break;
+            case 24: launch_trace<24, 2>(b, a, shadow, q, k, st); break;
+            case 32: launch_trace<32, 2>(b, a, shadow, q, k, st); break;
+            case 40: launch_trace<40, 2>(b, a, shadow, q, k, st); break;
+            case 48: launch_trace<48, 2>(b, a, shadow, q, k, st); break;
+            case 64: launch_trace<64, 2>(b, a, shadow, q, k, st); break;
+            default: launch_trace<128, 2>(b, a, shadow, q, k, st); break;
         }
     }
 }
@@ -242,15 +250,34 @@ void enqueue_frame(pulse_batch *b) {
     if (b->cfg.persistent_waves_per_cu)
         shade_waves = std::min<uint32_t>(shade_waves, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
     const uint32_t shade_grid = std::max<uint32_t>((shade_waves + 3) / 4, 1);
+    // concurrent: the shadow launches take the second spill part and the
+    // second stream, forked after each shade and joined before the next
+    BatchArgs a2 = a;
+    if (b->concurrent && a2.spill) a2.spill += spill_part_words(b);
     for (uint32_t s = 0; s < std::max(b->prm.spp, 1u); ++s) {
         hipLaunchKernelGGL(k_b_clear, dim3((clear_n + 255) / 256), dim3(256), 0, st, a, n_work);
         hipLaunchKernelGGL(k_b_generate, dim3((b->total_tiles + 3) / 4), dim3(256), 0, st, a, s);
         int q = 0;
-        for (uint32_t k = 0; k < b->traces; ++k) {
-            dispatch_trace(b, a, false, q, static_cast<int>(k));
-            hipLaunchKernelGGL(k_b_shade, dim3(shade_grid), dim3(256), 0, st, a, q, static_cast<int>(k));
-            dispatch_trace(b, a, true, q, static_cast<int>(k));
-            q ^= 1;
+        if (b->concurrent) {
+            hipStream_t st2 = b->ctx->stream2;
+            dispatch_trace(b, a, false, q, 0, st);
+            for (uint32_t k = 0; k < b->traces; ++k) {
+                hipLaunchKernelGGL(k_b_shade, dim3(shade_grid), dim3(256), 0, st, a, q, static_cast<int>(k));
+                (void)hipEventRecord(b->ctx->fork_ev[k], st);
+                (void)hipStreamWaitEvent(st2, b->ctx->fork_ev[k], 0);
+                dispatch_trace(b, a2, true, q, static_cast<int>(k), st2);
+                (void)hipEventRecord(b->ctx->join_ev[k], st2);
+                q ^= 1;
+                if (k + 1 < b->traces) dispatch_trace(b, a, false, q, static_cast<int>(k + 1), st);
+                (void)hipStreamWaitEvent(st, b->ctx->join_ev[k], 0);
+            }
+        } else {
+            for (uint32_t k = 0; k < b->traces; ++k) {
+                dispatch_trace(b, a, false, q, static_cast<int>(k), st);
+                hipLaunchKernelGGL(k_b_shade, dim3(shade_grid), dim3(256), 0, st, a, q, static_cast<int>(k));
+                dispatch_trace(b, a, true, q, static_cast<int>(k), st);
+                q ^= 1;
+            }
         }
         hipLaunchKernelGGL(k_b_accumulate, dim3(b->total_slots / 256), dim3(256), 0, st, a, s);
     }
@@ -282,10 +309,10 @@ int apply_config(pulse_batch *b, const pulse_batch_config &c) {
     b->stack_entries = resolve_stack(b, c);
     drop_graph(b);
     if ((rc = layout_views(b)) != PULSE_OK) return rc;  // (drains the stream)
-    if (refill_path(b) && static_cast<int>(b->stack_entries) > refill_lds(b)) {
-        size_t bytes = sizeof(uint32_t) * static_cast<size_t>(trace_grid(b)) * 64u *
-                       (b->stack_entries - static_cast<uint32_t>(refill_lds(b)));
-        if ((rc = b->spill.ensure(bytes)) != PULSE_OK) return rc;
+    b->concurrent = refill_path(b) && b->ctx->concurrent_shadow != 0;
+    if (b->concurrent && (rc = ensure_stream2(b->ctx, false)) != PULSE_OK) return rc;
+    if (const size_t words = spill_part_words(b)) {
+        if ((rc = b->spill.ensure(sizeof(uint32_t) * words * (b->concurrent ? 2 : 1))) != PULSE_OK) return rc;
     }
     return PULSE_OK;
 }

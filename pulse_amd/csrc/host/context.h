@@ -161,3 +161,10 @@ struct pulse_scene {
     pulse_ctx *ctx = nullptr;
     pulse_host::SceneData data;
 };
+
+namespace pulse_host {
+// The context's second stream and its fork/join events
+// (PULSE_OPT_CONCURRENT_SHADOW), made on first use at default priority;
+// with `timing` also the concurrent loop's timing events.
+int ensure_stream2(pulse_ctx *ctx, bool timing);
+}  // namespace pulse_host

@@ -194,22 +194,6 @@ int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow,
     }
     return PULSE_OK;
 }
-// The second stream and its events (PULSE_OPT_CONCURRENT_SHADOW), made on
-// first use; default priority like the context's stream.
-int ensure_stream2(pulse_ctx *ctx, bool timing) {
-    if (!ctx->stream2) {
-        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-        for (int i = 0; i < pt::kMaxTraces; ++i) {
-            HIP_TRY(hipEventCreateWithFlags(&ctx->fork_ev[i], hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
-        }
-    }
-    if (timing && !ctx->events2_created) {
-        for (auto &e : ctx->ev2) HIP_TRY(hipEventCreate(&e));
-        ctx->events2_created = true;
-    }
-    return PULSE_OK;
-}
 // Work census (PULSE_FLAG_COUNT_WORK) of queue q / the shadow queue at k
 // into category `cat` of ctx->census.
 template <int STACK, int LOOP>
@@ -302,6 +286,23 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
 }  // namespace
 
 namespace pulse_host {
+
+// The second stream and its events (PULSE_OPT_CONCURRENT_SHADOW), made on
+// first use; default priority like the context's stream.
+int ensure_stream2(pulse_ctx *ctx, bool timing) {
+    if (!ctx->stream2) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        for (int i = 0; i < pt::kMaxTraces; ++i) {
+            HIP_TRY(hipEventCreateWithFlags(&ctx->fork_ev[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+        }
+    }
+    if (timing && !ctx->events2_created) {
+        for (auto &e : ctx->ev2) HIP_TRY(hipEventCreate(&e));
+        ctx->events2_created = true;
+    }
+    return PULSE_OK;
+}
 
 constexpr int kHostThreads = 8;  // host threads for the per-instance TLAS boxes and the wide BLASes
 
