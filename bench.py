@@ -242,7 +242,7 @@ def pmc_traffic(max_depth: int, loop: int) -> dict | None:
     by their order within the frame: extend (trace index 0 = primary, then
     bounces) and shadow launches alternate with shading."""
     frame_args = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-adjoint", "--no-batch", "--no-pmc", "--no-loop1",
-                  "--max-depth", str(max_depth), "--loop", str(loop)]
+                  "--no-subdiv5", "--max-depth", str(max_depth), "--loop", str(loop)]
     scale = {}
     per_kind = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -292,6 +292,39 @@ def bench_loop1(dev, node, cam, args, frame: int, frames: int = 4) -> dict:
     finally:
         dev.set_option(N.OPT_TRAVERSAL_LOOP, args.loop)
     return {"value": round(rays / (ms / 1e3) / 1e6, 2), "unit": "Mrays/s", "frames": frames}
+
+
+def bench_subdiv5(args, frames: int = 10, warmup: int = 2) -> dict:
+    """The same workload with subdivision-5 meshes (20,480 triangles each:
+    1.31M unique / 16.4M instanced, about 1M unique as the workload's
+    description reads it), timed the same way in its own context."""
+    from pulse_amd import _native as N
+    from pulse_amd import procedural as P
+    from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
+
+    scene, camf = P.icosphere_field(subdiv=5, threads=min(16, os.cpu_count() or 1))
+    cam = camf(args.width, args.height)
+    dev = Device(0)
+    try:
+        dev.upload_scene(scene)
+        node = PulsePathTracerNode(dev, args.width, args.height)
+        flags = N.FLAG_STAGE_TIMING | N.FLAG_COLLECT_STATS
+        rays = ms = ms_total = 0.0
+        for f in range(warmup + frames):
+            node.run(cam, make_params(accumulation_count=f, max_depth=args.max_depth, mode=N.MODE_GGX_NEE, flags=flags))
+            st = dev.stats()
+            if st.error_flags:
+                raise RuntimeError(f"device error flags {st.error_flags}")
+            if f >= warmup:
+                rays += st.incoherent_rays
+                ms += st.ms_incoherent
+                ms_total += st.ms_total
+        dev.sync()
+    finally:
+        dev.close()
+    return {"value": round(rays / (ms / 1e3) / 1e6, 2), "unit": "Mrays/s", "frames": frames,
+            "ms_per_frame": round(ms_total / frames, 3), "unique_triangles": scene.info["unique_triangles"],
+            "instanced_triangles": scene.info["instanced_triangles"]}
 
 
 def bench_batch(frames: int = 2, check_views: int = 4) -> dict:
@@ -395,6 +428,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     ap.add_argument("--no-loop1", action="store_true", help="skip the reference-order (loop 1) comparison frames")
+    ap.add_argument("--no-subdiv5", action="store_true", help="skip the subdivision-5 variant of the workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -528,6 +562,10 @@ def main() -> int:
         result["reference_order_loop1"] = loop1
     if census is not None:
         result["work_census"] = census
+    if not args.no_subdiv5 and args.subdiv == 6:
+        result["subdiv5"] = bench_subdiv5(args)
+        print(f"bench: subdivision-5 meshes {result['subdiv5']['value']} incoherent Mrays/s", file=sys.stderr,
+              flush=True)
 
     if not args.no_adjoint:
         result["fwd_adjoint"] = bench_adjoint(scene, camf)

@@ -169,3 +169,31 @@ def test_batch_rejects_bad_views(dev, views):
     with pytest.raises(N.PulseError, match="stack_entries"):
         b.set_config(stack_entries=8)
     b.close()
+
+
+@pytest.mark.parametrize("loop", [1, 2])
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_concurrent_shadow_launches_keep_images(dev, views, loop, use_graph):
+    """PULSE_OPT_CONCURRENT_SHADOW (default 1): the shadow launches of bounce
+    k on the context's second stream beside bounce k+1's extension launch.
+    pulse_trace and the batch (graph replay or direct launches) give the
+    same images as with every launch in sequence, over two progressive
+    frames."""
+    prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
+    assert dev.get_option(N.OPT_CONCURRENT_SHADOW) == 1
+    with pytest.raises(N.PulseError):
+        dev.set_option(N.OPT_CONCURRENT_SHADOW, 2)
+    dev.set_option(N.OPT_CONCURRENT_SHADOW, 0)
+    try:
+        ref = _sequential(dev, views, prm, [0, 2], loop=loop)
+    finally:
+        dev.set_option(N.OPT_CONCURRENT_SHADOW, 1)
+    conc = _sequential(dev, views, prm, [0, 2], loop=loop)
+    b = Batch(dev, views, prm)
+    b.set_config(use_graph=use_graph, traversal_loop=loop)
+    b.render(0)
+    b.render(2)
+    for i, (img, c, r) in enumerate(zip(b.images(), conc, ref)):
+        assert same_bits(c, r).all(), f"pulse_trace view {i}"
+        assert same_bits(img, r).all(), f"batch view {i}"
+    b.close()
