@@ -270,6 +270,13 @@ int pulse_destroy(pulse_ctx *ctx);
 /* The context's stream as a hipStream_t. */
 void *pulse_stream(pulse_ctx *ctx);
 int pulse_sync(pulse_ctx *ctx);
+/* Self-test of the device arithmetic the triangle tests rely on (no
+ * counterpart in the reference; for maintainers bringing the library up on
+ * a new GPU or toolchain): counts the f32 a with |a| in [2^-100, 2^126),
+ * every mantissa and both signs (3.8e9 values), for which the hardware
+ * reciprocal refined by one fused Newton step differs from the correctly
+ * rounded 1.0f / a.  The kernels assume 0.  Synchronous, about a second. */
+int pulse_selftest_reciprocal(pulse_ctx *ctx, uint64_t *mismatches);
 
 /* Device memory helpers for callers without a HIP binding (ctypes, FFI). */
 int pulse_device_alloc(pulse_ctx *ctx, size_t bytes, void **out_device_ptr);

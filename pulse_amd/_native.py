@@ -239,6 +239,7 @@ SIGNATURES = [
     ("pulse_destroy", _I, [_P]),
     ("pulse_stream", _P, [_P]),
     ("pulse_sync", _I, [_P]),
+    ("pulse_selftest_reciprocal", _I, [_P, C.POINTER(C.c_uint64)]),
     ("pulse_device_alloc", _I, [_P, C.c_size_t, C.POINTER(_P)]),
     ("pulse_device_free", _I, [_P, _P]),
     ("pulse_copy_h2d", _I, [_P, _P, _P, C.c_size_t]),

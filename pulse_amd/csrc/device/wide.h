@@ -128,7 +128,11 @@ __device__ __forceinline__ void mt_select(V3 o, V3 d, const float4 *tri, uint32_
     const V3 e2 = mk(B.z, B.w, C.x);
     const V3 hv = cross(d, e2);
     const float a = dot(e1, hv);
-    const float f = 1.0f / a;
+    // rcp_newton is 1.0f / a wherever it can decide a hit: the 4-wide BVHs
+    // exist only for scenes whose edge components are finite and at most
+    // 2^60 (so |a| < 2^124 for the unit direction), and |a| < 1e-4, inf and
+    // NaN give no hit with either value
+    const float f = rcp_newton(a);
     const V3 s = sub(o, p0);
     const float u = f * dot(s, hv);
     const V3 q = cross(s, e1);

@@ -399,6 +399,9 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     std::vector<float4> tri_bvh(3 * std::max<size_t>(n_tidx, 1), make_float4(0, 0, 0, 0));
     std::vector<float4> nrm_bvh(3 * std::max<size_t>(n_tidx, 1), make_float4(0, 0, 0, 0));
     std::vector<int64_t> slot_owner(n_tidx, -1);  // triangle_offset that filled a slot
+    // every edge component finite and at most 2^60: the 4-wide triangle test's
+    // reciprocal (device/wide.h) is exact for such triangles
+    bool edges_small = true;
     uint32_t blas_depth = 0;
     std::vector<uint32_t> seen_roots;
     for (size_t i = 0; i < n_inst; ++i) {
@@ -430,6 +433,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
             // e1 = p1 - p0, e2 = p2 - p0 in f32 (utilities.wgsl:432-433)
             float e1x = p.p_second[0] - p.p_first[0], e1y = p.p_second[1] - p.p_first[1], e1z = p.p_second[2] - p.p_first[2];
             float e2x = p.p_third[0] - p.p_first[0], e2y = p.p_third[1] - p.p_first[1], e2z = p.p_third[2] - p.p_first[2];
+            for (float e : {e1x, e1y, e1z, e2x, e2y, e2z}) edges_small = edges_small && std::fabs(e) <= 0x1p60f;
             tri_bvh[3 * slot] = make_float4(p.p_first[0], p.p_first[1], p.p_first[2], e1x);
             tri_bvh[3 * slot + 1] = make_float4(e1y, e1z, e2x, e2y);
             tri_bvh[3 * slot + 2] = make_float4(e2z, 0.f, 0.f, 0.f);
@@ -494,7 +498,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     }
     WideTree wtlas;
     std::vector<uint32_t> qtlas, wblas;
-    bool has_wide = collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas);
+    bool has_wide = edges_small && collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas);
     if (has_wide && !empty_tlas) refit_wide_tlas(wtlas, iidx, ibox.data());
     has_wide = has_wide && quantize_wide(wtlas, qtlas);
     if (cut_boxes.empty()) cut_boxes.assign(6, 0.0f);
@@ -846,6 +850,43 @@ int pulse_sync(pulse_ctx *ctx) {
     HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
     if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
     if (err & kErrIterationCap) PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached the 100000-iteration cap");
+    return PULSE_OK;
+}
+
+namespace {
+// Every f32 a with biased exponent in [27, 253) (|a| in [2^-100, 2^126)),
+// every mantissa, both signs: counts those where rcp_newton(a) and the
+// correctly rounded 1.0f / a differ.
+__global__ void k_selftest_rcp(unsigned long long *bad) {
+    constexpr uint64_t kExps = 253u - 27u;
+    constexpr uint64_t kN = 2ull * kExps << 23;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    unsigned long long local = 0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < kN; i += stride) {
+        const uint32_t m = static_cast<uint32_t>(i & ((1u << 23) - 1u));
+        const uint32_t e = 27u + static_cast<uint32_t>((i >> 23) % kExps);
+        const uint32_t s = static_cast<uint32_t>((i >> 23) / kExps);
+        const float a = __uint_as_float((s << 31) | (e << 23) | m);
+        if (__float_as_uint(rcp_newton(a)) != __float_as_uint(1.0f / a)) ++local;
+    }
+    if (local) atomicAdd(bad, local);
+}
+}  // namespace
+
+int pulse_selftest_reciprocal(pulse_ctx *ctx, uint64_t *mismatches) {
+    if (!ctx || !mismatches) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_selftest_reciprocal: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // the counter: the context's 8-byte loss accumulator (cleared before each use)
+    if ((rc = ctx->loss_acc.ensure(sizeof(unsigned long long))) != PULSE_OK) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->loss_acc.ptr, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(k_selftest_rcp, dim3(static_cast<uint32_t>(ctx->num_cus) * 16u), dim3(256), 0, ctx->stream,
+                       ctx->loss_acc.as<unsigned long long>());
+    HIP_TRY(hipGetLastError());
+    unsigned long long n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, ctx->loss_acc.ptr, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *mismatches = n;
     return PULSE_OK;
 }
 

@@ -110,6 +110,13 @@ class Device:
     def sync(self) -> None:
         N.check(N.lib().pulse_sync(self._ctx))
 
+    def selftest_reciprocal(self) -> int:
+        """Values of |a| in [2^-100, 2^126) where the kernels' fast reciprocal
+        differs from 1.0f / a (expected 0)."""
+        n = C.c_uint64(0)
+        N.check(N.lib().pulse_selftest_reciprocal(self._ctx, C.byref(n)))
+        return n.value
+
     # --- scene and tracing ---
     def upload_scene(self, scene: SceneBuffers) -> None:
         d = scene.desc()
