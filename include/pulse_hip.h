@@ -199,15 +199,18 @@ typedef struct pulse_params {
                                       /* the moving sampled directions instead of */
                                       /* at fixed directions (the default)        */
 #define PULSE_FLAG_LIGHT_POWER 16u /* NEE picks the light by emission strength   */
+                                   /* (light_emission_strength_cdf, pdf = its    */
+                                   /* step) as the reference intended            */
+                                   /* (utilities.wgsl:501, 541-546), instead of   */
+                                   /* uniformly (its live code)                  */
 #define PULSE_FLAG_COUNT_WORK 32u  /* traversal loop 2: before each trace launch,  */
                                    /* walk its queue once more and count node      */
                                    /* visits, triangle tests and instances entered */
                                    /* into pulse_stats.work (extra work: not for   */
                                    /* timed frames)                                */
-                                   /* (light_emission_strength_cdf, pdf = its    */
-                                   /* step) as the reference intended            */
-                                   /* (utilities.wgsl:501, 541-546), instead of   */
-                                   /* uniformly (its live code)                  */
+#define PULSE_FLAG_ADJOINT_ACCUMULATE 64u /* adjoint: add to `grad` instead of     */
+                                          /* overwriting it (the gradient of a sum */
+                                          /* of per-view losses, one call a view)  */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */
@@ -501,9 +504,11 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
 
 /* dL/dparameters of the image rendered by pulse_trace with the same camera
  * and params (accumulation_count 0, params->spp samples, mode
- * PULSE_MODE_GGX_NEE, max_depth <= 4, no Russian roulette:
+ * PULSE_MODE_GGX_NEE, max_depth <= 8 -- <= 4 with
+ * PULSE_FLAG_ADJOINT_REPARAM --, no Russian roulette:
  * rr_start_depth >= max_depth).  Replays every path from its seed; `grad`
- * (device, n_materials*7 floats) is overwritten.  `replay_rgba` (device,
+ * (device, n_materials*7 floats) is overwritten, or added to with
+ * PULSE_FLAG_ADJOINT_ACCUMULATE.  `replay_rgba` (device,
  * optional) receives the replayed image, which equals the forward image bit
  * for bit.  Requires metallic == 0 for every material. */
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params, const float *image,

@@ -37,6 +37,7 @@ FLAG_HORIZON_CLAMP = 4
 FLAG_ADJOINT_REPARAM = 8
 FLAG_LIGHT_POWER = 16
 FLAG_COUNT_WORK = 32
+FLAG_ADJOINT_ACCUMULATE = 64
 PRESENT_RGBA32F = 0
 PRESENT_RGBA16F = 1
 PRESENT_RGBA8_SRGB = 2

@@ -4,8 +4,8 @@
 // Nothing of the forward paths is stored: each path is replayed from its
 // seed (the pixel index and accumulation count, path_tracer.wgsl:60-61) with
 // the same f32 traversal, so it takes exactly the forward decisions.  Along
-// the replay the vertex computations run in Dual<kAdjMaxVertices>
-// arithmetic: the tangent k is the derivative with respect to the roughness
+// the replay the vertex computations run in Dual<MAXV> arithmetic (MAXV the
+// kernel's vertex capacity): the tangent k is the derivative with respect to the roughness
 // used at vertex k, which moves the sampled half vector and therefore every
 // later ray, hit point, normal, NEE term and BSDF weight.  Albedo and emission
 // enter linearly; they are handled by a reverse sweep over the stored vertex
@@ -31,9 +31,9 @@
 
 namespace pt {
 
-constexpr int kAdjMaxVertices = 4;
-constexpr int kGradPerMaterial = 7;  // base.rgb, roughness, emissive.rgb
-using DA = Dual<kAdjMaxVertices>;
+constexpr int kAdjMaxVertices = 4;      // path vertices of the default kernels
+constexpr int kAdjMaxVerticesDeep = 8;  // the deep kernels (fixed-direction roughness only)
+constexpr int kGradPerMaterial = 7;     // base.rgb, roughness, emissive.rgb
 
 struct AdjArgs {
     pulse_camera cam;
@@ -99,9 +99,10 @@ __device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
 // false, tangents through the moving directions) does not capture.
 // LOOP: the traversal of the forward pass being replayed (1: the binary
 // BVHs, 2: the 4-wide ones), so every replayed path is the forward's.
-template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1>
+// MAXV: path vertices the per-vertex arrays hold (max_depth at most MAXV).
+template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1, int MAXV = kAdjMaxVertices>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
-    using S = typename std::conditional<DETACHED, float, DA>::type;
+    using S = typename std::conditional<DETACHED, float, Dual<MAXV>>::type;
     __shared__ uint32_t smem[STACK * BLOCK];
     extern __shared__ float sgrad[];
     const uint32_t nent = a.n_materials * kGradPerMaterial;
@@ -130,11 +131,11 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             V3T<S> T = lift<S>(mk(1.0f, 1.0f, 1.0f));
             V3T<S> C = lift<S>(mk(0.0f, 0.0f, 0.0f));
             // stored vertex values for the reverse sweep
-            V3 beta[kAdjMaxVertices], cvert[kAdjMaxVertices], wvert[kAdjMaxVertices];
-            V3 dDdb[kAdjMaxVertices], dDdLe[kAdjMaxVertices], dDdr[kAdjMaxVertices];
-            float dwdb[kAdjMaxVertices], dlogw[kAdjMaxVertices];
-            uint32_t mat[kAdjMaxVertices], lmat[kAdjMaxVertices];
-            bool vis[kAdjMaxVertices];
+            V3 beta[MAXV], cvert[MAXV], wvert[MAXV];
+            V3 dDdb[MAXV], dDdLe[MAXV], dDdr[MAXV];
+            float dwdb[MAXV], dlogw[MAXV];
+            uint32_t mat[MAXV], lmat[MAXV];
+            bool vis[MAXV];
             int nv = 0;
             for (uint32_t depth = 0; depth < a.max_depth; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 if constexpr (DETACHED) {
                     m.roughness = pm.perceptual_roughness;
                 } else {
-                    m.roughness = dvar<kAdjMaxVertices>(pm.perceptual_roughness, static_cast<int>(depth));
+                    m.roughness = dvar<MAXV>(pm.perceptual_roughness, static_cast<int>(depth));
                 }
                 m.reflectance = pm.reflectance;
                 m.metallic = pm.metallic;

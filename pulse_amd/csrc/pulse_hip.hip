@@ -769,6 +769,12 @@ constexpr int kAdjBlock = 128;
 template <int STACK, int LOOP>
 void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
     uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
+    if (a.max_depth > static_cast<uint32_t>(kAdjMaxVertices)) {
+        // deep paths: fixed-direction roughness derivative, the 64-entry stack
+        hipLaunchKernelGGL((k_adjoint<kAdjBlock, 64, true, LOOP, kAdjMaxVerticesDeep>), dim3(grid), dim3(kAdjBlock),
+                           lds, ctx->stream, dev_scene(ctx), a);
+        return;
+    }
     if (a.detached)
         hipLaunchKernelGGL((k_adjoint<kAdjBlock, STACK, true, LOOP>), dim3(grid), dim3(kAdjBlock), lds, ctx->stream,
                            dev_scene(ctx), a);
@@ -1353,8 +1359,11 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
     if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
     if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs PULSE_MODE_GGX_NEE");
-    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVertices);
+    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVerticesDeep);
+    if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d",
+                   kAdjMaxVertices);
     if (prm->rr_start_depth < prm->max_depth)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: Russian roulette must be off (rr_start_depth >= max_depth)");
     if (prm->accumulation_count != 0 || prm->spp == 0)
@@ -1367,7 +1376,7 @@ int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *p
     if (rc) return rc;
     const uint32_t npix = cam->width * cam->height;
     const uint32_t nent = ctx->cur->n_materials * kGradPerMaterial;
-    if (nent) {
+    if (nent && !(prm->flags & PULSE_FLAG_ADJOINT_ACCUMULATE)) {
         hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
     }
     AdjArgs a;

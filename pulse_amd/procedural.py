@@ -230,14 +230,19 @@ def cornell_lights(strengths=(2.0, 40.0, 8.0), seed: int = 0, threads: int = 8):
 
 def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800, n_emissive: int = 32,
                     box: float = 100.0, radius=(1.5, 4.0), roughness=(0.05, 1.0), emission: float = 8.0,
-                    seed: int = 2, threads: int = 8, n_materials: int | None = None, keep_builder: bool = False):
+                    seed: int = 2, threads: int = 8, n_materials: int | None = None, keep_builder: bool = False,
+                    per_instance_materials: bool = False):
     """Config-2 style scene; returns (SceneBuffers, camera factory).
 
     Mesh k gets material k % n_materials (default: one material per mesh);
     instance i uses mesh i % n_meshes; `n_emissive` instances (chosen by the
-    seeded hash) use one extra emissive material.
+    seeded hash) use one extra emissive material.  per_instance_materials:
+    instance i gets material i instead (n_instances materials: per-instance
+    albedo / roughness parameters for the adjoint).
     """
     rng = CounterRng(seed)
+    if per_instance_materials:
+        n_materials = n_instances
     n_materials = n_meshes if n_materials is None else n_materials
     sb = SceneBuilder()
     radii = rng.uniform(2, n_meshes, *radius)
@@ -257,7 +262,8 @@ def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800,
         m[:3, :3] = random_rotation(*rot[i])
         m[:3, 3] = tr[i]
         mesh = meshes[i % n_meshes]
-        sb.add_instance(mesh, m_emit if i in emissive else mats[(i % n_meshes) % n_materials], m)
+        mat = mats[i] if per_instance_materials else mats[(i % n_meshes) % n_materials]
+        sb.add_instance(mesh, m_emit if i in emissive else mat, m)
     scene = sb.build(threads, keep_builder=keep_builder)
 
     def camera(width=1920, height=1080):

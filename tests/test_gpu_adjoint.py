@@ -87,6 +87,14 @@ def _replay_case(dev, scene, cam, prm, frozen):
     assert (err <= 2e-3 * np.abs(gref) + 1e-4 * scale).all(), (g, gref)
 
 
+@pytest.mark.parametrize("depth", [6, 8])
+def test_deep_paths_replay_and_oracle_gradient(dev, depth):
+    """More than 4 bounces (the deep kernels, fixed-direction roughness
+    derivative): replay == forward and the gradient == the oracle's."""
+    scene, cam, prm = _cornell_case(depth=depth)
+    _replay_case(dev, scene, cam, prm, True)
+
+
 def test_emission_euler_identity_gpu(dev):
     scene, cam, prm = _cornell_case(spp=2, depth=3, size=40)
     dev.upload_scene(scene)

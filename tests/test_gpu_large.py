@@ -1,0 +1,149 @@
+"""GPU: the large workload's shape.
+
+Forward: 256 displaced subdivision-6 icosphere BLAS (21M unique
+triangles) under 1024 instances with 256 emissive ones, 3840x2160, 6
+bounces with NEE, progressive accumulation over frames.  With the
+reference's traversal (loop 1) the accumulated rows equal the oracle's
+progressive render of the same rows bit for bit; with the 4-wide BVHs
+(loop 2) they differ only in pixels whose path met a t-tie.
+
+Adjoint: one material per instance (1024 instances, 1025 materials x 7
+parameters: the gradient buffer exceeds the LDS and goes to device
+atomics), 6-bounce paths, 8 views.  Checked by identities that need no
+reference render: with target 0, Euler's identity for the emission
+(the image is linear in all emissions together, so sum_j Le_j dL/dLe_j =
+2 L) summed over the views, with each view's gradient added by
+PULSE_FLAG_ADJOINT_ACCUMULATE; and the accumulated gradient for nonzero
+targets equals the sum of the per-view gradients.  The deep-path adjoint
+against the oracle's central differences is in test_gpu_adjoint.py (the
+Cornell box at 6 and 8 bounces, where the f32 and f64 renders agree along
+the same decisions; in these dense fields some pixels do not, so f64
+differences are no reference there).
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleScene, same_bits
+from pulse_amd import _native as N
+from pulse_amd import procedural as P
+from pulse_amd.path_tracer import Device, PulseAdjoint, PulsePathTracerNode, make_params, material_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def large():
+    return P.icosphere_field(n_meshes=256, subdiv=6, n_instances=1024, n_emissive=256, box=110.0, seed=5,
+                             threads=16)
+
+
+def test_large_progressive_rows(dev, large):
+    scene, camf = large
+    assert scene.info["unique_triangles"] >= 16_000_000
+    cam = camf(3840, 2160)
+    dev.upload_scene(scene)
+    rows = [0, 517, 1080, 1081, 1702, 2159]
+    frames = 3
+    images = {}
+    for loop in (1, 2):
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+        node = PulsePathTracerNode(dev, cam.width, cam.height)
+        for f in range(frames):
+            node.run(cam, make_params(accumulation_count=f, max_depth=6, mode=N.MODE_GGX_NEE))
+        assert dev.stats().error_flags == 0
+        images[loop] = node.readback()
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+    orc = OracleScene(scene)
+    acc = np.zeros((cam.height, cam.width, 4), np.float32)
+    for r in rows:
+        for f in range(frames):
+            orc.render(cam, make_params(accumulation_count=f, max_depth=6, mode=N.MODE_GGX_NEE), accum=acc,
+                       threads=16, rows=(r, r + 1))
+    o = acc[rows]
+    assert np.isfinite(o[..., :3]).all() and o[..., :3].mean() > 0.0
+    assert same_bits(images[1][rows], o).all()
+    px = ~same_bits(images[2][rows], o).all(axis=-1)
+    assert px.sum() <= max(1, px.size // 1000), f"{px.sum()} of {px.size} pixels differ"
+
+
+@pytest.fixture(scope="module")
+def per_instance():
+    return P.icosphere_field(n_meshes=16, subdiv=3, n_instances=1024, n_emissive=64, box=60.0, seed=9,
+                             radius=(1.0, 3.0), roughness=(0.3, 1.0), per_instance_materials=True)
+
+
+def _views(camf):
+    """8 cameras on a circle around the field, looking at its centre."""
+    cams = []
+    for k in range(8):
+        a = 2.0 * np.pi * k / 8
+        eye = (70.0 * np.cos(a), 12.0 * np.sin(3 * a), 70.0 * np.sin(a))
+        cams.append(P.make_camera(80, 64, eye=eye, target=(0.0, 0.0, 0.0)))
+    return cams
+
+
+def test_multiview_adjoint_per_instance_parameters(dev, per_instance):
+    scene, camf = per_instance
+    nm = len(scene.buffers["materials"])
+    assert nm == 1024 + 1  # one per instance and the shared emitter's
+    dev.upload_scene(scene)
+    prm = make_params(spp=2, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6, flags=N.FLAG_HORIZON_CLAMP)
+    cams = _views(camf)
+    adj = PulseAdjoint(dev, cams[0].width, cams[0].height, nm)
+    mp = material_params(scene.buffers["materials"])
+
+    def pass_over_views(targets):
+        """Gradient accumulated over the views, the per-view gradients and
+        the per-view losses."""
+        per_view, losses, acc = [], [], None
+        for v, cam in enumerate(cams):
+            img = adj.forward(cam, prm)
+            tgt = targets(img)
+            adj.set_target(tgt)
+            losses.append(adj.loss())
+            per_view.append(adj.backward(cam, prm).astype(np.float64))
+            if v > 0:
+                adj.grad.copy_from(acc.astype(np.float32).reshape(-1))
+                p = N.Params.from_buffer_copy(prm)
+                p.flags |= N.FLAG_ADJOINT_ACCUMULATE  # add this view's gradient to the others'
+                acc = adj.backward(cam, p)
+            else:
+                acc = per_view[0].astype(np.float32)
+        return acc.astype(np.float64), per_view, losses
+
+    # Euler's identity for the emission, summed over the views
+    acc, per_view, losses = pass_over_views(lambda img: np.zeros_like(img))
+    euler = float(np.sum(mp[:, 4:7] * acc[:, 4:7]))
+    assert sum(losses) > 0.0
+    assert abs(euler - 2.0 * sum(losses)) <= 1e-4 * 2.0 * sum(losses), (euler, 2.0 * sum(losses))
+    # accumulation: the sum of the per-view gradients
+    acc, per_view, _ = pass_over_views(lambda img: (img * np.float32(0.7) + np.float32(0.02)).astype(np.float32))
+    total = np.sum(per_view, axis=0)
+    scale = np.abs(total).max()
+    assert np.all(np.abs(acc - total) <= 1e-5 * scale + 1e-4 * np.abs(total))
+    # every instance is a parameter: many of them receive gradient
+    assert np.count_nonzero(total[:1024, 0]) > 200
+
+
+def test_adjoint_depth_limits(dev, per_instance):
+    scene, camf = per_instance
+    nm = len(scene.buffers["materials"])
+    dev.upload_scene(scene)
+    cam = _views(camf)[0]
+    adj = PulseAdjoint(dev, cam.width, cam.height, nm)
+    ok = make_params(spp=1, max_depth=8, mode=N.MODE_GGX_NEE, rr_start_depth=8, flags=N.FLAG_HORIZON_CLAMP)
+    adj.forward(cam, ok)
+    adj.set_target(np.zeros((cam.height, cam.width, 4), np.float32))
+    adj.backward(cam, ok)
+    for bad in (make_params(spp=1, max_depth=9, mode=N.MODE_GGX_NEE, rr_start_depth=9),
+                make_params(spp=1, max_depth=5, mode=N.MODE_GGX_NEE, rr_start_depth=5,
+                            flags=N.FLAG_ADJOINT_REPARAM)):
+        with pytest.raises(N.PulseError, match="UNSUPPORTED"):
+            adj.backward(cam, bad)
