@@ -116,41 +116,49 @@ __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse
         }
 }
 
-// One thread: the boxes of every child of every wide TLAS node, deepest
-// first (a child's node index is above its parent's), into
-// child_box[24 node + 6 k ..] and node_box[6 node ..].
-__global__ void k_refit_boxes(const uint32_t *qnodes, uint32_t n_nodes, const uint32_t *iidx, const float *inst_box,
-                              float *child_box, float *node_box) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (uint32_t i = n_nodes; i-- > 0;) {
-        float nb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
-                       -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t e = qnodes[16u * i + 10u + k];
-            float cb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
+// One block: the boxes of every child of every wide TLAS node into
+// child_box[24 node + 6 k ..] and node_box[6 node ..], one level of the tree
+// at a time, deepest first (`levels`: {n_levels, level starts, node
+// indices}, host/wide_bvh.h: wide_levels), a thread per node of a level; a
+// node's children lie in earlier levels, done before the barrier.
+__global__ void __launch_bounds__(256) k_refit_boxes(const uint32_t *qnodes, const uint32_t *levels,
+                                                     const uint32_t *iidx, const float *inst_box, float *child_box,
+                                                     float *node_box) {
+    const uint32_t n_levels = levels[0];
+    const uint32_t *order = levels + 2u + n_levels;
+    for (uint32_t l = 0; l < n_levels; ++l) {
+        for (uint32_t j = levels[1u + l] + threadIdx.x; j < levels[2u + l]; j += blockDim.x) {
+            const uint32_t i = order[j];
+            float nb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
                            -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
-            if (e == 0xffffffffu) {
-                // empty position
-            } else if (e >> 31) {
-                const uint32_t first = e & ((1u << 26) - 1u), count = ((e >> 26) & 31u) + 1u;
-                for (uint32_t j = first; j < first + count; ++j) {
-                    const float *b = inst_box + 6u * iidx[j];
-                    for (int a = 0; a < 3; ++a) {
-                        cb[a] = fminf(cb[a], b[a]);
-                        cb[3 + a] = fmaxf(cb[3 + a], b[3 + a]);
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = qnodes[16u * i + 10u + k];
+                float cb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
+                               -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+                if (e == 0xffffffffu) {
+                    // empty position
+                } else if (e >> 31) {
+                    const uint32_t first = e & ((1u << 26) - 1u), count = ((e >> 26) & 31u) + 1u;
+                    for (uint32_t m = first; m < first + count; ++m) {
+                        const float *b = inst_box + 6u * iidx[m];
+                        for (int a = 0; a < 3; ++a) {
+                            cb[a] = fminf(cb[a], b[a]);
+                            cb[3 + a] = fmaxf(cb[3 + a], b[3 + a]);
+                        }
                     }
+                } else {
+                    for (int a = 0; a < 6; ++a) cb[a] = node_box[6u * e + a];
                 }
-            } else {
-                for (int a = 0; a < 6; ++a) cb[a] = node_box[6u * e + a];
+                for (int a = 0; a < 6; ++a) child_box[24u * i + 6u * k + a] = cb[a];
+                if (e != 0xffffffffu)
+                    for (int a = 0; a < 3; ++a) {
+                        nb[a] = fminf(nb[a], cb[a]);
+                        nb[3 + a] = fmaxf(nb[3 + a], cb[3 + a]);
+                    }
             }
-            for (int a = 0; a < 6; ++a) child_box[24u * i + 6u * k + a] = cb[a];
-            if (e != 0xffffffffu)
-                for (int a = 0; a < 3; ++a) {
-                    nb[a] = fminf(nb[a], cb[a]);
-                    nb[3 + a] = fmaxf(nb[3 + a], cb[3 + a]);
-                }
+            for (int a = 0; a < 6; ++a) node_box[6u * i + a] = nb[a];
         }
-        for (int a = 0; a < 6; ++a) node_box[6u * i + a] = nb[a];
+        __syncthreads();
     }
 }
 

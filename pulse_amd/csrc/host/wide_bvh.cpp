@@ -224,6 +224,32 @@ void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxe
     for (auto &th : pool) th.join();
 }
 
+void wide_levels(const WideTree &t, std::vector<uint32_t> &out) {
+    const size_t n = t.words.size() / kWideNodeWords;
+    std::vector<uint32_t> depth(n, 0u), frontier{0u};
+    uint32_t deepest = 0;
+    while (!frontier.empty() && n) {
+        std::vector<uint32_t> next;
+        for (uint32_t i : frontier)
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = t.words[static_cast<size_t>(i) * kWideNodeWords + 24 + k];
+                if (e == kWideEmpty || (e & kWideLeafBit) || e >= n) continue;
+                depth[e] = depth[i] + 1u;
+                deepest = std::max(deepest, depth[e]);
+                next.push_back(e);
+            }
+        frontier.swap(next);
+    }
+    const uint32_t levels = n ? deepest + 1u : 0u;
+    out.assign(2u + levels + n, 0u);
+    out[0] = levels;
+    std::vector<uint32_t> count(levels, 0u);
+    for (size_t i = 0; i < n; ++i) count[deepest - depth[i]] += 1u;  // level 0: the deepest
+    for (uint32_t l = 0; l < levels; ++l) out[2 + l] = out[1 + l] + count[l];
+    std::vector<uint32_t> fill(out.begin() + 1, out.begin() + 1 + levels);
+    for (size_t i = 0; i < n; ++i) out[2 + levels + fill[deepest - depth[i]]++] = static_cast<uint32_t>(i);
+}
+
 void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box) {
     const size_t n = t.words.size() / kWideNodeWords;
     std::vector<float> nb(6 * n);

@@ -84,6 +84,11 @@ void instance_world_box(const float *m, const float *boxes, size_t n, float out[
 // cut range, on up to `threads` threads.
 void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxes, const uint32_t *cut_range,
                           size_t n, float *out, int threads);
+// The nodes of a 4-wide tree grouped by depth, deepest first, for a
+// level-by-level refit: out = {n_levels, level_start[0 .. n_levels],
+// node indices}; level l holds out[1 + n_levels + 1 + level_start[l] ..
+// level_start[l + 1]).  Every child lies in an earlier level than its parent.
+void wide_levels(const WideTree &t, std::vector<uint32_t> &out);
 // Recomputes the f32 child boxes of a 4-wide TLAS (`t`, collapsed from the
 // binary TLAS) from per-instance world boxes inst_box[6 * instance] over
 // instance_indices `iidx`, deepest nodes first.

@@ -596,8 +596,11 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
             HIP_TRY(hipMemcpy(sd.refit[b].ptr, src, bytes, hipMemcpyHostToDevice));
             return PULSE_OK;
         };
+        std::vector<uint32_t> levels;
+        wide_levels(wtlas, levels);
         if ((rc = up_refit(5, cut_boxes.data(), cut_boxes.size() * 4)) ||
-            (rc = up_refit(6, cut_range.data(), cut_range.size() * 4)))
+            (rc = up_refit(6, cut_range.data(), cut_range.size() * 4)) ||
+            (rc = up_refit(7, levels.data(), levels.size() * 4)))
             return rc;
     }
     sd.cut_boxes = std::move(cut_boxes);
@@ -697,9 +700,17 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
     int rc;
+    std::vector<uint32_t> levels;
+    if (has_wide) wide_levels(wtlas, levels);
+    auto up_levels = [&]() -> int {
+        int r = sd.refit[7].ensure(levels.size() * 4);
+        if (r) return r;
+        HIP_TRY(hipMemcpy(sd.refit[7].ptr, levels.data(), levels.size() * 4, hipMemcpyHostToDevice));
+        return PULSE_OK;
+    };
     if ((rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) ||
         (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes)) ||
-        (has_wide && (rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)))) {
+        (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) || (rc = up_levels())))) {
         sd.valid = false;  // partially replaced: unusable until the next upload
         return rc;
     }
@@ -743,9 +754,9 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
     hipLaunchKernelGGL(k_refit_instances, dim3(n), dim3(64), 0, stream, sd.refit[0].as<float>(),
                        sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.refit[5].as<float>(),
                        sd.refit[6].as<uint32_t>(), sd.refit[1].as<float>(), n, err);
-    hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(1), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(), nodes,
-                       sd.buf[B_INST_IDX].as<uint32_t>(), sd.refit[1].as<float>(), sd.refit[2].as<float>(),
-                       sd.refit[3].as<float>());
+    hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(256), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(),
+                       sd.refit[7].as<uint32_t>(), sd.buf[B_INST_IDX].as<uint32_t>(), sd.refit[1].as<float>(),
+                       sd.refit[2].as<float>(), sd.refit[3].as<float>());
     hipLaunchKernelGGL(k_refit_quantize, dim3((nodes + 63) / 64), dim3(64), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(),
                        nodes, sd.refit[2].as<float>(), err);
     HIP_TRY(hipGetLastError());

@@ -335,6 +335,29 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
         }
         if (!pulse_host::quantize_wide(t, q)) return false;
         const size_t nodes = t.words.size() / pulse_host::kWideNodeWords;
+        {
+            // the refit's level order: every node once, every child in an earlier level
+            std::vector<uint32_t> lv;
+            pulse_host::wide_levels(t, lv);
+            const uint32_t n_levels = lv[0];
+            std::vector<int> level_of(nodes, -1);
+            for (uint32_t l = 0; l < n_levels; ++l)
+                for (uint32_t j = lv[1 + l]; j < lv[2 + l]; ++j) {
+                    const uint32_t i = lv[2 + n_levels + j];
+                    if (i >= nodes || level_of[i] != -1) ++out[2];
+                    else level_of[i] = static_cast<int>(l);
+                }
+            for (size_t i = 0; i < nodes; ++i) {
+                if (level_of[i] < 0) {
+                    ++out[2];
+                    continue;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t e = t.words[i * pulse_host::kWideNodeWords + 24 + k];
+                    if (e != kWideEmpty && !(e >> 31) && (e >= nodes || level_of[e] >= level_of[i])) ++out[2];
+                }
+            }
+        }
         std::vector<std::pair<uint32_t, uint32_t>> wide_leaves;
         for (size_t i = 0; i < nodes; ++i) {
             const uint32_t *w = t.words.data() + i * pulse_host::kWideNodeWords;
