@@ -1308,9 +1308,34 @@ int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width,
         PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, npix, k);
         return PULSE_OK;
     };
+    // as in pulse_trace: each shadow launch on the second stream beside the
+    // next extension launch (generate feeds the first pair like a shade)
+    const bool concurrent = persistent && ctx->concurrent_shadow;
+    if (concurrent && (rc = ensure_stream2(ctx, false)) != PULSE_OK) return rc;
     for (uint32_t s = 0; s < spp; ++s) {
         HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * kCounterWords, ctx->stream));
         hipLaunchKernelGGL(k_gi_generate, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, g, s);
+        if (concurrent) {
+            hipStream_t A = ctx->stream, B = ctx->stream2;
+            const int T = static_cast<int>(fa.max_traces);
+            int q = 0;
+            for (int k = -1; k < T; ++k) {
+                if (k >= 0) {
+                    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, A, dev_scene(ctx), wb, fa, q, k);
+                    q ^= 1;
+                }
+                HIP_TRY(hipEventRecord(ctx->fork_ev[k + 1], A));
+                HIP_TRY(hipStreamWaitEvent(B, ctx->fork_ev[k + 1], 0));
+                // k = -1: NEE at the G-buffer points
+                if ((rc = launch_queue(ctx, wb, npix, true, 0, k < 0 ? kMaxTraces : k, B, 1)) != PULSE_OK) return rc;
+                HIP_TRY(hipEventRecord(ctx->join_ev[k + 1], B));
+                if (k + 1 < T && (rc = launch_queue(ctx, wb, npix, false, q, k + 1, A, 0)) != PULSE_OK) return rc;
+                HIP_TRY(hipStreamWaitEvent(A, ctx->join_ev[k + 1], 0));
+            }
+            hipLaunchKernelGGL(k_gi_accumulate, dim3(grid), dim3(kShadeBlock), 0, A, wb,
+                               reinterpret_cast<float4 *>(gi_rgba), npix, s, spp_inv);
+            continue;
+        }
         if ((rc = shadow(kMaxTraces)) != PULSE_OK) return rc;  // NEE at the G-buffer points
         int q = 0;
         for (uint32_t k = 0; k < fa.max_traces; ++k) {

@@ -100,12 +100,15 @@ def test_gi_gpu_bit_exact(which, spp, depth, flags):
         gb = N.GBuffer(d["position"].ptr.value, d["normal"].ptr.value, d["base_color"].ptr.value,
                        d["emissive"].ptr.value)
         out = dev.alloc(72 * 40 * 16)
-        # traversal loop 1 (the reference's order) bit for bit; loop 2 (4-wide
-        # BVHs) the same up to pixels whose path met a t-tie
-        for loop, persistent in ((1, 24), (1, 0), (2, 24)):
+        # traversal loop 1 (the reference's order) bit for bit, with the
+        # shadow launches beside the extension launches (the default) or in
+        # sequence; loop 2 (4-wide BVHs) the same up to pixels whose path met
+        # a t-tie
+        for loop, persistent, conc in ((1, 24, 1), (1, 24, 0), (1, 0, 1), (2, 24, 1)):
             dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
             dev.upload_scene(scene)
             dev.set_option(N.OPT_PERSISTENT_WAVES, persistent)
+            dev.set_option(N.OPT_CONCURRENT_SHADOW, conc)
             N.check(N.lib().pulse_trace_gbuffer(dev.handle, gb, 72, 40, prm, out.ptr))
             got = out.copy_to(np.zeros((40, 72, 4), np.float32)) if dev.sync() is None else None
             if loop == 1:
