@@ -50,4 +50,36 @@ __device__ __forceinline__ uint32_t wave_append(bool valid, uint32_t *counter) {
     return base + before;
 }
 
+// Appends the `valid` items of a block of BLOCK threads (all of which call
+// it) to two queues at once with one atomic per queue per block: returns
+// this thread's slots in r0 / r1.  `lds`: 2 * (BLOCK / 64 + 1) words.
+template <int BLOCK>
+__device__ __forceinline__ void block_append2(bool v0, uint32_t *c0, bool v1, uint32_t *c1, uint32_t *lds, uint32_t &r0,
+                                              uint32_t &r1) {
+    constexpr uint32_t kWaves = BLOCK / 64;
+    const uint64_t m0 = __ballot(v0), m1 = __ballot(v1);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (lane == 0u) {
+        lds[wave] = static_cast<uint32_t>(__popcll(m0));
+        lds[kWaves + 1u + wave] = static_cast<uint32_t>(__popcll(m1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        uint32_t s0 = 0, s1 = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t a = lds[w], b = lds[kWaves + 1u + w];
+            lds[w] = s0;
+            lds[kWaves + 1u + w] = s1;
+            s0 += a;
+            s1 += b;
+        }
+        lds[kWaves] = s0 ? atomicAdd(c0, s0) : 0u;
+        lds[2u * kWaves + 1u] = s1 ? atomicAdd(c1, s1) : 0u;
+    }
+    __syncthreads();
+    r0 = lds[kWaves] + lds[wave] + static_cast<uint32_t>(__popcll(m0 & below));
+    r1 = lds[2u * kWaves + 1u] + lds[kWaves + 1u + wave] + static_cast<uint32_t>(__popcll(m1 & below));
+}
+
 }  // namespace pt

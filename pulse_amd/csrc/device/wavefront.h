@@ -54,10 +54,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_generate(WfBuffers wb, FrameArg
     const uint32_t tiles_x = (W + 7u) / 8u;
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t tile = s >> 6, within = s & 63u;
-    uint32_t x = (tile % tiles_x) * 8u + (within & 7u);
-    uint32_t y = (tile / tiles_x) * 8u + (within >> 3);
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    uint32_t x = tx * 8u + (within & 7u);
+    uint32_t y = ty * 8u + (within >> 3);
     bool valid = x < W && y < H;
-    uint32_t slot = wave_append(valid, &wb.ext_count[0]);
+    // queue slot: tile after tile, row-major inside a tile (only the last
+    // tile row and column are partial: the rows above hold 8 W pixels, the
+    // tiles to the left th * 8 each), so no counter is contended
+    const uint32_t tw = min(8u, W - tx * 8u), th = min(8u, H - min(H, ty * 8u));
+    const uint32_t slot = 8u * ty * W + th * 8u * tx + (within >> 3) * tw + (within & 7u);
+    if (s == 0u) wb.ext_count[0] = W * H;
     if (!valid) return;
     uint32_t pixel = x + y * W;
     V3 o, d;
@@ -89,6 +95,7 @@ __global__ void __launch_bounds__(BLOCK) k_extend(DevScene sc, WfBuffers wb, int
 // k).  Next-bounce rays go to queue q^1 / counter k+1, shadow rays to the
 // shadow queue / counter k.
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, WfBuffers wb, FrameArgs fa, int q, int k) {
+    __shared__ uint32_t append_lds[2 * (kShadeBlock / 64 + 1)];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < wb.ext_count[k];
     ShadeOut so;
@@ -106,8 +113,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, WfBuffers wb
             wb.state_c[pixel] = make_float4(C.x, C.y, C.z, 0.0f);
         }
     }
-    uint32_t rs = wave_append(so.emit_ray, &wb.ext_count[k + 1]);
-    uint32_t ss = wave_append(so.emit_shadow, &wb.sh_count[k]);
+    uint32_t rs, ss;
+    block_append2<kShadeBlock>(so.emit_ray, &wb.ext_count[k + 1], so.emit_shadow, &wb.sh_count[k], append_lds, rs, ss);
     if (so.emit_ray) {
         wb.ray_o[q ^ 1][rs] = make_float4(so.next_o.x, so.next_o.y, so.next_o.z, __uint_as_float(pixel));
         wb.ray_d[q ^ 1][rs] = make_float4(so.next_d.x, so.next_d.y, so.next_d.z, 0.0f);
