@@ -166,10 +166,15 @@ __global__ void __launch_bounds__(256) k_b_generate(BatchArgs b, uint32_t s) {
     const uint32_t W = bv.cam.width, H = bv.cam.height;
     const uint32_t tw = b.tile_w, th = 64u / b.tile_w;
     const uint32_t lt = tile - bv.tile_base;
-    const uint32_t x = (lt % bv.tiles_x) * tw + within % tw;
-    const uint32_t y = (lt / bv.tiles_x) * th + within / tw;
+    const uint32_t tx = lt % bv.tiles_x, ty = lt / bv.tiles_x;
+    const uint32_t x = tx * tw + within % tw;
+    const uint32_t y = ty * th + within / tw;
     const bool valid = x < W && y < H;
-    const uint32_t slot = wave_append(valid, &b.counts[v * b.cstride + 0u]);
+    // queue slot in closed form (tiles in order, row-major inside a tile;
+    // only the view's last tile row and column are partial): no counter
+    const uint32_t tw_v = min(tw, W - min(W, tx * tw)), th_v = min(th, H - min(H, ty * th));
+    const uint32_t slot = th * ty * W + th_v * tw * tx + (within / tw) * tw_v + within % tw;
+    if (lt == 0u && within == 0u) b.counts[v * b.cstride + 0u] = W * H;
     if (!valid) return;
     const uint32_t pixel = x + y * W;
     V3 o, d;
