@@ -271,9 +271,7 @@ struct WideTraversal {
                 if (want_advance) return true;
             } else {
                 entry = e0;
-                push(sc, stk, d3, e3);
-                push(sc, stk, d2, e2);
-                push(sc, stk, d1, e1);
+                push3(sc, stk, d1, e1, d2, e2, d3, e3);
             }
         }
         if (!(in_blas && tri >= kRangeOne)) {
@@ -305,6 +303,25 @@ struct WideTraversal {
             return false;
         }
         return pop(sc, stk);
+    }
+
+    // Pushes the hit ones of the sorted children 1-3, farthest first.  They
+    // are a prefix of that order (a miss sorts last), so with three free
+    // LDS entries the pushes are three unconditional writes (the unused ones
+    // above the new top are never read) and one add; else one push each.
+    __device__ __forceinline__ void push3(const DevScene &sc, LdsStack &stk, float d1, uint32_t e1, float d2,
+                                          uint32_t e2, float d3, uint32_t e3) {
+        if (sp + 3 <= stk.lds_cap && sp + 3 <= stk.capacity) {
+            const int n = (d1 != kTFar ? 1 : 0) + (d2 != kTFar ? 1 : 0) + (d3 != kTFar ? 1 : 0);
+            stk.base[sp * stk.stride] = n == 3 ? e3 : (n == 2 ? e2 : e1);
+            stk.base[(sp + 1) * stk.stride] = n == 3 ? e2 : e1;
+            stk.base[(sp + 2) * stk.stride] = e1;
+            sp += n;
+        } else {
+            push(sc, stk, d3, e3);
+            push(sc, stk, d2, e2);
+            push(sc, stk, d1, e1);
+        }
     }
 
     __device__ __forceinline__ void push(const DevScene &sc, LdsStack &stk, float d, uint32_t e) {
