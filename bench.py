@@ -297,7 +297,8 @@ def bench_loop1(dev, node, cam, args, frame: int, frames: int = 4) -> dict:
 def bench_subdiv5(args, frames: int = 10, warmup: int = 2) -> dict:
     """The same workload with subdivision-5 meshes (20,480 triangles each:
     1.31M unique / 16.4M instanced, about 1M unique as the workload's
-    description reads it), timed the same way in its own context."""
+    description reads it), timed the same way in its own context (same
+    traversal loop and launch options)."""
     from pulse_amd import _native as N
     from pulse_amd import procedural as P
     from pulse_amd.path_tracer import Device, PulsePathTracerNode, make_params
@@ -306,6 +307,9 @@ def bench_subdiv5(args, frames: int = 10, warmup: int = 2) -> dict:
     cam = camf(args.width, args.height)
     dev = Device(0)
     try:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, args.loop)
+        if args.concurrent is not None:
+            dev.set_option(N.OPT_CONCURRENT_SHADOW, args.concurrent)
         dev.upload_scene(scene)
         node = PulsePathTracerNode(dev, args.width, args.height)
         flags = N.FLAG_STAGE_TIMING | N.FLAG_COLLECT_STATS
