@@ -307,9 +307,36 @@ __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k
     }
 }
 
+// One path vertex of the batch: shades extension ray `i` (in slot base + i)
+// if valid; the path state is updated in place.
+__device__ __forceinline__ ShadeOut b_shade_one(const BatchArgs &b, const DevScene &sc, const FrameArgs &fa, int q,
+                                                int k, uint32_t base, uint32_t i, bool valid, uint32_t &pixel) {
+    ShadeOut so;
+    pixel = kDeadRay;
+    if (valid) {
+        const uint32_t slot = base + i;
+        float4 oa = b.ray_o[q][slot], da = b.ray_d[q][slot];
+        pixel = __float_as_uint(oa.w);
+        float4 ha = b.hit[slot];
+        if (pixel != kDeadRay && ha.x < kTFar) {
+            float4 st = b.state_t[base + pixel], sc4 = b.state_c[base + pixel];
+            V3 T = mk(st.x, st.y, st.z), C = mk(sc4.x, sc4.y, sc4.z);
+            uint32_t rng = __float_as_uint(st.w);
+            so = shade_ray(sc, fa, k, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), ha, b.hit_v[slot], T, C, rng);
+            b.state_t[base + pixel] = make_float4(T.x, T.y, T.z, __uint_as_float(rng));
+            b.state_c[base + pixel] = make_float4(C.x, C.y, C.z, 0.0f);
+        }
+    }
+    return so;
+}
+
 // shade: one bounce for extension queue q at trace index k, 4 waves per
-// workgroup, each wave on its own chunks.
+// workgroup.  Persistent: the workgroup takes one chunk at a time (all its
+// waves on one view) and appends the chunk's next and shadow rays with one
+// atomic per 256 rays; otherwise each wave on its own chunk.
 __global__ void __launch_bounds__(256) k_b_shade(BatchArgs b, int q, int k) {
+    __shared__ uint32_t s_chunk;
+    __shared__ uint32_t append_lds[2 * (256 / 64 + 1)];
     const uint32_t ci = static_cast<uint32_t>(k);
     const ChunkMap cm = chunk_map(b, ci);
     const uint32_t lane = __lane_id();
@@ -325,6 +352,49 @@ __global__ void __launch_bounds__(256) k_b_shade(BatchArgs b, int q, int k) {
     fa.light_power = b.light_power;
     fa.gi = 0;
     fa.accumulation_count = 0;
+    if (b.persistent) {
+        for (;;) {
+            if (threadIdx.x == 0u) s_chunk = atomicAdd(&b.work[wslot], 1u);
+            __syncthreads();
+            const uint32_t g = s_chunk;
+            __syncthreads();  // every thread has it before the next claim
+            if (g >= cm.total) break;
+            uint32_t start;
+            const uint32_t v = chunk_view(cm, b.n_views, g, start);
+            const DevScene sc = b.views[v].sc;
+            const uint32_t base = b.views[v].pix_base;
+            uint32_t *row = b.counts + v * b.cstride;
+            const uint32_t cnt = row[ci];
+            const uint32_t first = (g - start) * b.chunk;
+            if (!compact && first == 0u && threadIdx.x == 0u) row[ci + 1u] = cnt;  // in place: same length
+            for (uint32_t r = 0; r < b.chunk && first + r < cnt; r += 256u) {
+                const uint32_t i = first + r + threadIdx.x;
+                const bool valid = r + threadIdx.x < b.chunk && i < cnt;
+                uint32_t pixel;
+                const ShadeOut so = b_shade_one(b, sc, fa, q, k, base, i, valid, pixel);
+                uint32_t rs, ss;
+                block_append2<256>(compact && so.emit_ray, &row[ci + 1u], so.emit_shadow, &row[b.max_traces + 1u + ci],
+                                   append_lds, rs, ss);
+                if (compact) {
+                    if (so.emit_ray) {
+                        b.ray_o[q ^ 1][base + rs] =
+                            make_float4(so.next_o.x, so.next_o.y, so.next_o.z, __uint_as_float(pixel));
+                        b.ray_d[q ^ 1][base + rs] = make_float4(so.next_d.x, so.next_d.y, so.next_d.z, 0.0f);
+                    }
+                } else if (valid) {
+                    const uint32_t tag = so.emit_ray ? pixel : kDeadRay;
+                    b.ray_o[q ^ 1][base + i] = make_float4(so.next_o.x, so.next_o.y, so.next_o.z, __uint_as_float(tag));
+                    b.ray_d[q ^ 1][base + i] = make_float4(so.next_d.x, so.next_d.y, so.next_d.z, 0.0f);
+                }
+                if (so.emit_shadow) {
+                    b.sh_o[base + ss] = make_float4(so.sh_o.x, so.sh_o.y, so.sh_o.z, __uint_as_float(pixel));
+                    b.sh_d[base + ss] = make_float4(so.sh_d.x, so.sh_d.y, so.sh_d.z, so.sh_thr);
+                    b.sh_c[base + ss] = make_float4(so.sh_c.x, so.sh_c.y, so.sh_c.z, 0.0f);
+                }
+            }
+        }
+        return;
+    }
     for (uint32_t g = next_chunk(b, wslot, wave_global, true); g < cm.total;
          g = next_chunk(b, wslot, wave_global, false)) {
         uint32_t start;
@@ -338,22 +408,8 @@ __global__ void __launch_bounds__(256) k_b_shade(BatchArgs b, int q, int k) {
         for (uint32_t r = 0; r < b.chunk && first + r < cnt; r += 64u) {
             const uint32_t i = first + r + lane;
             const bool valid = i < cnt;
-            ShadeOut so;
-            uint32_t pixel = kDeadRay;
-            if (valid) {
-                const uint32_t slot = base + i;
-                float4 oa = b.ray_o[q][slot], da = b.ray_d[q][slot];
-                pixel = __float_as_uint(oa.w);
-                float4 ha = b.hit[slot];
-                if (pixel != kDeadRay && ha.x < kTFar) {
-                    float4 st = b.state_t[base + pixel], sc4 = b.state_c[base + pixel];
-                    V3 T = mk(st.x, st.y, st.z), C = mk(sc4.x, sc4.y, sc4.z);
-                    uint32_t rng = __float_as_uint(st.w);
-                    so = shade_ray(sc, fa, k, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), ha, b.hit_v[slot], T, C, rng);
-                    b.state_t[base + pixel] = make_float4(T.x, T.y, T.z, __uint_as_float(rng));
-                    b.state_c[base + pixel] = make_float4(C.x, C.y, C.z, 0.0f);
-                }
-            }
+            uint32_t pixel;
+            const ShadeOut so = b_shade_one(b, sc, fa, q, k, base, i, valid, pixel);
             if (compact) {
                 const uint32_t rs = wave_append(so.emit_ray, &row[ci + 1u]);
                 if (so.emit_ray) {
