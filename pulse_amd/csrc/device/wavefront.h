@@ -272,6 +272,7 @@ struct GBufferArgs {
 };
 __global__ void __launch_bounds__(kShadeBlock) k_gi_generate(DevScene sc, WfBuffers wb, FrameArgs fa, GBufferArgs g,
                                                              uint32_t s) {
+    __shared__ uint32_t append_lds[2 * (kShadeBlock / 64 + 1)];
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = pixel < g.npix;
     ShadeOut so;
@@ -307,8 +308,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_gi_generate(DevScene sc, WfBuff
         so.next_o = add(p, scale(0.001f, n));
         so.next_d = scatter;
     }
-    uint32_t rs = wave_append(so.emit_ray, &wb.ext_count[0]);
-    uint32_t ss = wave_append(so.emit_shadow, &wb.sh_count[kMaxTraces]);
+    uint32_t rs, ss;
+    block_append2<kShadeBlock>(so.emit_ray, &wb.ext_count[0], so.emit_shadow, &wb.sh_count[kMaxTraces], append_lds, rs,
+                               ss);
     if (so.emit_ray) {
         wb.ray_o[0][rs] = make_float4(so.next_o.x, so.next_o.y, so.next_o.z, __uint_as_float(pixel));
         wb.ray_d[0][rs] = make_float4(so.next_d.x, so.next_d.y, so.next_d.z, 0.0f);
