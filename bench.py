@@ -424,6 +424,7 @@ def main() -> int:
     ap.add_argument("--persistent", type=int, default=None, help="persistent traversal waves per CU (0: off)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a persistent refill")
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
+    ap.add_argument("--period", type=int, default=None, help="persistent: instance changes run at least every n steps")
     ap.add_argument("--lds", type=int, default=None, help="persistent: traversal stack entries kept in LDS")
     ap.add_argument("--tps", type=int, default=None, help="persistent: leaf triangles tested per step (1 or 2)")
     ap.add_argument("--concurrent", type=int, default=None, choices=(0, 1),
@@ -471,6 +472,8 @@ def main() -> int:
         dev.set_option(N.OPT_REFILL_LANES, args.refill)
     if args.gather is not None:
         dev.set_option(N.OPT_GATHER_LANES, args.gather)
+    if args.period is not None:
+        dev.set_option(N.OPT_ADVANCE_PERIOD, args.period)
     if args.lds is not None:
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, args.lds)
     if args.tps is not None:
