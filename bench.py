@@ -425,6 +425,8 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a persistent refill")
     ap.add_argument("--gather", type=int, default=None, help="lanes gathered before an instance change runs")
     ap.add_argument("--period", type=int, default=None, help="persistent: instance changes run at least every n steps")
+    ap.add_argument("--pair-waves", type=int, default=None,
+                    help="persistent: waves per CU of each of two concurrent launches (0: half)")
     ap.add_argument("--lds", type=int, default=None, help="persistent: traversal stack entries kept in LDS")
     ap.add_argument("--tps", type=int, default=None, help="persistent: leaf triangles tested per step (1 or 2)")
     ap.add_argument("--concurrent", type=int, default=None, choices=(0, 1),
@@ -474,6 +476,8 @@ def main() -> int:
         dev.set_option(N.OPT_GATHER_LANES, args.gather)
     if args.period is not None:
         dev.set_option(N.OPT_ADVANCE_PERIOD, args.period)
+    if args.pair_waves is not None:
+        dev.set_option(N.OPT_PAIR_WAVES, args.pair_waves)
     if args.lds is not None:
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, args.lds)
     if args.tps is not None:

@@ -416,6 +416,11 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                        /* bounce k+1's extension rays trace (the two */
                                        /* launches fill each other's tails); 0 = one */
                                        /* after the other.  Same results either way  */
+#define PULSE_OPT_PAIR_WAVES 11    /* persistent, concurrent shadow launches: waves */
+                                   /* per CU of each of the two launches that run at */
+                                   /* once (0, the default: half of                  */
+                                   /* PULSE_OPT_PERSISTENT_WAVES), so that both start */
+                                   /* at once; launches that run alone take all      */
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

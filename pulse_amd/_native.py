@@ -53,6 +53,7 @@ OPT_LDS_STACK_ENTRIES = 7
 OPT_TRIS_PER_STEP = 8
 OPT_ADVANCE_PERIOD = 9
 OPT_CONCURRENT_SHADOW = 10
+OPT_PAIR_WAVES = 11
 
 
 class PulseError(RuntimeError):

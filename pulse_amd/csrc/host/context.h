@@ -150,6 +150,8 @@ struct pulse_ctx {
     // stream, the fork/join events that order it against `stream` (which
     // stays the one every call completes on), and the timing events
     int concurrent_shadow = 1;
+    int pair_waves = 0;  // persistent waves per CU of each of two launches that run at once (0: half)
+    int eff_pair_waves() const { return pair_waves ? pair_waves : std::max(1, persistent / 2); }
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev[pt::kMaxTraces] = {}, join_ev[pt::kMaxTraces] = {};
     hipEvent_t ev2[6 * pt::kMaxTraces + 4] = {};

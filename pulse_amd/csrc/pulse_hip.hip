@@ -116,8 +116,10 @@ void launch_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ, ui
 }
 // Persistent grid: `persistent` waves per CU (never more than the queue
 // could need).
-uint32_t persistent_grid(pulse_ctx *ctx, uint32_t cap) {
-    uint32_t g = static_cast<uint32_t>(ctx->num_cus) * static_cast<uint32_t>(ctx->persistent);
+// Persistent grid: `waves` per CU (0: PULSE_OPT_PERSISTENT_WAVES), no more
+// waves than the queue's capacity needs.
+uint32_t persistent_grid(pulse_ctx *ctx, uint32_t cap, int waves = 0) {
+    uint32_t g = static_cast<uint32_t>(ctx->num_cus) * static_cast<uint32_t>(waves > 0 ? waves : ctx->persistent);
     return std::max<uint32_t>(1, std::min<uint32_t>(g, (cap + kTraceBlock - 1) / kTraceBlock));
 }
 // LDS entries of the persistent kernels' stacks: the compiled size at most
@@ -173,10 +175,11 @@ int ensure_spill(pulse_ctx *ctx, uint32_t cap) {
 }
 // Persistent extend/shadow on stream `st` (default: the context's) with
 // spill part `part` (0 or 1: launches that may run at once use different
-// parts); ensure_spill first.
+// parts) and `waves` per CU (0: PULSE_OPT_PERSISTENT_WAVES); ensure_spill
+// first.
 int launch_queue(pulse_ctx *ctx, const WfBuffers &wb, uint32_t cap, bool shadow, int q, int k,
-                 hipStream_t st = nullptr, int part = 0) {
-    const uint32_t grid = persistent_grid(ctx, cap);
+                 hipStream_t st = nullptr, int part = 0, int waves = 0) {
+    const uint32_t grid = persistent_grid(ctx, cap, waves);
     const int lds = persistent_lds(ctx);
     if (!st) st = ctx->stream;
     uint32_t *spill = static_cast<int>(ctx->stack_entries) > lds
@@ -1142,13 +1145,17 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
                 q ^= 1;
                 HIP_TRY(hipEventRecord(ctx->fork_ev[k], A));
                 HIP_TRY(hipStreamWaitEvent(B, ctx->fork_ev[k], 0));
+                // a pair of launches shares the CUs from the start: each
+                // takes the paired share of waves; the last shadow launch,
+                // alone, all of them
+                const int pair_waves = k + 1 < T ? ctx->eff_pair_waves() : 0;
                 if (timing) HIP_TRY(hipEventRecord(E[1 + 4 * T + k], B));
-                if ((rc = launch_queue(ctx, wb, cap, true, 0, k, B, 1)) != PULSE_OK) return rc;
+                if ((rc = launch_queue(ctx, wb, cap, true, 0, k, B, 1, pair_waves)) != PULSE_OK) return rc;
                 if (timing) HIP_TRY(hipEventRecord(E[1 + 5 * T + k], B));
                 HIP_TRY(hipEventRecord(ctx->join_ev[k], B));
                 if (k + 1 < T) {
                     if (timing) HIP_TRY(hipEventRecord(E[1 + 2 * (k + 1)], A));
-                    if ((rc = launch_queue(ctx, wb, cap, false, q, k + 1, A, 0)) != PULSE_OK) return rc;
+                    if ((rc = launch_queue(ctx, wb, cap, false, q, k + 1, A, 0, pair_waves)) != PULSE_OK) return rc;
                     if (timing) HIP_TRY(hipEventRecord(E[2 + 2 * (k + 1)], A));
                 }
                 HIP_TRY(hipStreamWaitEvent(A, ctx->join_ev[k], 0));
@@ -1327,9 +1334,12 @@ int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width,
                 HIP_TRY(hipEventRecord(ctx->fork_ev[k + 1], A));
                 HIP_TRY(hipStreamWaitEvent(B, ctx->fork_ev[k + 1], 0));
                 // k = -1: NEE at the G-buffer points
-                if ((rc = launch_queue(ctx, wb, npix, true, 0, k < 0 ? kMaxTraces : k, B, 1)) != PULSE_OK) return rc;
+                const int pair_waves = k + 1 < T ? ctx->eff_pair_waves() : 0;
+                if ((rc = launch_queue(ctx, wb, npix, true, 0, k < 0 ? kMaxTraces : k, B, 1, pair_waves)) != PULSE_OK)
+                    return rc;
                 HIP_TRY(hipEventRecord(ctx->join_ev[k + 1], B));
-                if (k + 1 < T && (rc = launch_queue(ctx, wb, npix, false, q, k + 1, A, 0)) != PULSE_OK) return rc;
+                if (k + 1 < T && (rc = launch_queue(ctx, wb, npix, false, q, k + 1, A, 0, pair_waves)) != PULSE_OK)
+                    return rc;
                 HIP_TRY(hipStreamWaitEvent(A, ctx->join_ev[k + 1], 0));
             }
             hipLaunchKernelGGL(k_gi_accumulate, dim3(grid), dim3(kShadeBlock), 0, A, wb,
@@ -1493,6 +1503,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "concurrent shadow must be 0 or 1");
             ctx->concurrent_shadow = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_PAIR_WAVES:
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
+            ctx->pair_waves = static_cast<int>(value);
+            return PULSE_OK;
         default:
             PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
     }
@@ -1507,6 +1521,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_GATHER_LANES: *value = ctx->eff_gather(); return PULSE_OK;
         case PULSE_OPT_ADVANCE_PERIOD: *value = ctx->eff_period(); return PULSE_OK;
         case PULSE_OPT_CONCURRENT_SHADOW: *value = ctx->concurrent_shadow; return PULSE_OK;
+        case PULSE_OPT_PAIR_WAVES: *value = ctx->eff_pair_waves(); return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
