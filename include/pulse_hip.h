@@ -398,7 +398,7 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                      /* take new rays as theirs finish               */
 #define PULSE_OPT_REFILL_LANES 5   /* persistent: idle lanes that trigger a refill   */
                                    /* (0: the traversal loop's tuned value, 16 for   */
-                                   /* loop 1, 32 for loop 2)                         */
+                                   /* loop 1, 24 for loop 2)                         */
 #define PULSE_OPT_GATHER_LANES 6   /* persistent: lanes waiting for an instance change */
                                    /* before the wave runs it (1: no waiting; 0: the */
                                    /* loop's tuned value, 1 for loop 1, 64 for 2)    */

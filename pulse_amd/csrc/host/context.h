@@ -135,7 +135,7 @@ struct pulse_ctx {
     int gather = 0;              // persistent: lanes gathered before an instance change runs (0: tuned)
     int advance_period = 0;      // persistent: instance changes run at least every n steps (0: tuned)
     // the values in effect: explicit settings, else the traversal loop's tuned ones
-    int eff_refill() const { return refill ? refill : (loop_variant == 2 ? 32 : 16); }
+    int eff_refill() const { return refill ? refill : (loop_variant == 2 ? 24 : 16); }
     int eff_gather() const { return gather ? gather : (loop_variant == 2 ? 64 : 1); }
     int eff_period() const { return advance_period ? advance_period : (loop_variant == 2 ? 2 : 1); }
     int lds_entries = 24;        // persistent: stack entries kept in LDS (0: all); the rest spill
