@@ -189,6 +189,19 @@ def test_concurrent_shadow_launches_keep_images(dev, views, loop, use_graph):
     finally:
         dev.set_option(N.OPT_CONCURRENT_SHADOW, 1)
     conc = _sequential(dev, views, prm, [0, 2], loop=loop)
+    # the waves each of a concurrent pair takes (PULSE_OPT_PAIR_WAVES) do not
+    # change the images either
+    assert dev.get_option(N.OPT_PAIR_WAVES) == dev.get_option(N.OPT_PERSISTENT_WAVES) // 2
+    with pytest.raises(N.PulseError):
+        dev.set_option(N.OPT_PAIR_WAVES, 65)
+    for w in (1, 24):
+        dev.set_option(N.OPT_PAIR_WAVES, w)
+        try:
+            other = _sequential(dev, views, prm, [0, 2], loop=loop)
+        finally:
+            dev.set_option(N.OPT_PAIR_WAVES, 0)
+        for i, (o, r) in enumerate(zip(other, ref)):
+            assert same_bits(o, r).all(), f"pair waves {w}, view {i}"
     b = Batch(dev, views, prm)
     b.set_config(use_graph=use_graph, traversal_loop=loop)
     b.render(0)
