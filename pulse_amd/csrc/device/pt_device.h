@@ -241,12 +241,17 @@ struct LdsStack {
     int lds_cap = 1 << 30;     // entries held in LDS
     uint32_t *spill = nullptr;
     uint32_t spill_stride = 0;
+    // `base` points into LDS (every stack's top part lives there): accessed
+    // through an LDS pointer, an entry is an LDS read or write, never a
+    // generic-address one that picks LDS or the spill column per lane.
+    using lds_word = __attribute__((address_space(3))) uint32_t;
+    __device__ __forceinline__ lds_word *lds() const { return (lds_word *)base; }
     __device__ __forceinline__ void put(int i, uint32_t v) {
-        if (i < lds_cap) base[i * stride] = v;
+        if (i < lds_cap) lds()[i * stride] = v;
         else spill[static_cast<uint32_t>(i - lds_cap) * spill_stride] = v;
     }
     __device__ __forceinline__ uint32_t get(int i) const {
-        if (i < lds_cap) return base[i * stride];
+        if (i < lds_cap) return lds()[i * stride];
         return spill[static_cast<uint32_t>(i - lds_cap) * spill_stride];
     }
 };

@@ -67,9 +67,9 @@ struct QuantFrame {
 __device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bn, float &bf) {
     sr = s * r;
     const float b = (org - o) * r;
-    const float pad = 0x1p-20f * __builtin_fmaf(255.0f, fabsf(sr), fabsf(b));
-    bn = b - pad;
-    bf = b + pad;
+    const float a = __builtin_fmaf(255.0f, fabsf(sr), fabsf(b));
+    bn = __builtin_fmaf(-0x1p-20f, a, b);  // b - 2^-20 a, one rounding
+    bf = __builtin_fmaf(0x1p-20f, a, b);
 }
 __device__ __forceinline__ QuantFrame quant_frame(const WideRay &w, uint4 a) {
     QuantFrame f;
@@ -165,7 +165,7 @@ struct WideTraversal {
     V3 wo, wd;         // world ray
     float threshold, slack_thr, any_cull;  // any_cull: occlusion mode's world culling bound
     WideRay cur;       // ray of the current level
-    uint32_t nbase;    // first node of the current level's tree (0 in the TLAS)
+    const uint4 *level;  // 16-byte words of the current level's tree (its root node first)
     bool in_blas;
     int sp, sp_base;
     int iter;             // wide node visits of this ray (a guard: validated trees stay far below)
@@ -206,7 +206,7 @@ struct WideTraversal {
             any_cull = threshold + slack_thr;
         }
         cur = wide_ray(wo, wd);
-        nbase = 0u;
+        level = reinterpret_cast<const uint4 *>(sc.wide_tlas);
         entry = 0u;
         in_blas = false;
         sp = 0;
@@ -247,8 +247,7 @@ struct WideTraversal {
             }
             iter += 1;
             const float cur_t = in_blas ? h.t : wcull();  // culling bound of this level
-            const uint4 *p =
-                reinterpret_cast<const uint4 *>(in_blas ? sc.wide_blas : sc.wide_tlas) + 4u * (nbase + entry);
+            const uint4 *p = level + 4u * entry;
             const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
             const QuantFrame f = quant_frame(cur, qa);
             const QuantPlanes pl = quant_planes(cur, qb, qc);
@@ -313,9 +312,9 @@ struct WideTraversal {
                                           uint32_t e2, float d3, uint32_t e3) {
         if (sp + 3 <= stk.lds_cap && sp + 3 <= stk.capacity) {
             const int n = (d1 != kTFar ? 1 : 0) + (d2 != kTFar ? 1 : 0) + (d3 != kTFar ? 1 : 0);
-            stk.base[sp * stk.stride] = n == 3 ? e3 : (n == 2 ? e2 : e1);
-            stk.base[(sp + 1) * stk.stride] = n == 3 ? e2 : e1;
-            stk.base[(sp + 2) * stk.stride] = e1;
+            stk.lds()[sp * stk.stride] = n == 3 ? e3 : (n == 2 ? e2 : e1);
+            stk.lds()[(sp + 1) * stk.stride] = n == 3 ? e2 : e1;
+            stk.lds()[(sp + 2) * stk.stride] = e1;
             sp += n;
         } else {
             push(sc, stk, d3, e3);
@@ -396,7 +395,7 @@ struct WideTraversal {
             h.tri = 0;
             h.u = 0.0f;
             h.v = 0.0f;
-            nbase = sc.wide_root[inst];
+            level = reinterpret_cast<const uint4 *>(sc.wide_blas) + 4u * sc.wide_root[inst];
             in_blas = true;
             sp_base = sp;
             // (no triangles remain here: every leaf runs to its end)
@@ -406,7 +405,7 @@ struct WideTraversal {
         // back to the TLAS
         in_blas = false;
         cur = wide_ray(wo, wd);
-        nbase = 0u;
+        level = reinterpret_cast<const uint4 *>(sc.wide_tlas);
         sp_base = 0;
         if (sp == 0) return false;
         sp -= 1;

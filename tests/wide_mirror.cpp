@@ -38,9 +38,9 @@ Frame frame(const uint32_t *q, V3<float> o, V3<float> r) {
         const float s = u2f(((q[3] >> (8 * a)) & 0xffu) << 23);
         f.sr[a] = s * rc[a];
         const float b = (u2f(q[a]) - oc[a]) * rc[a];
-        const float pad = 0x1p-20f * std::fma(255.0f, std::fabs(f.sr[a]), std::fabs(b));
-        f.bn[a] = b - pad;
-        f.bf[a] = b + pad;
+        const float m = std::fma(255.0f, std::fabs(f.sr[a]), std::fabs(b));
+        f.bn[a] = std::fma(-0x1p-20f, m, b);
+        f.bf[a] = std::fma(0x1p-20f, m, b);
         f.neg[a] = rc[a] < 0.0f;
     }
     return f;
