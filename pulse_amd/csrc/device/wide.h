@@ -51,8 +51,10 @@ __device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
 
 // Ray-side constants of one quantised node: along each axis the distance to
 // plane q is q * (scale / d) + (origin - o) / d, one fma per plane.  The
-// rounding of those distances (the reciprocal, the products, the fma) is a
-// few 2^-24 of |origin - o| / |d| + 255 scale / |d| on that axis; the
+// plane bytes enter that fma as the binary16 values 1024 + q (qpair below),
+// so the constant term is (origin - o) / d - 1024 scale / d.  The rounding of
+// those distances (the reciprocal, the products, the constant term, the fma)
+// is a few 2^-24 of |origin - o| / |d| + 2048 scale / |d| on that axis; the
 // constant term of the entry plane is lowered and that of the exit plane
 // raised by 2^-20 of it, so every computed slab contains the exact one.
 // Which plane of an axis is the entry is fixed by the sign of d on that
@@ -67,9 +69,10 @@ struct QuantFrame {
 __device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bn, float &bf) {
     sr = s * r;
     const float b = (org - o) * r;
-    const float a = __builtin_fmaf(255.0f, fabsf(sr), fabsf(b));
-    bn = __builtin_fmaf(-0x1p-20f, a, b);  // b - 2^-20 a, one rounding
-    bf = __builtin_fmaf(0x1p-20f, a, b);
+    const float b1 = __builtin_fmaf(-1024.0f, sr, b);
+    const float a = __builtin_fmaf(2048.0f, fabsf(sr), fabsf(b));
+    bn = __builtin_fmaf(-0x1p-20f, a, b1);  // b1 - 2^-20 a, one rounding
+    bf = __builtin_fmaf(0x1p-20f, a, b1);
 }
 __device__ __forceinline__ QuantFrame quant_frame(const WideRay &w, uint4 a) {
     QuantFrame f;
@@ -79,31 +82,48 @@ __device__ __forceinline__ QuantFrame quant_frame(const WideRay &w, uint4 a) {
     return f;
 }
 
-// Plane bytes of the four children: entry (n*) and exit (f*) plane per axis.
+// Two plane bytes of a plane word (children 0-1, or 2-3 with `hi`) as the
+// two binary16 halves 1024 + q of one word: byte q under the exponent byte
+// 0x64 is exactly 1024 + q (one byte permute, no conversions; the fma that
+// uses a half widens it to f32 itself, v_fma_mix_f32).
+__device__ __forceinline__ uint32_t qpair(uint32_t w, bool hi) {
+    return __builtin_amdgcn_perm(0x64646464u, w, hi ? 0x04030402u : 0x04010400u);
+}
+
+// Planes of the four children as binary16 pairs: entry (n*) and exit (f*)
+// plane per axis, children 0-1 in [0] and 2-3 in [1].
 struct QuantPlanes {
-    uint32_t nx, fx, ny, fy, nz, fz;
+    uint32_t nx[2], fx[2], ny[2], fy[2], nz[2], fz[2];
 };
+__device__ __forceinline__ void quant_axis_planes(bool neg, uint32_t lo, uint32_t hi, uint32_t *n, uint32_t *f) {
+    const uint32_t wn = neg ? hi : lo, wf = neg ? lo : hi;
+    n[0] = qpair(wn, false);
+    n[1] = qpair(wn, true);
+    f[0] = qpair(wf, false);
+    f[1] = qpair(wf, true);
+}
 __device__ __forceinline__ QuantPlanes quant_planes(const WideRay &w, uint4 b, uint4 c) {
     QuantPlanes p;
-    const bool sx = w.r.x < 0.0f, sy = w.r.y < 0.0f, sz = w.r.z < 0.0f;
-    p.nx = sx ? b.y : b.x;
-    p.fx = sx ? b.x : b.y;
-    p.ny = sy ? b.w : b.z;
-    p.fy = sy ? b.z : b.w;
-    p.nz = sz ? c.y : c.x;
-    p.fz = sz ? c.x : c.y;
+    quant_axis_planes(w.r.x < 0.0f, b.x, b.y, p.nx, p.fx);
+    quant_axis_planes(w.r.y < 0.0f, b.z, b.w, p.ny, p.fy);
+    quant_axis_planes(w.r.z < 0.0f, c.x, c.y, p.nz, p.fz);
     return p;
 }
 
-__device__ __forceinline__ float qbyte(uint32_t w, int k) { return static_cast<float>((w >> (8 * k)) & 0xffu); }
+// 1024 + the plane byte of child k.
+__device__ __forceinline__ float qplane(const uint32_t *pair, int k) {
+    const uint32_t w = pair[k >> 1];
+    const uint16_t bits = static_cast<uint16_t>((k & 1) ? w >> 16 : w & 0xffffu);
+    return static_cast<float>(__builtin_bit_cast(_Float16, bits));
+}
 
-// Slab test of child k (plane bytes k of p): its entry distance, or kTFar
-// when the box is missed, behind the origin or at or beyond `bound`.
+// Slab test of child k: its entry distance, or kTFar when the box is
+// missed, behind the origin or at or beyond `bound`.
 __device__ __forceinline__ float quant_slab(const QuantFrame &f, float bound, const QuantPlanes &p, int k) {
-    const float tmin = fmaxf(fmaxf(__builtin_fmaf(qbyte(p.nx, k), f.srx, f.bnx), __builtin_fmaf(qbyte(p.ny, k), f.sry, f.bny)),
-                             __builtin_fmaf(qbyte(p.nz, k), f.srz, f.bnz));
-    const float tmax = fminf(fminf(__builtin_fmaf(qbyte(p.fx, k), f.srx, f.bfx), __builtin_fmaf(qbyte(p.fy, k), f.sry, f.bfy)),
-                             __builtin_fmaf(qbyte(p.fz, k), f.srz, f.bfz));
+    const float tmin = fmaxf(fmaxf(__builtin_fmaf(qplane(p.nx, k), f.srx, f.bnx), __builtin_fmaf(qplane(p.ny, k), f.sry, f.bny)),
+                             __builtin_fmaf(qplane(p.nz, k), f.srz, f.bnz));
+    const float tmax = fminf(fminf(__builtin_fmaf(qplane(p.fx, k), f.srx, f.bfx), __builtin_fmaf(qplane(p.fy, k), f.sry, f.bfy)),
+                             __builtin_fmaf(qplane(p.fz, k), f.srz, f.bfz));
     return (tmax >= tmin && tmin < bound && tmax > 0.0f) ? tmin : kTFar;
 }
 

@@ -38,9 +38,10 @@ Frame frame(const uint32_t *q, V3<float> o, V3<float> r) {
         const float s = u2f(((q[3] >> (8 * a)) & 0xffu) << 23);
         f.sr[a] = s * rc[a];
         const float b = (u2f(q[a]) - oc[a]) * rc[a];
-        const float m = std::fma(255.0f, std::fabs(f.sr[a]), std::fabs(b));
-        f.bn[a] = std::fma(-0x1p-20f, m, b);
-        f.bf[a] = std::fma(0x1p-20f, m, b);
+        const float b1 = std::fma(-1024.0f, f.sr[a], b);  // the planes enter as 1024 + q
+        const float m = std::fma(2048.0f, std::fabs(f.sr[a]), std::fabs(b));
+        f.bn[a] = std::fma(-0x1p-20f, m, b1);
+        f.bf[a] = std::fma(0x1p-20f, m, b1);
         f.neg[a] = rc[a] < 0.0f;
     }
     return f;
@@ -49,8 +50,8 @@ float slab(const Frame &f, float bound, const uint32_t *q, int k) {
     float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
         // entry plane: the low one unless the direction is negative
-        const float qn = static_cast<float>((q[(f.neg[a] ? 5 : 4) + 2 * a] >> (8 * k)) & 0xffu);
-        const float qf = static_cast<float>((q[(f.neg[a] ? 4 : 5) + 2 * a] >> (8 * k)) & 0xffu);
+        const float qn = 1024.0f + static_cast<float>((q[(f.neg[a] ? 5 : 4) + 2 * a] >> (8 * k)) & 0xffu);
+        const float qf = 1024.0f + static_cast<float>((q[(f.neg[a] ? 4 : 5) + 2 * a] >> (8 * k)) & 0xffu);
         lo[a] = std::fma(qn, f.sr[a], f.bn[a]);
         hi[a] = std::fma(qf, f.sr[a], f.bf[a]);
     }
