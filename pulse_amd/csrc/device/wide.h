@@ -64,7 +64,7 @@ __device__ __forceinline__ WideRay wide_ray(V3 o, V3 d) {
 // entry term never exceeds the exit term).
 struct QuantFrame {
     float srx, sry, srz;                 // scale / d per axis
-    float bnx, bfx, bny, bfy, bnz, bfz;  // (origin - o) / d for the entry / exit planes, padded
+    float bnx, bfx, bny, bfy, bnz, bfz;  // (origin - o) / d - 1024 scale / d for the entry / exit planes, padded
 };
 __device__ __forceinline__ void quant_axis(float s, float org, float o, float r, float &sr, float &bn, float &bf) {
     sr = s * r;
