@@ -173,7 +173,8 @@ def random_rotation(u1: float, u2: float, u3: float) -> np.ndarray:
 # scenes
 # ---------------------------------------------------------------------------
 def cornell(seed: int = 0, threads: int = 8):
-    """Config-1 scene; returns (SceneBuffers, camera factory)."""
+    """Cornell box (walls, two boxes, ceiling light); returns (SceneBuffers,
+    camera factory)."""
     rng = CounterRng(seed)
     u = rng.uniform(1, 16)
     sb = SceneBuilder()
@@ -232,7 +233,8 @@ def icosphere_field(n_meshes: int = 64, subdiv: int = 6, n_instances: int = 800,
                     box: float = 100.0, radius=(1.5, 4.0), roughness=(0.05, 1.0), emission: float = 8.0,
                     seed: int = 2, threads: int = 8, n_materials: int | None = None, keep_builder: bool = False,
                     per_instance_materials: bool = False):
-    """Config-2 style scene; returns (SceneBuffers, camera factory).
+    """Field of instanced displaced icospheres in a `box`-sized cube; returns
+    (SceneBuffers, camera factory).
 
     Mesh k gets material k % n_materials (default: one material per mesh);
     instance i uses mesh i % n_meshes; `n_emissive` instances (chosen by the
