@@ -76,7 +76,12 @@ def cpu_baseline(scene, cam, prm, seconds: float):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleScene
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # one thread per host core this process may run on (the affinity mask:
+    # a container's CPU share, not the machine's core count)
+    try:
+        threads = max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        threads = max(1, os.cpu_count() or 1)
     orc = OracleScene(scene)
     acc = np.zeros((cam.height, cam.width, 4), np.float32)
     rows, ms_total = 0, 0.0
@@ -111,9 +116,12 @@ def cpu_baseline(scene, cam, prm, seconds: float):
         "value": incoherent / (ms_total / 1e3) / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model,
         "kind": "port",
         "sample": f"{rows} of {cam.height} rows of the same frame ({rows * cam.width} paths, {tot['rays']} rays, "
-                  f"{incoherent} incoherent) in {ms_total / 1e3:.1f} s with {threads} threads on {cpu_model}; "
+                  f"{incoherent} incoherent) in {ms_total / 1e3:.1f} s, one thread per core of the "
+                  f"affinity mask ({threads} of nproc {os.cpu_count()}); "
                   f"value = incoherent rays / whole render time",
         "all_rays_mrays_per_s": tot["rays"] / (ms_total / 1e3) / 1e6,
     }

@@ -37,7 +37,9 @@ extern "C" {
 #define PULSE_E_LAYOUT -3          /* buffer byte size not a multiple of the */
                                    /* element size, or an index out of range */
 #define PULSE_E_STACK_OVERFLOW -4  /* traversal stack exceeded (device flag) */
-#define PULSE_E_ITERATION_CAP -5   /* traversal hit the 100000 loop cap      */
+#define PULSE_E_ITERATION_CAP -5   /* traversal hit its loop cap (100000    */
+                                   /* steps per walk for loops 0/1, 2^22     */
+                                   /* node visits per ray for loop 2)        */
 #define PULSE_E_NO_SCENE -6        /* trace called before scene upload       */
 #define PULSE_E_UNSUPPORTED -7     /* parameter combination not supported    */
 #define PULSE_E_BUILD -8           /* host BVH builder failed                */
@@ -236,7 +238,7 @@ typedef struct pulse_stats {
     uint64_t extension_rays;   /* all closest-hit rays incl. primary       */
     uint64_t incoherent_rays;  /* extension rays at depth >= 1 + shadow    */
     uint64_t shadow_rays;
-    uint32_t error_flags;      /* bit 0 stack overflow, bit 1 iteration cap */
+    uint32_t error_flags;      /* PULSE_ERR_* raised since the last read    */
     uint32_t stack_entries;    /* traversal stack depth used by the kernels */
     double ms_total;           /* generate .. accumulate, all samples      */
     double ms_primary_extend;  /* closest-hit stage at trace index 0       */
@@ -421,6 +423,15 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                    /* once (0, the default: half of                  */
                                    /* PULSE_OPT_PERSISTENT_WAVES), so that both start */
                                    /* at once; launches that run alone take all      */
+#define PULSE_OPT_INJECT_ERROR 12  /* write-only, for tests of error reporting: ORs */
+                                   /* the value (PULSE_ERR_* bits) into the device */
+                                   /* error word, as a kernel that met the          */
+                                   /* condition would                                */
+/* Bits of the device error word (pulse_stats.error_flags).  pulse_sync,
+ * pulse_get_stats and pulse_batch_get_stats report the bits raised since the
+ * word was last read and clear it: each condition is reported once. */
+#define PULSE_ERR_STACK_OVERFLOW 1u
+#define PULSE_ERR_ITERATION_CAP 2u
 int pulse_set_option(pulse_ctx *ctx, int option, int64_t value);
 int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value);
 

@@ -463,6 +463,55 @@ void oracle_direct_light_f32(oracle_scene *sc, const float *p0, const float *n0,
     out[1] = d.y;
     out[2] = d.z;
 }
+// n NEE samples at one receiver (p0, n0, material, wo), sample i drawn from
+// rng state seeds[i]: ggx = 0 -> sample_direct_light (Lambert), 1 ->
+// sample_direct_light_ggx.  out = n x direct[3].
+void oracle_direct_light_batch_f32(oracle_scene *sc, const float *p0, const float *n0, const pulse_material *m,
+                                   const float *wo, int ggx, const uint32_t *seeds, uint32_t n, float *out,
+                                   int threads) {
+    const Scene &s = S(sc);
+    const MaterialR<float> mat = load_material<float>(*m);
+    parallel_for(n, threads, [&](uint32_t i, int) {
+        Counters c;
+        DecisionTape tape;
+        uint32_t rng = seeds[i];
+        V3<float> d = ggx ? sample_direct_light_ggx<float>(s, load3<float>(p0), load3<float>(n0), mat, load3<float>(wo),
+                                                           rng, c, tape)
+                          : sample_direct_light<float>(s, load3<float>(p0), load3<float>(n0), mat.base, rng, c, tape);
+        out[3 * i] = d.x;
+        out[3 * i + 1] = d.y;
+        out[3 * i + 2] = d.z;
+    });
+}
+// importance_sample_ggx_d for n draws at one (n, wo, material), draw i from
+// rng state seeds[i]: out = n x (wi[3], reflectance[3]).
+void oracle_sample_ggx_batch_f32(const float *nrm, const float *wo, const pulse_material *m, const uint32_t *seeds,
+                                 uint32_t n, float *out) {
+    const MaterialR<float> mat = load_material<float>(*m);
+    for (uint32_t i = 0; i < n; ++i) {
+        DecisionTape tape;
+        uint32_t rng = seeds[i];
+        SampleResult<float> r = importance_sample_ggx_d<float>(load3<float>(nrm), load3<float>(wo), mat, rng, tape);
+        float *o = out + 6 * i;
+        o[0] = r.wi.x;
+        o[1] = r.wi.y;
+        o[2] = r.wi.z;
+        o[3] = r.reflectance.x;
+        o[4] = r.reflectance.y;
+        o[5] = r.reflectance.z;
+    }
+}
+// One Russian-roulette step (path_tracer.wgsl:104-108) on throughput t[3]:
+// returns 1 if the path survives (t then divided by p), 0 if it stops.
+int oracle_russian_roulette_f32(float *t, uint32_t *rng) {
+    DecisionTape tape;
+    V3<float> v = load3<float>(t);
+    bool alive = russian_roulette<float>(v, *rng, tape);
+    t[0] = v.x;
+    t[1] = v.y;
+    t[2] = v.z;
+    return alive ? 1 : 0;
+}
 void oracle_pt_math(const float *x, uint32_t n, float *sin_out, float *cos_out, float *acos_out) {
     for (uint32_t i = 0; i < n; ++i) {
         sin_out[i] = pt_sinf(x[i]);

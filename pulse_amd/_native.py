@@ -54,6 +54,9 @@ OPT_TRIS_PER_STEP = 8
 OPT_ADVANCE_PERIOD = 9
 OPT_CONCURRENT_SHADOW = 10
 OPT_PAIR_WAVES = 11
+OPT_INJECT_ERROR = 12
+ERR_STACK_OVERFLOW = 1
+ERR_ITERATION_CAP = 2
 
 
 class PulseError(RuntimeError):

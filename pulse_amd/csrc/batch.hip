@@ -512,7 +512,8 @@ int pulse_batch_get_stats(pulse_batch *b, pulse_stats *out) {
     uint64_t t[2] = {0, 0};
     uint32_t err = 0;
     HIP_TRY(hipMemcpy(t, b->totals.ptr, sizeof(t), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(&err, b->ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    int rc = pulse_host::take_device_errors(b->ctx, &err);
+    if (rc != PULSE_OK) return rc;
     HIP_TRY(hipMemset(b->totals.ptr, 0, sizeof(t)));
     *out = pulse_stats{};
     out->extension_rays = t[0];

@@ -108,6 +108,9 @@ int scene_refit_instances(SceneData &s, const float *object_world, uint32_t n, h
 int check_traversal(const SceneData &s, int loop);
 // Device view of a scene; `errors` is the context's device error word.
 pt::DevScene dev_scene(const SceneData &s, uint32_t *errors);
+// Reads the context's device error word into *out and clears it when set
+// (each condition is reported once); the caller has synchronised the stream.
+int take_device_errors(pulse_ctx *ctx, uint32_t *out);
 // Smallest compiled stack size that holds `need` entries (0: none).
 uint32_t pick_stack(uint32_t need, int loop_variant);
 // Stack entries the traversal `loop_variant` needs on scene `s` (0: the

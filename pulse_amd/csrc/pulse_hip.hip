@@ -329,6 +329,13 @@ uint32_t stack_for(const SceneData &s, int loop_variant) {
     return pick_stack(s.stack_need, loop_variant);
 }
 
+int take_device_errors(pulse_ctx *ctx, uint32_t *out) {
+    *out = 0;
+    HIP_TRY(hipMemcpy(out, ctx->errors.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (*out != 0) HIP_TRY(hipMemset(ctx->errors.ptr, 0, sizeof(uint32_t)));
+    return PULSE_OK;
+}
+
 DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     DevScene s;
     s.tri_bvh = sd.buf[B_TRI_BVH].as<float4>();
@@ -867,9 +874,12 @@ int pulse_sync(pulse_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     uint32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    int rc = pulse_host::take_device_errors(ctx, &err);
+    if (rc != PULSE_OK) return rc;
     if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
-    if (err & kErrIterationCap) PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached the 100000-iteration cap");
+    if (err & kErrIterationCap)
+        PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached its loop cap (%s)",
+                   ctx->loop_variant == 2 ? "2^22 node visits per ray" : "100000 steps per walk");
     return PULSE_OK;
 }
 
@@ -1507,6 +1517,16 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
             ctx->pair_waves = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_INJECT_ERROR: {
+            if (value < 0 || value > 0xffffffffll) PULSE_FAIL(PULSE_E_INVALID_ARG, "error bits must fit 32 bits");
+            HIP_TRY(hipSetDevice(ctx->device));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            uint32_t err = 0;
+            HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+            err |= static_cast<uint32_t>(value);
+            HIP_TRY(hipMemcpy(ctx->errors.ptr, &err, sizeof(err), hipMemcpyHostToDevice));
+            return PULSE_OK;
+        }
         default:
             PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_set_option: unknown or read-only option %d", option);
     }
@@ -1535,7 +1555,8 @@ int pulse_get_stats(pulse_ctx *ctx, pulse_stats *out) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     uint32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, ctx->errors.ptr, sizeof(err), hipMemcpyDeviceToHost));
+    int rc = pulse_host::take_device_errors(ctx, &err);
+    if (rc != PULSE_OK) return rc;
     *out = ctx->stats;
     out->error_flags = err;
     return PULSE_OK;

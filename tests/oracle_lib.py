@@ -71,6 +71,10 @@ def lib() -> C.CDLL:
             ("oracle_cdf_scan_array", _U32, [_P, _U32, C.c_float, C.c_int]),
             ("oracle_direct_light_f32", None, [_P, _P, _P, _P, C.POINTER(_U32), _P]),
             ("oracle_pt_math", None, [_P, _U32, _P, _P, _P]),
+            ("oracle_direct_light_batch_f32", None, [_P, _P, _P, C.POINTER(N.Material), _P, C.c_int, _P, _U32, _P,
+                                                     C.c_int]),
+            ("oracle_sample_ggx_batch_f32", None, [_P, _P, C.POINTER(N.Material), _P, _U32, _P]),
+            ("oracle_russian_roulette_f32", C.c_int, [_P, C.POINTER(_U32)]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
