@@ -350,6 +350,21 @@ int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
 int pulse_trace(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params,
                 float *accum_rgba);
 
+/* No counterpart in the reference.  pulse_trace for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same image bit for bit, with the frame's queues and path state
+ * in a fixed scratch array that the library holds on each device until the
+ * process ends (about 10.8 MB) instead of buffers it allocates.  Calls no
+ * hipMalloc/hipFree.  Limits: width*height <= pulse_trace_noalloc_max_pixels
+ * (65536, e.g. 256x256), else PULSE_E_UNSUPPORTED; no PULSE_FLAG_STAGE_TIMING
+ * or PULSE_FLAG_COUNT_WORK.  Launches run one after the other on the context
+ * stream with one ray per lane.  Every call on a device shares that device's
+ * scratch: calls that overlap on one GPU (two streams, two contexts, two
+ * threads) race; calls ordered on one stream do not. */
+int pulse_trace_noalloc(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params,
+                        float *accum_rgba);
+int pulse_trace_noalloc_max_pixels(uint64_t *out);
+
 /* The realtime GI pass (src/pulse/gi.wgsl:56-126, dispatched by the GI
  * plugin): paths start at the caller's G-buffer instead of camera rays.
  * Device arrays of width*height float4 each (w unused): world position,

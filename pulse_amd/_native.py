@@ -254,6 +254,8 @@ SIGNATURES = [
     ("pulse_scene_generation", _I, [_P, C.POINTER(C.c_uint64)]),
     ("pulse_scene_update_materials", _I, [_P, _P, _U32]),
     ("pulse_trace", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P]),
+    ("pulse_trace_noalloc", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P]),
+    ("pulse_trace_noalloc_max_pixels", _I, [C.POINTER(C.c_uint64)]),
     ("pulse_trace_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_trace_shadow_rays", _I, [_P, _P, _P, _U32]),
     ("pulse_get_stats", _I, [_P, C.POINTER(Stats)]),

@@ -127,6 +127,7 @@ struct pulse_ctx {
     pulse_host::SceneData *cur = nullptr; // the scene the launches use
     pulse_host::DevBuf wf[pulse_host::W_COUNT];
     pulse_host::DevBuf errors;  // u32 device error word
+    void *noalloc_scratch = nullptr;  // this device's address of pulse_trace_noalloc's scratch array
     size_t wf_pixels = 0;
     uint64_t generation = 0;
     uint32_t stack_entries = 0;  // traversal stack capacity in use

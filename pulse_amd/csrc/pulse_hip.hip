@@ -23,6 +23,18 @@ using namespace pt;
 
 using namespace pulse_host;
 
+// Scratch of pulse_trace_noalloc: the wavefront queues, path state and queue
+// counters of one frame of at most kNoAllocMaxPixels pixels (ten float4
+// arrays and one float array per pixel, then the counter words), one copy
+// per device, held until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress gives for the current device
+// (pulse_ctx::noalloc_scratch); every call on a device shares it.
+constexpr size_t kNoAllocMaxPixels = size_t(1) << 16;
+constexpr size_t kNoAllocF4 = 10;
+constexpr size_t kNoAllocBytes =
+    kNoAllocMaxPixels * (kNoAllocF4 * sizeof(float4) + sizeof(float)) + sizeof(uint32_t) * kCounterWords;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
 
 int set_device(pulse_ctx *ctx) {
@@ -264,6 +276,29 @@ int ensure_wavefront(pulse_ctx *ctx, size_t npix) {
     if ((rc = ctx->wf[W_COUNTERS].ensure(sizeof(uint32_t) * kCounterWords)) != PULSE_OK) return rc;
     ctx->wf_pixels = npix;
     return PULSE_OK;
+}
+
+// The same buffers laid out in one region of `max_pixels` slots each (the
+// allocation-free entry point's scratch).
+WfBuffers wf_buffers_in(unsigned char *base, size_t max_pixels) {
+    float4 *f4 = reinterpret_cast<float4 *>(base);
+    WfBuffers wb;
+    wb.ray_o[0] = f4 + 0 * max_pixels;
+    wb.ray_o[1] = f4 + 1 * max_pixels;
+    wb.ray_d[0] = f4 + 2 * max_pixels;
+    wb.ray_d[1] = f4 + 3 * max_pixels;
+    wb.hit = f4 + 4 * max_pixels;
+    wb.state_t = f4 + 5 * max_pixels;
+    wb.state_c = f4 + 6 * max_pixels;
+    wb.sh_o = f4 + 7 * max_pixels;
+    wb.sh_d = f4 + 8 * max_pixels;
+    wb.sh_c = f4 + 9 * max_pixels;
+    wb.hit_v = reinterpret_cast<float *>(f4 + kNoAllocF4 * max_pixels);
+    wb.ext_count = reinterpret_cast<uint32_t *>(wb.hit_v + max_pixels);
+    wb.sh_count = wb.ext_count + (kMaxTraces + 1);
+    wb.ext_head = wb.ext_count + 2 * (kMaxTraces + 1);
+    wb.sh_head = wb.ext_count + 3 * (kMaxTraces + 1);
+    return wb;
 }
 
 WfBuffers wf_buffers(pulse_ctx *ctx) {
@@ -837,6 +872,12 @@ int pulse_create(int device, pulse_ctx **out) {
         pulse_destroy(ctx);
         PULSE_FAIL(PULSE_E_HIP, "pulse_create: cannot allocate the error word");
     }
+    // this device's copy of the allocation-free entry point's scratch
+    e = hipGetSymbolAddress(&ctx->noalloc_scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ctx->noalloc_scratch) {
+        pulse_destroy(ctx);
+        PULSE_FAIL(PULSE_E_HIP, "pulse_create: no address for the scratch array: %s", hipGetErrorString(e));
+    }
     *out = ctx;
     return PULSE_OK;
 }
@@ -1076,29 +1117,25 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
     return PULSE_OK;
 }
 
-int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
-    if (!ctx || !cam || !prm || !accum_rgba) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: null argument");
-    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_trace: no scene uploaded");
-    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
-    if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: empty image");
-    if (prm->mode > PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown mode %u", prm->mode);
-    if (prm->nee_eval > PULSE_NEE_GGX) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace: unknown nee_eval %u", prm->nee_eval);
-    uint32_t traces = prm->mode == PULSE_MODE_REF_NEE_LAMBERT ? prm->max_depth + 1u : prm->max_depth;
-    if (traces > static_cast<uint32_t>(kMaxTraces))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace: max_depth %u above %d", prm->max_depth, kMaxTraces - 1);
-    int rc = set_device(ctx);
-    if (rc) return rc;
-    const size_t npix = static_cast<size_t>(cam->width) * cam->height;
-    if (npix > 0xffffffffull / 4) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace: image too large");
-    if ((rc = ensure_wavefront(ctx, npix)) != PULSE_OK) return rc;
-    WfBuffers wb = wf_buffers(ctx);
-    const bool timing = (prm->flags & PULSE_FLAG_STAGE_TIMING) != 0;
-    if (timing && !ctx->events_created) {
-        for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
-        ctx->events_created = true;
-    }
+}  // extern "C"
 
-    FrameArgs fa;
+namespace {
+// Checks of pulse_trace's arguments and the frame arguments they give; `who`
+// names the entry point in messages.
+int frame_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *accum_rgba,
+                const char *who, FrameArgs &fa, uint32_t &traces) {
+    if (!ctx || !cam || !prm || !accum_rgba) PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: null argument", who);
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "%s: no scene uploaded", who);
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
+    if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: empty image", who);
+    if (prm->mode > PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: unknown mode %u", who, prm->mode);
+    if (prm->nee_eval > PULSE_NEE_GGX) PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: unknown nee_eval %u", who, prm->nee_eval);
+    traces = prm->mode == PULSE_MODE_REF_NEE_LAMBERT ? prm->max_depth + 1u : prm->max_depth;
+    if (traces > static_cast<uint32_t>(kMaxTraces))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth %u above %d", who, prm->max_depth, kMaxTraces - 1);
+    const size_t npix = static_cast<size_t>(cam->width) * cam->height;
+    if (npix > 0xffffffffull / 4) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: image too large", who);
+    fa = FrameArgs{};
     fa.cam = *cam;
     fa.mode = prm->mode;
     fa.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
@@ -1108,8 +1145,28 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     fa.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
     fa.gi = 0;
     if (fa.light_power && ctx->cur->n_strength_cdf < ctx->cur->uniform.light_count)
-        PULSE_FAIL(PULSE_E_LAYOUT, "pulse_trace: PULSE_FLAG_LIGHT_POWER needs one light_emission_strength_cdf entry "
-                   "per light (%u for %u lights)", ctx->cur->n_strength_cdf, ctx->cur->uniform.light_count);
+        PULSE_FAIL(PULSE_E_LAYOUT, "%s: PULSE_FLAG_LIGHT_POWER needs one light_emission_strength_cdf entry "
+                   "per light (%u for %u lights)", who, ctx->cur->n_strength_cdf, ctx->cur->uniform.light_count);
+    return PULSE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
+    FrameArgs fa;
+    uint32_t traces = 0;
+    int rc = frame_setup(ctx, cam, prm, accum_rgba, "pulse_trace", fa, traces);
+    if (rc != PULSE_OK) return rc;
+    if ((rc = set_device(ctx)) != PULSE_OK) return rc;
+    const size_t npix = static_cast<size_t>(cam->width) * cam->height;
+    if ((rc = ensure_wavefront(ctx, npix)) != PULSE_OK) return rc;
+    WfBuffers wb = wf_buffers(ctx);
+    const bool timing = (prm->flags & PULSE_FLAG_STAGE_TIMING) != 0;
+    if (timing && !ctx->events_created) {
+        for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+        ctx->events_created = true;
+    }
     const uint32_t cap = static_cast<uint32_t>(npix);
     const uint32_t tiles = ((cam->width + 7) / 8) * ((cam->height + 7) / 8);
     const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
@@ -1277,6 +1334,74 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
+    return PULSE_OK;
+}
+
+int pulse_trace_noalloc(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
+    FrameArgs fa;
+    uint32_t traces = 0;
+    int rc = frame_setup(ctx, cam, prm, accum_rgba, "pulse_trace_noalloc", fa, traces);
+    if (rc != PULSE_OK) return rc;
+    const size_t npix = static_cast<size_t>(cam->width) * cam->height;
+    if (npix > kNoAllocMaxPixels)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_noalloc: %ux%u = %zu pixels, above the scratch's %zu", cam->width,
+                   cam->height, npix, kNoAllocMaxPixels);
+    if (prm->flags & (PULSE_FLAG_STAGE_TIMING | PULSE_FLAG_COUNT_WORK))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_trace_noalloc: stage timing and the work census need buffers of "
+                   "their own; use pulse_trace");
+    if (!ctx->noalloc_scratch) PULSE_FAIL(PULSE_E_HIP, "pulse_trace_noalloc: no scratch address for this device");
+    if ((rc = set_device(ctx)) != PULSE_OK) return rc;
+    WfBuffers wb = wf_buffers_in(static_cast<unsigned char *>(ctx->noalloc_scratch), kNoAllocMaxPixels);
+    const uint32_t cap = static_cast<uint32_t>(npix);
+    const uint32_t tiles = ((cam->width + 7) / 8) * ((cam->height + 7) / 8);
+    const uint32_t gen_grid = (tiles * 64 + kShadeBlock - 1) / kShadeBlock;
+    const uint32_t shade_grid = (cap + kShadeBlock - 1) / kShadeBlock;
+    const uint32_t clear_grid = (static_cast<uint32_t>(kCounterWords) + 255u) / 256u;
+    std::array<uint32_t, 2 * (kMaxTraces + 1)> counts{};
+    uint64_t ext_total = 0, ext_primary = 0, sh_total = 0;
+    // one ray per lane, the whole traversal stack in LDS (no spill area);
+    // every launch on the context's stream
+    for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
+        fa.accumulation_count = prm->accumulation_count + s;
+        hipLaunchKernelGGL(k_fill_zero, dim3(clear_grid), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<float *>(wb.ext_count), static_cast<uint32_t>(kCounterWords));
+        hipLaunchKernelGGL(k_generate, dim3(gen_grid), dim3(kShadeBlock), 0, ctx->stream, wb, fa);
+        int q = 0;
+        for (uint32_t k = 0; k < traces; ++k) {
+            PULSE_STACK_DISPATCH(launch_extend, ctx, wb, cap, q, static_cast<int>(k));
+            hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, dev_scene(ctx), wb, fa, q,
+                               static_cast<int>(k));
+            PULSE_STACK_DISPATCH(launch_shadow, ctx, wb, cap, static_cast<int>(k));
+            q ^= 1;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(shade_grid), dim3(kShadeBlock), 0, ctx->stream, wb,
+                           reinterpret_cast<float4 *>(accum_rgba), cap, fa.accumulation_count);
+        HIP_TRY(hipGetLastError());
+        if (prm->flags & PULSE_FLAG_COLLECT_STATS) {
+            HIP_TRY(hipMemcpyAsync(counts.data(), wb.ext_count, sizeof(uint32_t) * counts.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            for (uint32_t k = 0; k < traces; ++k) {
+                ext_total += counts[k];
+                sh_total += counts[kMaxTraces + 1 + k];
+            }
+            ext_primary += counts[0];
+        }
+    }
+    ctx->stats = pulse_stats{};
+    ctx->stats.primary_rays = ext_primary;
+    ctx->stats.extension_rays = ext_total;
+    ctx->stats.shadow_rays = sh_total;
+    ctx->stats.incoherent_rays = (ext_total - ext_primary) + sh_total;
+    ctx->stats.stack_entries = ctx->stack_entries;
+    ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
+    ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
+    return PULSE_OK;
+}
+
+int pulse_trace_noalloc_max_pixels(uint64_t *out) {
+    if (!out) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_trace_noalloc_max_pixels: null out");
+    *out = kNoAllocMaxPixels;
     return PULSE_OK;
 }
 
