@@ -5,6 +5,7 @@
 #include "pulse_oracle.hpp"
 
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstdlib>
 #include <thread>
@@ -147,6 +148,32 @@ int oracle_trace_rays_f64(oracle_scene *sc, const pulse_ray *rays, double *tuv, 
         ids[2 * i] = r.instance_index;
         ids[2 * i + 1] = r.triangle_index;
     });
+    return 0;
+}
+
+// One given candidate per ray: the reference's triangle test of triangle
+// slot ids[2i+1] of instance ids[2i] (object space, record t starting at
+// 1e30, utilities.wgsl:430-460) and the world t through the hit point
+// (318-320).  hits[i].t = 1e30 if the triangle is not hit.
+int oracle_intersect_candidates_f32(oracle_scene *sc, const pulse_ray *rays, const uint32_t *ids, pulse_hit *hits,
+                                    uint32_t n) {
+    const Scene &s = S(sc);
+    Counters c;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t inst = ids[2 * i], tri = ids[2 * i + 1];
+        hits[i] = pulse_hit{kTFar, 0u, 0u, 0.0f, 0.0f};
+        if (inst >= s.n_instances || tri >= s.instances[inst].triangle_count) return -1;
+        const pulse_mesh_instance &mi = s.instances[inst];
+        Ray<float> ro;
+        V3<float> o = load3<float>(rays[i].origin);
+        ro.origin = transform_position<float>(mi.world_object, o);
+        ro.dir = normalize(transform_direction<float>(mi.world_object, load3<float>(rays[i].dir)));
+        ro.t = kTFar;
+        ray_triangle_intersect(s, ro, tri, inst, c);
+        if (ro.t == kTFar) continue;
+        V3<float> hw = transform_position<float>(mi.object_world, ro.origin + ro.t * ro.dir);
+        hits[i] = pulse_hit{length(hw - o), inst, tri, ro.u, ro.v};
+    }
     return 0;
 }
 
@@ -302,67 +329,110 @@ int oracle_render_pair(oracle_scene *sc, const pulse_camera *cam, const pulse_pa
 // included; use a larger h).
 // Returns 0; grad_out[i] = dL/dparam[ids[i]].  loss_out (optional) = L at
 // the unperturbed parameters.
-int oracle_grad_fd_ex(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
-                      const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
-                      double *grad_out, double *loss_out, int threads, int frozen_dirs) {
+// Rows [row0, row1) only: the loss is the mean over those rows' pixels (the
+// target holds the whole image); pixels whose target is NaN are left out of
+// the loss and of its pixel count.  eval32 = 1: the perturbed renders are
+// evaluated in f32 along the same decisions, with the half vectors frozen as
+// the f32 sampling drew them (the f32 estimator itself; FD noise of f32
+// arithmetic: use h of about 1e-3) instead of f64.  pixel_grad (optional, n_ids x W x
+// (row1 - row0) doubles): each pixel's share of each derivative.
+int oracle_grad_fd_rows(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                        const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
+                        double *grad_out, double *loss_out, int threads, int frozen_dirs, uint32_t row0,
+                        uint32_t row1, int eval32, double *pixel_grad) {
     OwnedScene *os = reinterpret_cast<OwnedScene *>(sc);
     Scene &s = os->s;
-    const uint32_t W = cam->width, H = cam->height, npix = W * H, spp = std::max(prm->spp, 1u);
-    // record the f32 decisions of every path once
+    const uint32_t W = cam->width, spp = std::max(prm->spp, 1u);
+    row1 = std::min(row1, cam->height);
+    if (row0 >= row1) return -1;
+    const uint32_t H = row1 - row0, npix = W * H;
+    // record the f32 decisions of every path once (index: row - row0)
     std::vector<std::vector<uint64_t>> logs(static_cast<size_t>(npix) * spp);
     std::vector<Counters> pc(std::max(1, threads));
-    parallel_for(H, threads, [&](uint32_t y, int tid) {
+    parallel_for(H, threads, [&](uint32_t yy, int tid) {
+        const uint32_t y = row0 + yy;
         for (uint32_t x = 0; x < W; ++x)
             for (uint32_t k = 0; k < spp; ++k) {
                 DecisionTape rec;
                 rec.mode = DecisionTape::RECORD;
-                rec.log = &logs[(static_cast<size_t>(y) * W + x) * spp + k];
+                rec.log = &logs[(static_cast<size_t>(yy) * W + x) * spp + k];
                 trace_path<float>(s, *cam, *prm, x, y, prm->accumulation_count + k, pc[tid], rec);
             }
     });
     std::vector<double> p(params, params + 7 * static_cast<size_t>(n_materials));
     std::vector<std::vector<double>> dirs(frozen_dirs == 1 ? logs.size() : 0);
     DecisionTape::DirMode dmode = frozen_dirs == 1 ? DecisionTape::RECORD_DIRS : DecisionTape::DIRS_OFF;
-    auto loss = [&](const std::vector<double> &pp) {
+    uint64_t used = 0;
+    for (uint32_t yy = 0; yy < H; ++yy)
+        for (uint32_t x = 0; x < W; ++x) used += std::isnan(target[4 * (static_cast<size_t>(row0 + yy) * W + x)]) ? 0 : 1;
+    if (used == 0) return -1;
+    // one pixel's mean over its samples at precision R
+    auto pixel = [&](auto zero, uint32_t yy, uint32_t x, int tid) {
+        using R = decltype(zero);
+        V3<R> I{R(0), R(0), R(0)};
+        for (uint32_t k = 0; k < spp; ++k) {
+            DecisionTape rep;
+            rep.mode = frozen_dirs == 2 ? DecisionTape::OFF : DecisionTape::REPLAY;
+            rep.probe = false;
+            rep.log = &logs[(static_cast<size_t>(yy) * W + x) * spp + k];
+            if (frozen_dirs == 1) {
+                rep.dir_mode = dmode;
+                rep.dirs = &dirs[(static_cast<size_t>(yy) * W + x) * spp + k];
+            }
+            V3<R> c = trace_path<R>(s, *cam, *prm, x, row0 + yy, prm->accumulation_count + k, pc[tid], rep);
+            I = accumulate(I, c, k);
+        }
+        return V3<double>{static_cast<double>(I.x), static_cast<double>(I.y), static_cast<double>(I.z)};
+    };
+    std::vector<double> terms(static_cast<size_t>(npix), 0.0);  // per-pixel loss terms of the last call
+    auto loss = [&](const std::vector<double> &pp, bool f32) {
         s.mat_override = pp.data();
         std::vector<double> part(H, 0.0);
-        parallel_for(H, threads, [&](uint32_t y, int tid) {
+        parallel_for(H, threads, [&](uint32_t yy, int tid) {
+            const uint32_t y = row0 + yy;
             double acc = 0.0;
             for (uint32_t x = 0; x < W; ++x) {
-                V3<double> I{0, 0, 0};
-                for (uint32_t k = 0; k < spp; ++k) {
-                    DecisionTape rep;
-                    rep.mode = frozen_dirs == 2 ? DecisionTape::OFF : DecisionTape::REPLAY;
-                    rep.probe = false;
-                    rep.log = &logs[(static_cast<size_t>(y) * W + x) * spp + k];
-                    if (frozen_dirs == 1) {
-                        rep.dir_mode = dmode;
-                        rep.dirs = &dirs[(static_cast<size_t>(y) * W + x) * spp + k];
-                    }
-                    V3<double> c = trace_path<double>(s, *cam, *prm, x, y, prm->accumulation_count + k, pc[tid], rep);
-                    I = accumulate(I, c, k);
-                }
                 const float *t = target + 4 * (static_cast<size_t>(y) * W + x);
+                terms[static_cast<size_t>(yy) * W + x] = 0.0;
+                if (std::isnan(t[0])) continue;
+                V3<double> I = f32 ? pixel(0.0f, yy, x, tid) : pixel(0.0, yy, x, tid);
                 double dx = I.x - t[0], dy = I.y - t[1], dz = I.z - t[2];
-                acc += dx * dx + dy * dy + dz * dz;
+                double e = dx * dx + dy * dy + dz * dz;
+                terms[static_cast<size_t>(yy) * W + x] = e;
+                acc += e;
             }
-            part[y] = acc;
+            part[yy] = acc;
         });
         s.mat_override = nullptr;
         double tot = 0.0;
         for (double v : part) tot += v;
-        return tot / (3.0 * npix);
+        return tot / (3.0 * static_cast<double>(used));
     };
-    double l0 = loss(p);  // also records the frozen directions
+    // the unperturbed loss, which also records the frozen directions, in
+    // the precision of the perturbed evaluations
+    double l0 = loss(p, eval32 != 0);
     if (loss_out) *loss_out = l0;
     if (frozen_dirs == 1) dmode = DecisionTape::FREEZE_DIRS;
     for (uint32_t i = 0; i < n_ids; ++i) {
         std::vector<double> pp = p, pm = p;
         pp[ids[i]] += h;
         pm[ids[i]] -= h;
-        grad_out[i] = (loss(pp) - loss(pm)) / (2.0 * h);
+        const double lp = loss(pp, eval32 != 0);
+        std::vector<double> tp = terms;
+        const double lm = loss(pm, eval32 != 0);
+        grad_out[i] = (lp - lm) / (2.0 * h);
+        if (pixel_grad)
+            for (size_t k = 0; k < terms.size(); ++k)
+                pixel_grad[i * terms.size() + k] = (tp[k] - terms[k]) / (2.0 * h * 3.0 * static_cast<double>(used));
     }
     return 0;
+}
+
+int oracle_grad_fd_ex(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,
+                      const double *params, uint32_t n_materials, const uint32_t *ids, uint32_t n_ids, double h,
+                      double *grad_out, double *loss_out, int threads, int frozen_dirs) {
+    return oracle_grad_fd_rows(sc, cam, prm, target, params, n_materials, ids, n_ids, h, grad_out, loss_out, threads,
+                               frozen_dirs, 0, cam->height, 0, nullptr);
 }
 
 int oracle_grad_fd(oracle_scene *sc, const pulse_camera *cam, const pulse_params *prm, const float *target,

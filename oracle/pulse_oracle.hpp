@@ -706,7 +706,7 @@ inline MaterialR<R> load_material(const pulse_material &m) {
 template <class R>
 inline MaterialR<R> scene_material(const Scene &s, uint32_t idx) {
     MaterialR<R> m = load_material<R>(s.materials[idx]);
-    if (std::is_same<R, double>::value && s.mat_override) {
+    if (s.mat_override) {
         const double *p = s.mat_override + 7 * static_cast<size_t>(idx);
         m.base = {static_cast<R>(p[0]), static_cast<R>(p[1]), static_cast<R>(p[2])};
         m.roughness = static_cast<R>(p[3]);

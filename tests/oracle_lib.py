@@ -47,12 +47,15 @@ def lib() -> C.CDLL:
             ("oracle_trace_rays_f32", C.c_int, [_P, _P, _P, _U32, C.c_int, C.c_int, _P]),
             ("oracle_trace_rays_f64", C.c_int, [_P, _P, _P, _P, _U32, C.c_int]),
             ("oracle_shadow_rays_f32", C.c_int, [_P, _P, _P, _U32, C.c_int]),
+            ("oracle_intersect_candidates_f32", C.c_int, [_P, _P, _P, _P, _U32]),
             ("oracle_render_f32", C.c_double, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, C.c_int, _P, _U32, _U32]),
             ("oracle_render_pair", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, C.c_int, C.c_int, _P]),
             ("oracle_grad_fd", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
                                          C.c_double, _P, _P, C.c_int]),
             ("oracle_grad_fd_ex", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
                                             C.c_double, _P, _P, C.c_int, C.c_int]),
+            ("oracle_grad_fd_rows", C.c_int, [_P, C.POINTER(N.Camera), C.POINTER(N.Params), _P, _P, _U32, _P, _U32,
+                                              C.c_double, _P, _P, C.c_int, C.c_int, _U32, _U32, C.c_int, _P]),
             ("oracle_strength_pick_array", _U32, [_P, _U32, C.c_float]),
             ("oracle_render_gi_f32", C.c_double, [_P, C.POINTER(N.Params), _P, _P, _P, _P, _U32, _U32, _P, C.c_int,
                                                   _P]),
@@ -113,6 +116,16 @@ class OracleScene:
                                     cnt.ctypes.data)
         return hits, parse_counters(cnt)
 
+    def intersect_candidates(self, rays: np.ndarray, inst: np.ndarray, tri: np.ndarray):
+        """The reference's triangle test of one given (instance, triangle
+        slot) per ray and the world t of its hit (t = 1e30: not hit)."""
+        rays = np.ascontiguousarray(rays, dtype=N.RAY_DTYPE)
+        ids = np.ascontiguousarray(np.stack([inst, tri], 1), dtype=np.uint32)
+        hits = np.zeros(len(rays), N.HIT_DTYPE)
+        assert lib().oracle_intersect_candidates_f32(self.h, rays.ctypes.data, ids.ctypes.data, hits.ctypes.data,
+                                                     len(rays)) == 0
+        return hits
+
     def trace_rays_f64(self, rays: np.ndarray, threads=8):
         rays = np.ascontiguousarray(rays, dtype=N.RAY_DTYPE)
         tuv = np.zeros((len(rays), 3), np.float64)
@@ -144,20 +157,28 @@ class OracleScene:
                                    threads, None)
         return out
 
-    def grad_fd(self, camera, params, target, mat_params, ids, h=1e-6, threads=8, frozen=False):
+    def grad_fd(self, camera, params, target, mat_params, ids, h=1e-6, threads=8, frozen=False, rows=None,
+                eval32=False, per_pixel=False):
         """dL/dparam[ids] by f64 central differences along the f32 decisions;
         frozen=True also holds the sampled specular half vectors fixed (the
         detached roughness derivative); frozen="free" replays nothing (f64
-        renders with free decisions: the derivative of the estimator itself)."""
+        renders with free decisions: the derivative of the estimator itself).
+        rows=(r0, r1): the loss over those rows only; pixels whose target is
+        NaN are left out.  eval32: perturbed renders in f32 (use h ~ 1e-3).
+        per_pixel: also return each pixel's share, (len(ids), rows, width)."""
         target = np.ascontiguousarray(target, np.float32)
         mp = np.ascontiguousarray(mat_params, np.float64)
         ids = np.ascontiguousarray(ids, np.uint32)
         out = np.zeros(len(ids), np.float64)
         loss = C.c_double()
-        lib().oracle_grad_fd_ex(self.h, C.byref(camera), C.byref(params), target.ctypes.data, mp.ctypes.data,
-                                len(mp) // 7, ids.ctypes.data, len(ids), h, out.ctypes.data, C.byref(loss), threads,
-                                2 if frozen == "free" else int(bool(frozen)))
-        return out, loss.value
+        r0, r1 = (0, camera.height) if rows is None else rows
+        px = np.zeros((len(ids), r1 - r0, camera.width), np.float64) if per_pixel else None
+        rc = lib().oracle_grad_fd_rows(self.h, C.byref(camera), C.byref(params), target.ctypes.data, mp.ctypes.data,
+                                       len(mp) // 7, ids.ctypes.data, len(ids), h, out.ctypes.data, C.byref(loss),
+                                       threads, 2 if frozen == "free" else int(bool(frozen)), r0, r1, int(eval32),
+                                       px.ctypes.data if per_pixel else None)
+        assert rc == 0
+        return (out, loss.value, px) if per_pixel else (out, loss.value)
 
     def render_pair(self, camera, params, replay=True, threads=8):
         a32 = np.zeros((camera.height, camera.width, 4), np.float32)

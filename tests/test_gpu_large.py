@@ -2,10 +2,15 @@
 
 Forward: 256 displaced subdivision-6 icosphere BLAS (21M unique
 triangles) under 1024 instances with 256 emissive ones, 3840x2160, 6
-bounces with NEE, progressive accumulation over frames.  With the
-reference's traversal (loop 1) the accumulated rows equal the oracle's
-progressive render of the same rows bit for bit; with the 4-wide BVHs
-(loop 2) they differ only in pixels whose path met a t-tie.
+bounces with NEE, progressive accumulation to 64 samples per pixel: 64
+frames with the reference's traversal (loop 1), one 64-sample call with the
+4-wide BVHs (loop 2; equal to 64 frames by the running mean's definition).
+Both equal the oracle's 64-frame progressive render of sampled rows bit for
+bit.
+
+Per-instance parameters on the same geometry (one material per instance):
+the albedo gradient against central differences (h = 1e-3) of the GPU loss
+on 16 parameters, relative error <= 1e-2.
 
 Adjoint: one material per instance (1024 instances, 1025 materials x 7
 parameters: the gradient buffer exceeds the LDS and goes to device
@@ -26,7 +31,8 @@ import pytest
 from oracle_lib import OracleScene, same_bits
 from pulse_amd import _native as N
 from pulse_amd import procedural as P
-from pulse_amd.path_tracer import Device, PulseAdjoint, PulsePathTracerNode, make_params, material_params
+from pulse_amd.path_tracer import (Device, PulseAdjoint, PulsePathTracerNode, apply_material_params, make_params,
+                                   material_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -50,15 +56,19 @@ def test_large_progressive_rows(dev, large):
     cam = camf(3840, 2160)
     dev.upload_scene(scene)
     rows = [0, 517, 1080, 1081, 1702, 2159]
-    frames = 3
+    frames = 64
     images = {}
     for loop in (1, 2):
         dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
         node = PulsePathTracerNode(dev, cam.width, cam.height)
-        for f in range(frames):
-            node.run(cam, make_params(accumulation_count=f, max_depth=6, mode=N.MODE_GGX_NEE))
+        if loop == 1:
+            for f in range(frames):
+                node.run(cam, make_params(accumulation_count=f, max_depth=6, mode=N.MODE_GGX_NEE))
+        else:
+            node.run(cam, make_params(accumulation_count=0, spp=frames, max_depth=6, mode=N.MODE_GGX_NEE))
         assert dev.stats().error_flags == 0
         images[loop] = node.readback()
+        print(f"loop {loop}: {frames} samples rendered", flush=True)
     dev.set_option(N.OPT_TRAVERSAL_LOOP, 2)
     orc = OracleScene(scene)
     acc = np.zeros((cam.height, cam.width, 4), np.float32)
@@ -67,10 +77,60 @@ def test_large_progressive_rows(dev, large):
             orc.render(cam, make_params(accumulation_count=f, max_depth=6, mode=N.MODE_GGX_NEE), accum=acc,
                        threads=16, rows=(r, r + 1))
     o = acc[rows]
-    assert np.isfinite(o[..., :3]).all() and o[..., :3].mean() > 0.0
+    # the reference's estimator is unbounded (GGX weights, Q6) and rand_f can
+    # reach 1.0000005 (Q1): a few of the 64-sample means are not finite
+    fin = np.isfinite(o[..., :3]).all(axis=-1)
+    assert fin.mean() >= 0.999 and o[fin][:, :3].mean() > 0.0
     assert same_bits(images[1][rows], o).all()
     px = ~same_bits(images[2][rows], o).all(axis=-1)
-    assert px.sum() <= max(1, px.size // 1000), f"{px.sum()} of {px.size} pixels differ"
+    assert not px.any(), f"{px.sum()} of {px.size} pixels differ"
+
+
+@pytest.fixture(scope="module")
+def large_per_instance():
+    return P.icosphere_field(n_meshes=256, subdiv=6, n_instances=1024, n_emissive=256, box=110.0, seed=5,
+                             threads=16, roughness=(0.3, 1.0), per_instance_materials=True)
+
+
+def test_large_per_instance_albedo_vs_fd(dev, large_per_instance):
+    scene, camf = large_per_instance
+    mats = scene.buffers["materials"]
+    nm = len(mats)
+    assert nm == 1025
+    cam = camf(192, 128)
+    prm = make_params(spp=4, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6, flags=N.FLAG_HORIZON_CLAMP)
+    dev.upload_scene(scene)
+    adj = PulseAdjoint(dev, cam.width, cam.height, nm)
+    start = material_params(mats)
+    rng = np.random.default_rng(7)
+    hidden = start.copy()
+    hidden[:, 0:3] = np.clip(start[:, 0:3] + rng.uniform(-0.3, 0.3, (nm, 3)), 0.02, 0.98)
+    adj.set_materials(apply_material_params(mats, hidden))
+    adj.set_target(adj.forward(cam, make_params(spp=16, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6,
+                                                flags=N.FLAG_HORIZON_CLAMP)))
+    adj.set_materials(mats)
+    adj.forward(cam, prm)
+    g = adj.backward(cam, prm)
+    scale = np.abs(g[:1024, 0:3]).max()
+    cand = [7 * m + j for m in range(1024) for j in (0, 1, 2) if abs(g[m, j]) > 0.02 * scale]
+    assert len(cand) >= 16
+    ids = rng.choice(cand, 16, replace=False)
+    h = 1e-3
+    fd = []
+    for pid in ids:
+        vals = []
+        for sgn in (1, -1):
+            p = start.copy().reshape(-1)
+            p[pid] += sgn * h
+            adj.set_materials(apply_material_params(mats, p.reshape(nm, 7)))
+            adj.forward(cam, prm)
+            vals.append(adj.loss())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    adj.set_materials(mats)
+    fd = np.array(fd)
+    rel = np.abs(g.reshape(-1)[ids] - fd) / np.maximum(np.abs(fd), 1e-2 * scale)
+    print("per-instance ids", ids.tolist(), "rel", np.round(rel, 6).tolist(), flush=True)
+    assert (rel <= 1e-2).all()
 
 
 @pytest.fixture(scope="module")

@@ -180,3 +180,22 @@ def test_linear_cdf_scan_renders_identically(field):
     o.set_linear_scan(True)
     a1, _, _ = o.render(cam, p)
     assert same_bits(a0, a1).all()
+
+
+def test_candidate_reintersection_reproduces_closest_hits(field):
+    """The single-candidate test (the reference's triangle test of one given
+    instance/triangle and its world-t conversion) reproduces every closest
+    hit's t, u, v bit for bit: the tool the 4-wide traversal's tie checks
+    use to re-intersect both sides of a difference."""
+    scene, _ = field
+    o = OracleScene(scene)
+    rays = random_rays(4000, np.random.default_rng(21), (0, 0, 0), 16.0)
+    h, _ = o.trace_rays(rays)
+    hit = h["t"] < 1e30
+    r = o.intersect_candidates(rays[hit], h["instance_index"][hit], h["triangle_index"][hit])
+    for k in ("t", "u", "v"):
+        assert same_bits(r[k], h[k][hit]).all()
+    # a candidate the ray misses reports no hit
+    other = (h["triangle_index"][hit] + 7) % 320
+    m = o.intersect_candidates(rays[hit], h["instance_index"][hit], other)
+    assert (m["t"] == np.float32(1e30)).mean() > 0.9
