@@ -274,6 +274,7 @@ def pmc_traffic(max_depth: int, loop: int) -> dict | None:
         per_kind.setdefault("extend_primary", {})[counter] = ext[0]
         per_kind.setdefault("extend_bounce", {})[counter] = sum(ext[1:]) / max(len(ext) - 1, 1)
         per_kind.setdefault("shadow", {})[counter] = sum(sh) / len(sh)
+        per_kind.setdefault("extend", {})[counter] = sum(ext) / len(ext)
     out = {"scale": {k: round(v, 3) for k, v in scale.items()}}
     for kind, v in per_kind.items():
         out[kind] = (v["FETCH_SIZE"] * scale["FETCH_SIZE"] + v["WRITE_SIZE"] * scale["WRITE_SIZE"]) * 1024.0
@@ -435,6 +436,8 @@ def main() -> int:
     ap.add_argument("--period", type=int, default=None, help="persistent: instance changes run at least every n steps")
     ap.add_argument("--pair-waves", type=int, default=None,
                     help="persistent: waves per CU of each of two concurrent launches (0: half)")
+    ap.add_argument("--pair-waves-shadow", type=int, default=None,
+                    help="persistent: waves per CU of the shadow launch of a concurrent pair (0: --pair-waves)")
     ap.add_argument("--lds", type=int, default=None, help="persistent: traversal stack entries kept in LDS")
     ap.add_argument("--tps", type=int, default=None, help="persistent: leaf triangles tested per step (1 or 2)")
     ap.add_argument("--concurrent", type=int, default=None, choices=(0, 1),
@@ -486,6 +489,8 @@ def main() -> int:
         dev.set_option(N.OPT_ADVANCE_PERIOD, args.period)
     if args.pair_waves is not None:
         dev.set_option(N.OPT_PAIR_WAVES, args.pair_waves)
+    if args.pair_waves_shadow is not None:
+        dev.set_option(N.OPT_PAIR_WAVES_SHADOW, args.pair_waves_shadow)
     if args.lds is not None:
         dev.set_option(N.OPT_LDS_STACK_ENTRIES, args.lds)
     if args.tps is not None:
@@ -606,32 +611,41 @@ def main() -> int:
             bpr = {"primary": bytes_per_ray(work["primary"], B_RAY_EXTEND),
                    "bounce": bytes_per_ray(work["bounce"], B_RAY_EXTEND),
                    "shadow": bytes_per_ray(work["shadow"], B_RAY_SHADOW)}
-        # dominant kernel by total stage time
+        # the two traversal kernels over all their launches: the extension
+        # kernel (primary and bounce rays: one kernel, the largest share of
+        # GPU time in the kernel trace) and the shadow kernel.  roofline
+        # names the extension kernel; each launch's own duration (hipEvents
+        # on the stream it runs on) includes the share of the GPU a launch on
+        # the other stream takes while both run
         persistent = result["persistent_waves_per_cu"] > 0 and args.loop != 0
         ext_name = "k_trace_queue<..., false>" if persistent else "k_extend"
         sh_name = "k_trace_queue<..., true>" if persistent else "k_shadow"
-        cands = {
-            f"{ext_name} (extension rays, depth >= 1)": (
-                agg["ms_extend_bounce"], agg["extension_rays"] - agg["primary_rays"], bpr["bounce"],
-                agg["extend_launches"] - args.steps, "extend_bounce"),
-            f"{sh_name} (shadow rays)": (agg["ms_shadow"], agg["shadow_rays"], bpr["shadow"], agg["shadow_launches"],
-                                         "shadow"),
-            f"{ext_name} (primary rays)": (agg["ms_primary_extend"], agg["primary_rays"], bpr["primary"], args.steps,
-                                           "extend_primary"),
+        bounce_rays = agg["extension_rays"] - agg["primary_rays"]
+        kernels = {
+            "extend": (f"{ext_name} (extension rays, primary and bounce)",
+                       agg["ms_primary_extend"] + agg["ms_extend_bounce"],
+                       agg["primary_rays"] * bpr["primary"] + bounce_rays * bpr["bounce"], agg["extend_launches"]),
+            "shadow": (f"{sh_name} (shadow rays)", agg["ms_shadow"], agg["shadow_rays"] * bpr["shadow"],
+                       agg["shadow_launches"]),
         }
-        name, (ms, rays, b, launches, kind) = max(cands.items(), key=lambda kv: kv[1][0])
-        achieved = rays * b / (ms / 1e3) / 1e9
-        traffic = None
         pmc = None
         if not args.no_pmc and persistent:
             pmc = pmc_traffic(args.max_depth, args.loop)
-            if pmc is not None:
-                traffic = round(pmc[kind])
-        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
-                              "avg_launch_ms": round(ms / max(launches, 1), 4),
-                              "algorithmic_bytes_per_launch": round(rays * b / max(launches, 1)),
-                              "bytes_per_ray": {k: round(v, 1) for k, v in bpr.items()}}
+        per_kernel = {}
+        for kind, (kname, ms, nbytes, launches) in kernels.items():
+            rate = nbytes / (ms / 1e3) / 1e9
+            per_kernel[kind] = {"kernel": kname, "achieved": round(rate, 1), "frac": round(rate / HBM_PEAK_GBS, 4),
+                                "launches_per_step": round(launches / args.steps, 2),
+                                "avg_launch_ms": round(ms / max(launches, 1), 4),
+                                "algorithmic_bytes_per_launch": round(nbytes / max(launches, 1)),
+                                "traffic": round(pmc[kind]) if pmc is not None else None}
+        dom = per_kernel["extend"]
+        result["roofline"] = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": dom["frac"], "traffic": dom["traffic"], "kernel": dom["kernel"],
+                              "avg_launch_ms": dom["avg_launch_ms"],
+                              "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
+                              "bytes_per_ray": {k: round(v, 1) for k, v in bpr.items()},
+                              "per_kernel": per_kernel}
         if pmc is not None:
             result["roofline"]["hbm_bytes_per_launch"] = {k: round(v) for k, v in pmc.items() if k != "scale"}
             result["roofline"]["pmc_scale"] = pmc["scale"]

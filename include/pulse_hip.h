@@ -438,6 +438,10 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                    /* once (0, the default: half of                  */
                                    /* PULSE_OPT_PERSISTENT_WAVES), so that both start */
                                    /* at once; launches that run alone take all      */
+#define PULSE_OPT_PAIR_WAVES_SHADOW 14 /* persistent, concurrent launches: waves */
+                                       /* per CU of the shadow launch of a pair   */
+                                       /* (0: PULSE_OPT_PAIR_WAVES).  A launch    */
+                                       /* takes at most PULSE_OPT_PERSISTENT_WAVES */
 #define PULSE_OPT_INJECT_ERROR 12  /* write-only, for tests of error reporting: ORs */
                                    /* the value (PULSE_ERR_* bits) into the device */
                                    /* error word, as a kernel that met the          */

@@ -55,6 +55,7 @@ OPT_ADVANCE_PERIOD = 9
 OPT_CONCURRENT_SHADOW = 10
 OPT_PAIR_WAVES = 11
 OPT_INJECT_ERROR = 12
+OPT_PAIR_WAVES_SHADOW = 14
 ERR_STACK_OVERFLOW = 1
 ERR_ITERATION_CAP = 2
 

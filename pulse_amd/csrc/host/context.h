@@ -156,6 +156,8 @@ struct pulse_ctx {
     int concurrent_shadow = 1;
     int pair_waves = 0;  // persistent waves per CU of each of two launches that run at once (0: half)
     int eff_pair_waves() const { return pair_waves ? pair_waves : std::max(1, persistent / 2); }
+    int pair_waves_shadow = 0;  // the shadow launch's share of a concurrent pair (0: eff_pair_waves)
+    int eff_pair_waves_shadow() const { return pair_waves_shadow ? pair_waves_shadow : eff_pair_waves(); }
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev[pt::kMaxTraces] = {}, join_ev[pt::kMaxTraces] = {};
     hipEvent_t ev2[6 * pt::kMaxTraces + 4] = {};

@@ -130,8 +130,11 @@ void launch_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ, ui
 // could need).
 // Persistent grid: `waves` per CU (0: PULSE_OPT_PERSISTENT_WAVES), no more
 // waves than the queue's capacity needs.
+// A launch never takes more than PULSE_OPT_PERSISTENT_WAVES per CU: the
+// spill area holds that grid's columns (spill_part_words).
 uint32_t persistent_grid(pulse_ctx *ctx, uint32_t cap, int waves = 0) {
-    uint32_t g = static_cast<uint32_t>(ctx->num_cus) * static_cast<uint32_t>(waves > 0 ? waves : ctx->persistent);
+    const int w = waves > 0 ? std::min(waves, ctx->persistent) : ctx->persistent;
+    uint32_t g = static_cast<uint32_t>(ctx->num_cus) * static_cast<uint32_t>(w);
     return std::max<uint32_t>(1, std::min<uint32_t>(g, (cap + kTraceBlock - 1) / kTraceBlock));
 }
 // LDS entries of the persistent kernels' stacks: the compiled size at most
@@ -1216,8 +1219,9 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
                 // takes the paired share of waves; the last shadow launch,
                 // alone, all of them
                 const int pair_waves = k + 1 < T ? ctx->eff_pair_waves() : 0;
+                const int pair_waves_sh = k + 1 < T ? ctx->eff_pair_waves_shadow() : 0;
                 if (timing) HIP_TRY(hipEventRecord(E[1 + 4 * T + k], B));
-                if ((rc = launch_queue(ctx, wb, cap, true, 0, k, B, 1, pair_waves)) != PULSE_OK) return rc;
+                if ((rc = launch_queue(ctx, wb, cap, true, 0, k, B, 1, pair_waves_sh)) != PULSE_OK) return rc;
                 if (timing) HIP_TRY(hipEventRecord(E[1 + 5 * T + k], B));
                 HIP_TRY(hipEventRecord(ctx->join_ev[k], B));
                 if (k + 1 < T) {
@@ -1470,7 +1474,8 @@ int pulse_trace_gbuffer(pulse_ctx *ctx, const pulse_gbuffer *gb, uint32_t width,
                 HIP_TRY(hipStreamWaitEvent(B, ctx->fork_ev[k + 1], 0));
                 // k = -1: NEE at the G-buffer points
                 const int pair_waves = k + 1 < T ? ctx->eff_pair_waves() : 0;
-                if ((rc = launch_queue(ctx, wb, npix, true, 0, k < 0 ? kMaxTraces : k, B, 1, pair_waves)) != PULSE_OK)
+                const int pair_waves_sh = k + 1 < T ? ctx->eff_pair_waves_shadow() : 0;
+                if ((rc = launch_queue(ctx, wb, npix, true, 0, k < 0 ? kMaxTraces : k, B, 1, pair_waves_sh)) != PULSE_OK)
                     return rc;
                 HIP_TRY(hipEventRecord(ctx->join_ev[k + 1], B));
                 if (k + 1 < T && (rc = launch_queue(ctx, wb, npix, false, q, k + 1, A, 0, pair_waves)) != PULSE_OK)
@@ -1638,6 +1643,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value != 0 && value != 1) PULSE_FAIL(PULSE_E_INVALID_ARG, "concurrent shadow must be 0 or 1");
             ctx->concurrent_shadow = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_PAIR_WAVES_SHADOW:
+            if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
+            ctx->pair_waves_shadow = static_cast<int>(value);
+            return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES:
             if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
             ctx->pair_waves = static_cast<int>(value);
@@ -1667,6 +1676,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_ADVANCE_PERIOD: *value = ctx->eff_period(); return PULSE_OK;
         case PULSE_OPT_CONCURRENT_SHADOW: *value = ctx->concurrent_shadow; return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES: *value = ctx->eff_pair_waves(); return PULSE_OK;
+        case PULSE_OPT_PAIR_WAVES_SHADOW: *value = ctx->eff_pair_waves_shadow(); return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;
