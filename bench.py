@@ -18,6 +18,7 @@ Prints one JSON line.  --gpus N > 1 is not supported on this build.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import csv
 import json
 import os
@@ -156,9 +157,14 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
     prm = make_params(spp=16, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3, flags=clamp)
     adj.forward(cam, prm)
     adj.forward_backward(cam, prm)  # warm-up
+    # both legs end with the image on the device: the forward is pulse_trace
+    # and a synchronisation, the other pulse_forward_adjoint (forward, loss,
+    # adjoint) and the gradient's copy to the host
+    lib = N.lib()
     t0 = time.perf_counter()
     for _ in range(iters):
-        adj.forward(cam, prm)
+        N.check(lib.pulse_trace(dev.handle, C.byref(cam), C.byref(prm), adj.image.ptr))
+        dev.sync()
     fwd_ms = (time.perf_counter() - t0) * 1e3 / iters
     t0 = time.perf_counter()
     for _ in range(iters):
@@ -169,6 +175,100 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
             "ratio": round(fa_ms / fwd_ms, 3), "loss": loss, "grad_norm": float(np.linalg.norm(g)),
             "config": "512x512, 16 spp, max_depth 3, GGX + Lambert NEE, no RR, horizon clamp, "
                       "65 materials x 7 parameters"}
+
+
+def bench_adjoint_views(iters: int = 3) -> dict:
+    """Batched adjoint on the large workload's geometry: 256 displaced
+    subdivision-6 icosphere BLAS (21M unique triangles), 1024 instances (256
+    emissive), one material per instance (4096 per-instance albedo and
+    roughness parameters, 7175 gradient entries with the emission: the
+    gradient lives in device memory); 8 views of 512x512 on a circle around
+    the field, 4 spp, 6 bounces, GGX + Lambert NEE, no RR, horizon clamp.
+    Targets rendered at 16 spp from hidden parameters (albedo and roughness
+    +-0.3).  Timed (best of `iters`, each ending in a synchronisation): the 8
+    forward renders; pulse_adjoint_views over the 8 views in one launch (the
+    default: each wave's lanes summed per material before their atomic);
+    the same with per-lane atomics; 8 pulse_adjoint calls accumulating into
+    one gradient."""
+    from pulse_amd import _native as N
+    from pulse_amd import procedural as P
+    from pulse_amd.path_tracer import Device, MultiViewAdjoint, apply_material_params, make_params, material_params
+
+    try:
+        threads = max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        threads = max(1, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    scene, _ = P.icosphere_field(n_meshes=256, subdiv=6, n_instances=1024, n_emissive=256, box=110.0, seed=5,
+                                 threads=min(threads, 16), roughness=(0.3, 1.0), per_instance_materials=True)
+    build_s = time.perf_counter() - t0
+    dev = Device(0)
+    lib = N.lib()
+    try:
+        dev.upload_scene(scene)
+        mats = scene.buffers["materials"]
+        nm = len(mats)
+        cams = [P.make_camera(512, 512, eye=(140.0 * np.cos(a), 25.0, 140.0 * np.sin(a)), target=(0.0, 0.0, 0.0))
+                for a in np.linspace(0.0, 2.0 * np.pi, 8, endpoint=False)]
+        mv = MultiViewAdjoint(dev, cams, nm)
+        rng = np.random.default_rng(5)
+        start = material_params(mats)
+        hidden = start.copy()
+        hidden[:, 0:3] = np.clip(start[:, 0:3] + rng.uniform(-0.3, 0.3, (nm, 3)), 0.02, 0.98)
+        hidden[:, 3] = np.clip(start[:, 3] + rng.uniform(-0.3, 0.3, nm), 0.05, 1.0)
+        clamp = N.FLAG_HORIZON_CLAMP
+        mh = np.ascontiguousarray(apply_material_params(mats, hidden), N.MATERIAL_DTYPE)
+        N.check(lib.pulse_scene_update_materials(dev.handle, mh.ctypes.data, nm))
+        mv.set_targets(mv.forward(make_params(spp=16, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6,
+                                              flags=clamp)))
+        m0 = np.ascontiguousarray(mats, N.MATERIAL_DTYPE)
+        N.check(lib.pulse_scene_update_materials(dev.handle, m0.ctypes.data, nm))
+        prm = make_params(spp=4, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6, flags=clamp)
+        acc = N.Params.from_buffer_copy(prm)
+        acc.flags |= N.FLAG_ADJOINT_ACCUMULATE
+        mv.forward(prm)
+        singles = [(N.AdjointView * 1)(mv.views[v]) for v in range(len(cams))]
+
+        def best(fn):
+            fn()
+            dev.sync()
+            t = 1e30
+            for _ in range(iters):
+                t1 = time.perf_counter()
+                fn()
+                dev.sync()
+                t = min(t, time.perf_counter() - t1)
+            return t * 1e3
+
+        def forward():
+            for cam, img in zip(mv.cameras, mv.images):
+                N.check(lib.pulse_trace(dev.handle, C.byref(cam), C.byref(prm), img.ptr))
+
+        def batched():
+            N.check(lib.pulse_adjoint_views(dev.handle, mv.views, len(cams), C.byref(prm), mv.grad.ptr))
+
+        def sequential():
+            for v, one in enumerate(singles):
+                N.check(lib.pulse_adjoint_views(dev.handle, one, 1, C.byref(prm if v == 0 else acc), mv.grad.ptr))
+
+        fwd_ms = best(forward)
+        batched_ms = best(batched)
+        g = mv.grad.copy_to(np.zeros(nm * 7, np.float32))
+        seq_ms = best(sequential)
+        dev.set_option(N.OPT_ADJOINT_REDUCE_ROUNDS, 0)
+        per_lane_ms = best(batched)
+        dev.set_option(N.OPT_ADJOINT_REDUCE_ROUNDS, -1)
+        losses = mv.losses()
+    finally:
+        dev.close()
+    return {"forward_8_views_ms": round(fwd_ms, 3), "adjoint_batched_ms": round(batched_ms, 3),
+            "adjoint_sequential_ms": round(seq_ms, 3), "adjoint_batched_per_lane_atomics_ms": round(per_lane_ms, 3),
+            "batched_speedup": round(seq_ms / batched_ms, 3),
+            "forward_plus_batched_adjoint_ratio": round((fwd_ms + batched_ms) / fwd_ms, 3),
+            "loss_sum": float(sum(losses)), "grad_norm": float(np.linalg.norm(g)),
+            "scene_build_s": round(build_s, 1),
+            "config": "256 BLAS x 1024 instances (21M unique triangles), 1025 materials x 7 parameters, "
+                      "8 views 512x512, 4 spp, max_depth 6, GGX + Lambert NEE, no RR, horizon clamp"}
 
 
 def bench_instance_update(dev, scene, full_upload_s: float) -> dict:
@@ -586,6 +686,14 @@ def main() -> int:
         result["reference_order_loop1"] = loop1
     if census is not None:
         result["work_census"] = census
+    # The legs below each open a context of their own; close this one first.
+    # HIP serves a process's streams from a few hardware queues per device
+    # (GPU_MAX_HW_QUEUES, 4), and with this context still open the next
+    # one's two streams were placed on one queue, which runs its concurrent
+    # extension and shadow launches one after the other (the 512x512 forward
+    # of the adjoint leg: 35.5 ms alone, 47.5 ms beside an open context).
+    del node
+    dev.close()
     if not args.no_subdiv5 and args.subdiv == 6:
         result["subdiv5"] = bench_subdiv5(args)
         print(f"bench: subdivision-5 meshes {result['subdiv5']['value']} incoherent Mrays/s", file=sys.stderr,
@@ -594,6 +702,9 @@ def main() -> int:
     if not args.no_adjoint:
         result["fwd_adjoint"] = bench_adjoint(scene, camf)
         print(f"bench: forward+adjoint {result['fwd_adjoint']['ratio']}x forward", file=sys.stderr, flush=True)
+        result["adjoint_views"] = bench_adjoint_views()
+        print(f"bench: batched adjoint over 8 views {result['adjoint_views']['batched_speedup']}x sequential",
+              file=sys.stderr, flush=True)
 
     if not args.no_batch:
         result["batched"] = bench_batch()
@@ -657,7 +768,6 @@ def main() -> int:
                                                    "frac": round(inc_rate / HBM_PEAK_GBS, 4),
                                                    "ms_per_step": round(agg["ms_incoherent"] / args.steps, 3)}
     print(json.dumps(result))
-    dev.close()
     return 0
 
 

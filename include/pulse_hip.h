@@ -442,6 +442,13 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                        /* per CU of the shadow launch of a pair   */
                                        /* (0: PULSE_OPT_PAIR_WAVES).  A launch    */
                                        /* takes at most PULSE_OPT_PERSISTENT_WAVES */
+#define PULSE_OPT_ADJOINT_REDUCE_ROUNDS 15 /* adjoint: per add of gradient terms, up to */
+                                           /* n materials of a wave (in lane order)   */
+                                           /* are summed over the wave and added by   */
+                                           /* one atomic each; the other lanes add    */
+                                           /* their own (0: per-lane atomics; 0..16). */
+                                           /* -1 (default): 0 while the gradient fits */
+                                           /* in LDS, 2 when it is in device memory   */
 #define PULSE_OPT_INJECT_ERROR 12  /* write-only, for tests of error reporting: ORs */
                                    /* the value (PULSE_ERR_* bits) into the device */
                                    /* error word, as a kernel that met the          */
@@ -492,7 +499,11 @@ typedef struct {
     uint32_t regroup_every;           /* compact the extension queue every n bounces */
                                       /* (in between, rays stay in place)           */
     uint32_t tile_width;              /* primary-ray tile: width x (64 / width)      */
-    uint32_t use_graph;               /* 1: replay one captured hipGraph per frame   */
+    uint32_t use_graph;               /* 1: replay one captured hipGraph per frame,  */
+                                      /* its launches in sequence; 0 (default):     */
+                                      /* direct launches, the shadow launches of    */
+                                      /* bounce k beside bounce k+1's extension     */
+                                      /* launch (PULSE_OPT_CONCURRENT_SHADOW)        */
     uint32_t lds_stack_entries;       /* persistent single loop: stack entries per   */
                                       /* lane in LDS, the rest in device memory     */
                                       /* (0: all in LDS)                            */
@@ -548,6 +559,28 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
  * for bit.  Requires metallic == 0 for every material. */
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params, const float *image,
                   const float *target, float *grad, float *replay_rgba);
+
+/* One view of a multi-view adjoint: the camera, the forward image rendered by
+ * pulse_trace with it (device, width*height*4), the target (device, same
+ * size) and an optional replay buffer as in pulse_adjoint. */
+typedef struct pulse_adjoint_view {
+    pulse_camera camera;
+    const float *image;
+    const float *target;
+    float *replay_rgba;
+} pulse_adjoint_view;
+
+/* The batched adjoint (the reference has no differentiable form; its
+ * per-view camera is the View uniform of path_tracer.wgsl:45-75): dL/dparams
+ * of L = sum over the views of each view's loss, for n_views >= 1 views of
+ * the bound scene that may differ in camera and resolution, in one launch
+ * per PULSE_ADJOINT_MAX_VIEWS views (each launch adds to `grad`).  Same
+ * requirements on params as pulse_adjoint; `grad` is overwritten, or added
+ * to with PULSE_FLAG_ADJOINT_ACCUMULATE.  Equals the sum of pulse_adjoint
+ * over the views up to the order of the f32 additions. */
+#define PULSE_ADJOINT_MAX_VIEWS 16
+int pulse_adjoint_views(pulse_ctx *ctx, const pulse_adjoint_view *views, uint32_t n_views, const pulse_params *params,
+                        float *grad);
 
 /* Forward + loss + adjoint in one call: renders into image_out (device,
  * width*height*4), writes the loss (host) and the gradient (device). */

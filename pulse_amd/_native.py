@@ -56,6 +56,8 @@ OPT_CONCURRENT_SHADOW = 10
 OPT_PAIR_WAVES = 11
 OPT_INJECT_ERROR = 12
 OPT_PAIR_WAVES_SHADOW = 14
+OPT_ADJOINT_REDUCE_ROUNDS = 15
+ADJOINT_MAX_VIEWS = 16
 ERR_STACK_OVERFLOW = 1
 ERR_ITERATION_CAP = 2
 
@@ -102,6 +104,10 @@ class Camera(C.Structure):
         ("width", C.c_uint32),
         ("height", C.c_uint32),
     ]
+
+
+class AdjointView(C.Structure):
+    _fields_ = [("camera", Camera), ("image", C.c_void_p), ("target", C.c_void_p), ("replay_rgba", C.c_void_p)]
 
 
 class Params(C.Structure):
@@ -265,6 +271,7 @@ SIGNATURES = [
     ("pulse_present", _I, [_P, _P, _U32, _U32, _P, _U32, _U32, _I]),
     ("pulse_trace_gbuffer", _I, [_P, C.POINTER(GBuffer), _U32, _U32, C.POINTER(Params), _P]),
     ("pulse_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, _P]),
+    ("pulse_adjoint_views", _I, [_P, C.POINTER(AdjointView), _U32, C.POINTER(Params), _P]),
     ("pulse_forward_adjoint", _I, [_P, C.POINTER(Camera), C.POINTER(Params), _P, _P, _P, C.POINTER(C.c_double)]),
     ("pulse_get_option", _I, [_P, _I, C.POINTER(C.c_int64)]),
     ("pulse_scene_create", _I, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),

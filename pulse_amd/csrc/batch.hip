@@ -125,6 +125,11 @@ BatchArgs batch_args(const pulse_batch *b) {
     a.chunk = b->cfg.chunk_rays;
     a.persistent = b->cfg.persistent_waves_per_cu;
     a.refill = b->cfg.refill_lanes;
+    // the context's instance-change gathering, tuned per traversal loop as
+    // for single views (PULSE_OPT_GATHER_LANES, PULSE_OPT_ADVANCE_PERIOD)
+    const bool wide = b->cfg.traversal_loop == 2;
+    a.gather = static_cast<uint32_t>(b->ctx->gather ? b->ctx->gather : (wide ? 64 : 1));
+    a.period = static_cast<uint32_t>(b->ctx->advance_period ? b->ctx->advance_period : (wide ? 2 : 1));
     a.stack_cap = static_cast<int>(b->stack_entries);
     a.spill = b->spill.as<uint32_t>();
     a.regroup = b->cfg.regroup_every;
@@ -179,6 +184,13 @@ uint32_t trace_grid(const pulse_batch *b) {
         grid = std::min<uint32_t>(grid, static_cast<uint32_t>(b->ctx->num_cus) * b->cfg.persistent_waves_per_cu);
     return std::max<uint32_t>(grid, 1);
 }
+// A persistent launch that runs beside another (the shadow launch of bounce
+// k and the extension launch of bounce k+1) takes half the waves, so that
+// both are resident at once; a launch running alone takes them all.
+uint32_t pair_grid(const pulse_batch *b) {
+    const uint32_t half = std::max<uint32_t>(b->cfg.persistent_waves_per_cu / 2, 1);
+    return std::max<uint32_t>(std::min<uint32_t>(b->max_chunks, static_cast<uint32_t>(b->ctx->num_cus) * half), 1);
+}
 // Spilled stack words of one persistent launch (0: the whole stack in LDS
 // or not persistent); concurrent launches take one part each.
 size_t spill_part_words(const pulse_batch *b) {
@@ -186,29 +198,34 @@ size_t spill_part_words(const pulse_batch *b) {
     return static_cast<size_t>(trace_grid(b)) * 64u * (b->stack_entries - static_cast<uint32_t>(refill_lds(b)));
 }
 template <int LDS, int LOOP>
-void launch_refill_loop(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
+void launch_refill_loop(const BatchArgs &a, bool shadow, int q, int k, hipStream_t st, uint32_t grid) {
     if (shadow)
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, true, LOOP>), dim3(trace_grid(b)), dim3(64), 0, st, a, q, k);
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, true, LOOP>), dim3(grid), dim3(64), 0, st, a,
+                           static_cast<const BatchView *>(a.views), q, k);
     else
-        hipLaunchKernelGGL((k_b_trace_refill<LDS, false, LOOP>), dim3(trace_grid(b)), dim3(64), 0, st, a, q, k);
+        hipLaunchKernelGGL((k_b_trace_refill<LDS, false, LOOP>), dim3(grid), dim3(64), 0, st, a,
+                           static_cast<const BatchView *>(a.views), q, k);
 }
 template <int LDS>
-void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
-    if (b->cfg.traversal_loop == 2) launch_refill_loop<LDS, 2>(b, a, shadow, q, k, st);
-    else launch_refill_loop<LDS, 1>(b, a, shadow, q, k, st);
+void launch_refill(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st, uint32_t grid) {
+    if (b->cfg.traversal_loop == 2) launch_refill_loop<LDS, 2>(a, shadow, q, k, st, grid);
+    else launch_refill_loop<LDS, 1>(a, shadow, q, k, st, grid);
 }
 
-void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st) {
+// paired: the launch runs beside another one (persistent launches only).
+void dispatch_trace(pulse_batch *b, const BatchArgs &a, bool shadow, int q, int k, hipStream_t st,
+                    bool paired = false) {
     if (refill_path(b)) {
+        const uint32_t g = paired ? pair_grid(b) : trace_grid(b);
         switch (refill_lds(b)) {
-            case 8: launch_refill<8>(b, a, shadow, q, k, st); break;
-            case 16: launch_refill<16>(b, a, shadow, q, k, st); break;
-            case 24: launch_refill<24>(b, a, shadow, q, k, st); break;
-            case 32: launch_refill<32>(b, a, shadow, q, k, st); break;
-            case 40: launch_refill<40>(b, a, shadow, q, k, st); break;
-            case 48: launch_refill<48>(b, a, shadow, q, k, st); break;
-            case 64: launch_refill<64>(b, a, shadow, q, k, st); break;
-            default: launch_refill<128>(b, a, shadow, q, k, st); break;
+            case 8: launch_refill<8>(b, a, shadow, q, k, st, g); break;
+            case 16: launch_refill<16>(b, a, shadow, q, k, st, g); break;
+            case 24: launch_refill<24>(b, a, shadow, q, k, st, g); break;
+            case 32: launch_refill<32>(b, a, shadow, q, k, st, g); break;
+            case 40: launch_refill<40>(b, a, shadow, q, k, st, g); break;
+            case 48: launch_refill<48>(b, a, shadow, q, k, st, g); break;
+            case 64: launch_refill<64>(b, a, shadow, q, k, st, g); break;
+            default: launch_refill<128>(b, a, shadow, q, k, st, g); break;
         }
         return;
     }
@@ -265,10 +282,11 @@ void enqueue_frame(pulse_batch *b) {
                 hipLaunchKernelGGL(k_b_shade, dim3(shade_grid), dim3(256), 0, st, a, q, static_cast<int>(k));
                 (void)hipEventRecord(b->ctx->fork_ev[k], st);
                 (void)hipStreamWaitEvent(st2, b->ctx->fork_ev[k], 0);
-                dispatch_trace(b, a2, true, q, static_cast<int>(k), st2);
+                const bool pair = k + 1 < b->traces;  // the last shadow launch runs alone
+                dispatch_trace(b, a2, true, q, static_cast<int>(k), st2, pair);
                 (void)hipEventRecord(b->ctx->join_ev[k], st2);
                 q ^= 1;
-                if (k + 1 < b->traces) dispatch_trace(b, a, false, q, static_cast<int>(k + 1), st);
+                if (pair) dispatch_trace(b, a, false, q, static_cast<int>(k + 1), st, true);
                 (void)hipStreamWaitEvent(st, b->ctx->join_ev[k], 0);
             }
         } else {
@@ -309,7 +327,11 @@ int apply_config(pulse_batch *b, const pulse_batch_config &c) {
     b->stack_entries = resolve_stack(b, c);
     drop_graph(b);
     if ((rc = layout_views(b)) != PULSE_OK) return rc;  // (drains the stream)
-    b->concurrent = refill_path(b) && b->ctx->concurrent_shadow != 0;
+    // concurrent shadow launches with direct launches only: a graph replay
+    // runs the two streams' branches one after the other (measured on the
+    // 32-view workload: replay with the fork/join 88.9 ms, replay in sequence
+    // 87.6 ms, direct launches with concurrent pairs 83.0 ms)
+    b->concurrent = refill_path(b) && b->ctx->concurrent_shadow != 0 && c.use_graph == 0;
     if (b->concurrent && (rc = ensure_stream2(b->ctx, false)) != PULSE_OK) return rc;
     if (const size_t words = spill_part_words(b)) {
         if ((rc = b->spill.ensure(sizeof(uint32_t) * words * (b->concurrent ? 2 : 1))) != PULSE_OK) return rc;
@@ -372,7 +394,7 @@ int pulse_batch_default_config(pulse_batch_config *out) {
     out->traversal_loop = 2;
     out->regroup_every = 1;
     out->tile_width = 8;
-    out->use_graph = 1;
+    out->use_graph = 0;
     out->lds_stack_entries = 24;
     out->refill_lanes = 16;
     return PULSE_OK;
@@ -582,7 +604,7 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
         for (uint32_t s : kStacks)
             if (s > pick_stack(std::max(b->stack_need, b->wide_need), 1) && stack_opts.size() < 2) stack_opts.push_back(s);
         for (int pass = 0; pass < 2 && rc == PULSE_OK; ++pass) {
-            for (int knob = 0; knob < 8 && rc == PULSE_OK; ++knob) {
+            for (int knob = 0; knob < 9 && rc == PULSE_OK; ++knob) {
                 std::vector<pulse_batch_config> cands;
                 auto add = [&](auto set) {
                     pulse_batch_config c = best;
@@ -618,6 +640,9 @@ int pulse_batch_autotune(pulse_batch *b, uint32_t reps, pulse_batch_config *best
                         break;
                     case 5:
                         for (uint32_t t : {8u, 16u, 64u}) add([&](pulse_batch_config &c) { c.tile_width = t; });
+                        break;
+                    case 8:
+                        for (uint32_t g : {0u, 1u}) add([&](pulse_batch_config &c) { c.use_graph = g; });
                         break;
                 }
                 for (const auto &c : cands) {

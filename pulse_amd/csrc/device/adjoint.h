@@ -21,7 +21,9 @@
 // for the adjoint (their lobe probability would depend on the albedo).
 //
 // Per block the gradient is accumulated in LDS (float atomics), then added to
-// the global gradient with one atomic per entry.
+// the global gradient with one atomic per entry; a gradient too large for the
+// LDS is added to device memory directly.  Either way each add first sums the
+// lanes of a wave that hold the same material (grad_add_wave).
 #pragma once
 
 #include <type_traits>
@@ -35,21 +37,37 @@ constexpr int kAdjMaxVertices = 4;      // path vertices of the default kernels
 constexpr int kAdjMaxVerticesDeep = 8;  // the deep kernels (fixed-direction roughness only)
 constexpr int kGradPerMaterial = 7;     // base.rgb, roughness, emissive.rgb
 
-struct AdjArgs {
+// One launch covers up to kAdjMaxViews views of the bound scene; the loss is
+// the sum of the views' losses, so the gradient is the sum of theirs.  Block
+// b of view v replays the view's pixels [128 (b - block0), 128 (b - block0 +
+// 1)) in row order (measured faster than 8x8 squares per wave; a wave's 64
+// neighbouring primary rays still mostly hit the same instances, which the
+// wave sums of the gradient adds exploit).
+constexpr int kAdjMaxViews = 16;
+constexpr uint32_t kAdjBlockPixels = 128;
+
+struct AdjView {
     pulse_camera cam;
+    const float4 *image;   // forward image (running mean over spp samples)
+    const float4 *target;  // target image
+    float4 *replay;        // optional: the replayed running mean (verification)
+    float grad_scale;      // dL/dI = grad_scale * (I - T): 2 / (3 npix) of this view
+    uint32_t npix;         // width * height
+    uint32_t block0;       // the view's first block
+};
+
+struct AdjArgs {
+    AdjView views[kAdjMaxViews];
+    uint32_t n_views;
     uint32_t max_depth;
     uint32_t nee_ggx;
     uint32_t spp;
-    uint32_t npix;
-    const float4 *image;   // forward image (running mean over spp samples)
-    const float4 *target;  // target image
     float *grad;           // n_materials * 7, accumulated into
-    float4 *replay;        // optional: the replayed running mean (verification)
-    float grad_scale;      // dL/dI = grad_scale * (I - T)
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
     uint32_t horizon_clamp;
     uint32_t detached;     // roughness derivative at fixed sampled directions
+    uint32_t reduce_rounds;  // distinct keys per wave summed before the atomic (0: per-lane atomics)
 };
 
 template <class S>
@@ -87,8 +105,46 @@ __device__ __forceinline__ void intersect_known(const DevScene &sc, const V3T<S>
     tw = length(sub(hit_world, o));
 }
 
-__device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
-    if (v != 0.0f) atomicAdd(buf + idx, v);
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+// Sum of x over the lanes with `on` set, in every lane (butterfly over the
+// 64 lanes; every lane of the wave must be active).
+__device__ __forceinline__ float wave_sum(float x, bool on) {
+    float s = on ? x : 0.0f;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    return s;
+}
+
+// Adds v[j] to buf[key + j], j < N, for every lane whose key is not kNoKey.
+// First up to `rounds` distinct keys of the wave (in lane order: the lowest
+// pending lane's key each round) are summed over the lanes holding them and
+// added by one atomic per entry; the lanes still pending then add their own
+// values.  Floating-point sums over lanes in another order than per-lane
+// atomics would take: the result equals those up to rounding.  Every lane of
+// the wave must reach the call (the kernels call it from uniform control
+// flow; dead lanes pass kNoKey).
+template <int N>
+__device__ __forceinline__ void grad_add_wave(float *buf, uint32_t key, const float (&v)[N], uint32_t rounds) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint64_t pend = __ballot(key != kNoKey);
+        if (pend == 0) return;
+        const int leader = __builtin_ctzll(pend);
+        const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(key), leader));
+        const bool mine = key == k;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float s = wave_sum(v[j], mine);
+            if (lane == static_cast<uint32_t>(leader) && s != 0.0f) atomicAdd(buf + k + j, s);
+        }
+        if (mine) key = kNoKey;
+    }
+    if (key != kNoKey) {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (v[j] != 0.0f) atomicAdd(buf + key + j, v[j]);
+    }
 }
 
 // DETACHED = true: the roughness derivative holds every sampled direction
@@ -102,6 +158,7 @@ __device__ __forceinline__ void grad_add(float *buf, uint32_t idx, float v) {
 // MAXV: path vertices the per-vertex arrays hold (max_depth at most MAXV).
 template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1, int MAXV = kAdjMaxVertices>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
+    static_assert(BLOCK == kAdjBlockPixels, "a block replays 128 pixels");
     using S = typename std::conditional<DETACHED, float, Dual<MAXV>>::type;
     __shared__ uint32_t smem[STACK * BLOCK];
     extern __shared__ float sgrad[];
@@ -111,22 +168,36 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
         __syncthreads();
     }
     float *gbuf = a.use_lds ? sgrad : a.grad;
-    const uint32_t pixel = blockIdx.x * BLOCK + threadIdx.x;
-    if (pixel < a.npix) {
+    // the block's view and pixels; lanes past the view's last pixel run the
+    // sample loop without vertices (the gradient adds need the whole wave)
+    uint32_t vi = 0;
+    for (uint32_t i = 1; i < a.n_views; ++i)
+        if (blockIdx.x >= a.views[i].block0) vi = i;
+    const AdjView &vw = a.views[vi];
+    const uint32_t p = (blockIdx.x - vw.block0) * kAdjBlockPixels + threadIdx.x;
+    const uint32_t W = vw.cam.width;
+    const bool live = p < vw.npix;
+    const uint32_t pixel = live ? p : 0u;
+    const uint32_t x = pixel % W, y = pixel / W;
+    const uint32_t depth_end = live ? a.max_depth : 0u;
+    const uint32_t rounds = a.reduce_rounds;
+    {
         LdsStack stk{smem + threadIdx.x, BLOCK, STACK};
-        const uint32_t W = a.cam.width;
-        const uint32_t x = pixel % W, y = pixel / W;
-        float4 im = a.image[pixel], tg = a.target[pixel];
+        float4 im = make_float4(0.0f, 0.0f, 0.0f, 0.0f), tg = im;
+        if (live) {
+            im = vw.image[pixel];
+            tg = vw.target[pixel];
+        }
         // dL/dC of one sample: dL/dI * dI/dC, I the mean of spp samples
         const float inv_spp = 1.0f / static_cast<float>(a.spp);
-        const float gx = a.grad_scale * (im.x - tg.x) * inv_spp;
-        const float gy = a.grad_scale * (im.y - tg.y) * inv_spp;
-        const float gz = a.grad_scale * (im.z - tg.z) * inv_spp;
+        const float gx = vw.grad_scale * (im.x - tg.x) * inv_spp;
+        const float gy = vw.grad_scale * (im.y - tg.y) * inv_spp;
+        const float gz = vw.grad_scale * (im.z - tg.z) * inv_spp;
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         for (uint32_t s = 0; s < a.spp; ++s) {
             uint32_t rng = pixel_seed(pixel, s);
             V3 o0, d0;
-            primary_ray(a.cam, x, y, o0, d0);
+            primary_ray(vw.cam, x, y, o0, d0);
             V3T<S> o = lift<S>(o0), d = lift<S>(d0);
             V3T<S> T = lift<S>(mk(1.0f, 1.0f, 1.0f));
             V3T<S> C = lift<S>(mk(0.0f, 0.0f, 0.0f));
@@ -137,7 +208,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             uint32_t mat[MAXV], lmat[MAXV];
             bool vis[MAXV];
             int nv = 0;
-            for (uint32_t depth = 0; depth < a.max_depth; ++depth) {
+            for (uint32_t depth = 0; depth < depth_end; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
                 uint32_t inst = 0, tri = 0;
                 trace_any<LOOP, false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
@@ -213,51 +284,88 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 }
                 nv += 1;
             }
+            // Both sweeps visit the vertex slots the wave's longest path has,
+            // in every lane, so the wave-wide adds see the whole wave; a lane
+            // whose path is shorter reads nothing and adds nothing there.
+            int kmax = 0;
+#pragma unroll
+            for (int k = 0; k < MAXV; ++k)
+                if (__ballot(nv > k) != 0) kmax = k + 1;
             if constexpr (!DETACHED) {
                 // roughness: tangents of the replayed colour
-                for (int k = 0; k < nv; ++k) {
-                    float dr = gx * C.x.d[k] + gy * C.y.d[k] + gz * C.z.d[k];
-                    if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
+                for (int k = 0; k < kmax; ++k) {
+                    uint32_t key = kNoKey;
+                    float v1[1] = {0.0f};
+                    if (k < nv) {
+                        const float dr = gx * C.x.d[k] + gy * C.y.d[k] + gz * C.z.d[k];
+                        if (dr == dr) {
+                            key = mat[k] * kGradPerMaterial + 3;
+                            v1[0] = dr;
+                        }
+                    }
+                    grad_add_wave(gbuf, key, v1, rounds);
                 }
             }
             // albedo and emission: reverse sweep over the vertices
             float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
-            for (int k = nv - 1; k >= 0; --k) {
-                const V3 b = beta[k];
-                float ax = gx * b.x * (dDdb[k].x + Sx * dwdb[k]);
-                float ay = gy * b.y * (dDdb[k].y + Sy * dwdb[k]);
-                float az = gz * b.z * (dDdb[k].z + Sz * dwdb[k]);
-                if (ax == ax && ay == ay && az == az) {
-                    grad_add(gbuf, mat[k] * kGradPerMaterial + 0, ax);
-                    grad_add(gbuf, mat[k] * kGradPerMaterial + 1, ay);
-                    grad_add(gbuf, mat[k] * kGradPerMaterial + 2, az);
-                }
-                if (vis[k]) {
-                    float ex = gx * b.x * dDdLe[k].x, ey = gy * b.y * dDdLe[k].y, ez = gz * b.z * dDdLe[k].z;
-                    if (ex == ex && ey == ey && ez == ez) {
-                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 4, ex);
-                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 5, ey);
-                        grad_add(gbuf, lmat[k] * kGradPerMaterial + 6, ez);
+            for (int k = kmax - 1; k >= 0; --k) {
+                uint32_t kb = kNoKey, ke = kNoKey, k0 = kNoKey;  // albedo+roughness, NEE emission, direct emission
+                float v4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, e3[3] = {0.0f, 0.0f, 0.0f}, d3[3] = {0.0f, 0.0f, 0.0f};
+                if (k < nv) {
+                    const V3 b = beta[k];
+                    const uint32_t m = mat[k];
+                    const float dwb = dwdb[k];
+                    const float ax = gx * b.x * (dDdb[k].x + Sx * dwb);
+                    const float ay = gy * b.y * (dDdb[k].y + Sy * dwb);
+                    const float az = gz * b.z * (dDdb[k].z + Sz * dwb);
+                    if (ax == ax && ay == ay && az == az) {
+                        kb = m * kGradPerMaterial;
+                        v4[0] = ax;
+                        v4[1] = ay;
+                        v4[2] = az;
                     }
+                    const V3 wk = wvert[k];
+                    if constexpr (DETACHED) {
+                        // roughness at fixed directions: NEE (GGX evaluator) and
+                        // the sampling weight w_k (d w = w d ln f)
+                        const float dl = dlogw[k];
+                        const float dr = gx * b.x * (dDdr[k].x + Sx * wk.x * dl) +
+                                         gy * b.y * (dDdr[k].y + Sy * wk.y * dl) +
+                                         gz * b.z * (dDdr[k].z + Sz * wk.z * dl);
+                        if (dr == dr) {
+                            kb = m * kGradPerMaterial;
+                            v4[3] = dr;
+                        }
+                    }
+                    if (vis[k]) {
+                        const float ex = gx * b.x * dDdLe[k].x, ey = gy * b.y * dDdLe[k].y, ez = gz * b.z * dDdLe[k].z;
+                        if (ex == ex && ey == ey && ez == ez) {
+                            ke = lmat[k] * kGradPerMaterial + 4;
+                            e3[0] = ex;
+                            e3[1] = ey;
+                            e3[2] = ez;
+                        }
+                    }
+                    if (k == 0) {
+                        // emission seen directly: dC/dLe of the first hit = 1
+                        const float ex = gx * b.x, ey = gy * b.y, ez = gz * b.z;
+                        if (ex == ex && ey == ey && ez == ez) {
+                            k0 = m * kGradPerMaterial + 4;
+                            d3[0] = ex;
+                            d3[1] = ey;
+                            d3[2] = ez;
+                        }
+                    }
+                    const V3 ck = cvert[k];
+                    Sx = ck.x + wk.x * Sx;
+                    Sy = ck.y + wk.y * Sy;
+                    Sz = ck.z + wk.z * Sz;
                 }
-                if constexpr (DETACHED) {
-                    // roughness at fixed directions: NEE (GGX evaluator) and
-                    // the sampling weight w_k (d w = w d ln f)
-                    float dr = gx * b.x * (dDdr[k].x + Sx * wvert[k].x * dlogw[k]) +
-                               gy * b.y * (dDdr[k].y + Sy * wvert[k].y * dlogw[k]) +
-                               gz * b.z * (dDdr[k].z + Sz * wvert[k].z * dlogw[k]);
-                    if (dr == dr) grad_add(gbuf, mat[k] * kGradPerMaterial + 3, dr);
-                }
-                if (k == 0) {
-                    grad_add(gbuf, mat[0] * kGradPerMaterial + 4, gx * b.x);
-                    grad_add(gbuf, mat[0] * kGradPerMaterial + 5, gy * b.y);
-                    grad_add(gbuf, mat[0] * kGradPerMaterial + 6, gz * b.z);
-                }
-                Sx = cvert[k].x + wvert[k].x * Sx;
-                Sy = cvert[k].y + wvert[k].y * Sy;
-                Sz = cvert[k].z + wvert[k].z * Sz;
+                grad_add_wave(gbuf, kb, v4, rounds);
+                grad_add_wave(gbuf, ke, e3, rounds);
+                if (k == 0) grad_add_wave(gbuf, k0, d3, rounds);
             }
-            if (a.replay) {
+            if (vw.replay) {
                 // running mean of path_tracer.wgsl:116-123 over the replayed samples
                 float w = 1.0f / (static_cast<float>(s) + 1.0f);
                 float omw = 1.0f - w;
@@ -265,7 +373,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 acc = make_float4(ox * omw + val(C.x) * w, oy * omw + val(C.y) * w, oz * omw + val(C.z) * w, 1.0f);
             }
         }
-        if (a.replay) a.replay[pixel] = acc;
+        if (vw.replay && live) vw.replay[pixel] = acc;
     }
     if (a.use_lds) {
         __syncthreads();

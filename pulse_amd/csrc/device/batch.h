@@ -70,6 +70,8 @@ struct BatchArgs {
     uint32_t chunk;        // rays per chunk, a multiple of 64
     uint32_t persistent;   // 0: one wave per chunk; else a fixed grid drains the queue
     uint32_t refill;       // persistent single-loop traversal: idle lanes that take new rays
+    uint32_t gather;       // persistent: lanes waiting for an instance change before the wave runs it
+    uint32_t period;       // persistent: instance changes run at least every `period` wave steps
     int stack_cap;         // traversal stack entries per lane
     uint32_t *spill;       // stack entries below the LDS part, one column per lane of the grid
     uint32_t regroup;      // compact the extension queue of index k when k % regroup == 0
@@ -237,8 +239,14 @@ __global__ void __launch_bounds__(64) k_b_trace(BatchArgs b, int q, int k) {
 // The top LDS entries of each lane's stack are in LDS, the rest in `spill`.
 // Each ray is traversed exactly as by k_b_trace (LOOP 1: the binary BVHs,
 // LOOP 2: the 4-wide ones).
+//
+// `views` is b.views passed again as a read-only, unaliased pointer: the
+// scene of the chunk being drained is read through it at every use (scalar
+// loads the compiler may repeat) instead of being held in registers across
+// the loop.
 template <int LDS, bool SHADOW, int LOOP = 1>
-__global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LOOP == 2 ? 6 : 1)))
+k_b_trace_refill(BatchArgs b, const BatchView *__restrict__ views, int q, int k) {
     __shared__ uint32_t smem[LDS * 64];
     const uint32_t ci = SHADOW ? b.max_traces + 1u + static_cast<uint32_t>(k) : static_cast<uint32_t>(k);
     const uint32_t wslot = 2u * static_cast<uint32_t>(k) + (SHADOW ? 1u : 0u);
@@ -254,9 +262,10 @@ __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k
     bool alive = false;
     uint32_t slot = 0, pixel = 0;
     // wave-uniform: the chunk being drained
-    uint32_t base = 0, next = 0, end = 0;
-    DevScene sc = b.views[0].sc;
+    uint32_t base = 0, next = 0, end = 0, vcur = 0;
+    uint32_t wave_step = 0;
     for (;;) {
+        const DevScene &sc = views[vcur].sc;
         const uint64_t idle = __ballot(!alive);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
         if (next < end && (n_idle >= b.refill || n_idle == 64u)) {
@@ -280,7 +289,8 @@ __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k
             }
             next = min(next + n_idle, end);
         }
-        if (__ballot(alive) == 0ull) {
+        const uint64_t act = __ballot(alive);
+        if (act == 0ull) {
             if (next < end) continue;
             uint32_t g = 0;
             if (lane == 0u) g = atomicAdd(&b.work[wslot], 1u);
@@ -288,14 +298,19 @@ __global__ void __launch_bounds__(64) k_b_trace_refill(BatchArgs b, int q, int k
             if (g >= cm.total) break;
             uint32_t start;
             const uint32_t v = chunk_view(cm, b.n_views, g, start);
-            sc = b.views[v].sc;
-            base = b.views[v].pix_base;
+            vcur = v;
+            base = views[v].pix_base;
             const uint32_t cnt = b.counts[v * b.cstride + ci];
             next = (g - start) * b.chunk;
             end = min(next + b.chunk, cnt);
             continue;
         }
-        if (alive && !tr.step(sc, stk)) {
+        // instance changes gathered as in the single-view persistent kernel
+        const uint64_t waiting = __ballot(alive && tr.want_advance);
+        wave_step += 1u;
+        const bool run_advance = waiting == act || static_cast<uint32_t>(__popcll(waiting)) >= b.gather ||
+                                 wave_step % b.period == 0u;
+        if (alive && !tr.step(sc, stk, run_advance)) {
             alive = false;
             if (SHADOW) {
                 if (!tr.occluded) b.state_c[base + pixel] = b.sh_c[slot];

@@ -158,6 +158,9 @@ struct pulse_ctx {
     int eff_pair_waves() const { return pair_waves ? pair_waves : std::max(1, persistent / 2); }
     int pair_waves_shadow = 0;  // the shadow launch's share of a concurrent pair (0: eff_pair_waves)
     int eff_pair_waves_shadow() const { return pair_waves_shadow ? pair_waves_shadow : eff_pair_waves(); }
+    // distinct materials per wave summed before their atomic; -1: none for a
+    // gradient in LDS, 2 for one in device memory (the measured best)
+    int adjoint_reduce_rounds = -1;
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev[pt::kMaxTraces] = {}, join_ev[pt::kMaxTraces] = {};
     hipEvent_t ev2[6 * pt::kMaxTraces + 4] = {};

@@ -825,9 +825,11 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
 
 namespace {
 constexpr int kAdjBlock = 128;
+// The view table travels in the kernel arguments (no allocation, nothing to
+// copy before the launch); HIP takes at most 4 KB of them.
+static_assert(sizeof(AdjArgs) + sizeof(DevScene) <= 4096, "adjoint kernel arguments exceed 4 KB");
 template <int STACK, int LOOP>
-void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
-    uint32_t grid = (a.npix + kAdjBlock - 1) / kAdjBlock;
+void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds, uint32_t grid) {
     if (a.max_depth > static_cast<uint32_t>(kAdjMaxVertices)) {
         // deep paths: fixed-direction roughness derivative, the 64-entry stack
         hipLaunchKernelGGL((k_adjoint<kAdjBlock, 64, true, LOOP, kAdjMaxVerticesDeep>), dim3(grid), dim3(kAdjBlock),
@@ -844,9 +846,87 @@ void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
 // The forward pass's traversal: the 4-wide BVHs for loop 2, the binary ones
 // (same results as loops 0 and 1) otherwise.
 template <int STACK>
-void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds) {
-    if (ctx->loop_variant == 2) launch_adjoint_loop<STACK, 2>(ctx, a, lds);
-    else launch_adjoint_loop<STACK, 1>(ctx, a, lds);
+void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds, uint32_t grid) {
+    if (ctx->loop_variant == 2) launch_adjoint_loop<STACK, 2>(ctx, a, lds, grid);
+    else launch_adjoint_loop<STACK, 1>(ctx, a, lds, grid);
+}
+
+// What the adjoint supports (params, scene, context); `who` names the entry
+// point in the message.
+int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
+    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "%s: no scene uploaded", who);
+    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
+    if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs PULSE_MODE_GGX_NEE", who);
+    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be 1..%d", who, kAdjMaxVerticesDeep);
+    if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d", who, kAdjMaxVertices);
+    if (prm->rr_start_depth < prm->max_depth)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: Russian roulette must be off (rr_start_depth >= max_depth)", who);
+    if (prm->accumulation_count != 0 || prm->spp == 0)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs accumulation_count 0 and spp >= 1", who);
+    if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: materials with metallic != 0", who);
+    if (prm->flags & PULSE_FLAG_LIGHT_POWER)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_LIGHT_POWER (the light pdf depends on emission)", who);
+    if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: BVH too deep for the adjoint stack", who);
+    return PULSE_OK;
+}
+
+// The adjoint over views[0..n): one launch per kAdjMaxViews views, every
+// launch adding to `grad` (zeroed first unless the params ask to accumulate).
+// The views have been checked (pointers, sizes).
+int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_view *views, uint32_t n, float *grad) {
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const uint32_t nent = ctx->cur->n_materials * kGradPerMaterial;
+    if (nent && !(prm->flags & PULSE_FLAG_ADJOINT_ACCUMULATE)) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
+    }
+    AdjArgs a{};
+    a.max_depth = prm->max_depth;
+    a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
+    a.spp = prm->spp;
+    a.grad = grad;
+    a.n_materials = ctx->cur->n_materials;
+    a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
+    a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
+    const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
+    a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
+    const size_t lds = a.use_lds ? lds_bytes : 0;
+    const int rounds = ctx->adjoint_reduce_rounds;
+    a.reduce_rounds = rounds >= 0 ? static_cast<uint32_t>(rounds) : (a.use_lds ? 0u : 2u);
+    for (uint32_t first = 0; first < n; first += kAdjMaxViews) {
+        const uint32_t cnt = std::min<uint32_t>(n - first, kAdjMaxViews);
+        uint64_t blocks = 0;
+        a.n_views = cnt;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const pulse_adjoint_view &v = views[first + i];
+            AdjView &w = a.views[i];
+            w.cam = v.camera;
+            w.image = reinterpret_cast<const float4 *>(v.image);
+            w.target = reinterpret_cast<const float4 *>(v.target);
+            w.replay = reinterpret_cast<float4 *>(v.replay_rgba);
+            const uint64_t npix = static_cast<uint64_t>(v.camera.width) * v.camera.height;
+            if (npix > 0xffffffffull) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: view of %llu pixels",
+                                                 static_cast<unsigned long long>(npix));
+            w.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
+            w.npix = static_cast<uint32_t>(npix);
+            w.block0 = static_cast<uint32_t>(blocks);
+            blocks += (npix + kAdjBlockPixels - 1) / kAdjBlockPixels;
+        }
+        if (blocks > 0x7fffffffu) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: %llu blocks in one launch",
+                                             static_cast<unsigned long long>(blocks));
+        const uint32_t grid = static_cast<uint32_t>(blocks);
+        switch (ctx->stack_entries) {
+            case 24: launch_adjoint<24>(ctx, a, lds, grid); break;
+            case 32: launch_adjoint<32>(ctx, a, lds, grid); break;
+            case 40: launch_adjoint<40>(ctx, a, lds, grid); break;
+            case 48: launch_adjoint<48>(ctx, a, lds, grid); break;
+            default: launch_adjoint<64>(ctx, a, lds, grid); break;
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    return PULSE_OK;
 }
 }  // namespace
 
@@ -1542,55 +1622,28 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *image,
                   const float *target, float *grad, float *replay_rgba) {
     if (!ctx || !cam || !prm || !image || !target || !grad) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: null argument");
-    if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "pulse_adjoint: no scene uploaded");
-    if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
-    if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs PULSE_MODE_GGX_NEE");
-    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: max_depth must be 1..%d", kAdjMaxVerticesDeep);
-    if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d",
-                   kAdjMaxVertices);
-    if (prm->rr_start_depth < prm->max_depth)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: Russian roulette must be off (rr_start_depth >= max_depth)");
-    if (prm->accumulation_count != 0 || prm->spp == 0)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: needs accumulation_count 0 and spp >= 1");
-    if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: materials with metallic != 0");
-    if (prm->flags & PULSE_FLAG_LIGHT_POWER)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: PULSE_FLAG_LIGHT_POWER (the light pdf depends on emission)");
-    if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_adjoint: BVH too deep for the adjoint stack");
-    int rc = set_device(ctx);
-    if (rc) return rc;
-    const uint32_t npix = cam->width * cam->height;
-    const uint32_t nent = ctx->cur->n_materials * kGradPerMaterial;
-    if (nent && !(prm->flags & PULSE_FLAG_ADJOINT_ACCUMULATE)) {
-        hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
+    if (int rc = adjoint_check(ctx, prm, "pulse_adjoint")) return rc;
+    if (cam->width == 0 || cam->height == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint: empty image");
+    pulse_adjoint_view v;
+    v.camera = *cam;
+    v.image = image;
+    v.target = target;
+    v.replay_rgba = replay_rgba;
+    return adjoint_run(ctx, prm, &v, 1, grad);
+}
+
+int pulse_adjoint_views(pulse_ctx *ctx, const pulse_adjoint_view *views, uint32_t n_views, const pulse_params *prm,
+                        float *grad) {
+    if (!ctx || !views || !prm || !grad) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint_views: null argument");
+    if (n_views == 0) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint_views: no views");
+    if (int rc = adjoint_check(ctx, prm, "pulse_adjoint_views")) return rc;
+    for (uint32_t i = 0; i < n_views; ++i) {
+        const pulse_adjoint_view &v = views[i];
+        if (!v.image || !v.target) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint_views: view %u: null image or target", i);
+        if (v.camera.width == 0 || v.camera.height == 0)
+            PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_adjoint_views: view %u: empty image", i);
     }
-    AdjArgs a;
-    a.cam = *cam;
-    a.max_depth = prm->max_depth;
-    a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
-    a.spp = prm->spp;
-    a.npix = npix;
-    a.image = reinterpret_cast<const float4 *>(image);
-    a.target = reinterpret_cast<const float4 *>(target);
-    a.grad = grad;
-    a.replay = reinterpret_cast<float4 *>(replay_rgba);
-    a.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
-    a.n_materials = ctx->cur->n_materials;
-    a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
-    a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
-    const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
-    a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
-    const size_t lds = a.use_lds ? lds_bytes : 0;
-    switch (ctx->stack_entries) {
-        case 24: launch_adjoint<24>(ctx, a, lds); break;
-        case 32: launch_adjoint<32>(ctx, a, lds); break;
-        case 40: launch_adjoint<40>(ctx, a, lds); break;
-        case 48: launch_adjoint<48>(ctx, a, lds); break;
-        default: launch_adjoint<64>(ctx, a, lds); break;
-    }
-    HIP_TRY(hipGetLastError());
-    return PULSE_OK;
+    return adjoint_run(ctx, prm, views, n_views, grad);
 }
 
 int pulse_forward_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *target,
@@ -1647,6 +1700,10 @@ int pulse_set_option(pulse_ctx *ctx, int option, int64_t value) {
             if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
             ctx->pair_waves_shadow = static_cast<int>(value);
             return PULSE_OK;
+        case PULSE_OPT_ADJOINT_REDUCE_ROUNDS:
+            if (value < -1 || value > 16) PULSE_FAIL(PULSE_E_INVALID_ARG, "adjoint reduce rounds must be -1..16");
+            ctx->adjoint_reduce_rounds = static_cast<int>(value);
+            return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES:
             if (value < 0 || value > 64) PULSE_FAIL(PULSE_E_INVALID_ARG, "pair waves must be 0..64");
             ctx->pair_waves = static_cast<int>(value);
@@ -1677,6 +1734,7 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_CONCURRENT_SHADOW: *value = ctx->concurrent_shadow; return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES: *value = ctx->eff_pair_waves(); return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES_SHADOW: *value = ctx->eff_pair_waves_shadow(); return PULSE_OK;
+        case PULSE_OPT_ADJOINT_REDUCE_ROUNDS: *value = ctx->adjoint_reduce_rounds; return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;

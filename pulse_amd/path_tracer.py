@@ -446,3 +446,61 @@ class PulseAdjoint:
         self.device.sync()
         g = self.grad.copy_to(np.zeros(self.n_materials * N.GRAD_PER_MATERIAL, np.float32))
         return loss.value, g.reshape(self.n_materials, N.GRAD_PER_MATERIAL)
+
+
+class MultiViewAdjoint:
+    """The batched differentiable form: several views of the bound scene
+    (cameras of any resolution), each rendered by pulse_trace, and the
+    gradient of the sum of their losses from one pulse_adjoint_views call
+    (one launch per N.ADJOINT_MAX_VIEWS views)."""
+
+    def __init__(self, device: Device, cameras, n_materials: int):
+        self.device, self.n_materials = device, n_materials
+        self.cameras = list(cameras)
+        self.images = [device.alloc(c.width * c.height * 16) for c in self.cameras]
+        self.targets = [device.alloc(c.width * c.height * 16) for c in self.cameras]
+        self.replays = None
+        self.grad = device.alloc(max(n_materials * N.GRAD_PER_MATERIAL * 4, 4))
+        self.views = (N.AdjointView * len(self.cameras))()
+        for v, (cam, img, tgt) in enumerate(zip(self.cameras, self.images, self.targets)):
+            self.views[v].camera = cam
+            self.views[v].image = img.ptr
+            self.views[v].target = tgt.ptr
+
+    def _shape(self, v: int):
+        return (self.cameras[v].height, self.cameras[v].width, 4)
+
+    def set_targets(self, targets) -> None:
+        for buf, t in zip(self.targets, targets):
+            buf.copy_from(np.ascontiguousarray(t, np.float32))
+
+    def forward(self, params: N.Params):
+        """Renders every view; returns the images."""
+        lib = N.lib()
+        for cam, img in zip(self.cameras, self.images):
+            N.check(lib.pulse_trace(self.device.handle, C.byref(cam), C.byref(params), img.ptr))
+        self.device.sync()
+        return [img.copy_to(np.zeros(self._shape(v), np.float32)) for v, img in enumerate(self.images)]
+
+    def losses(self):
+        out, res = C.c_double(), []
+        for cam, img, tgt in zip(self.cameras, self.images, self.targets):
+            N.check(N.lib().pulse_image_loss(self.device.handle, img.ptr, tgt.ptr, cam.width * cam.height,
+                                             C.byref(out)))
+            res.append(out.value)
+        return res
+
+    def backward(self, params: N.Params, want_replay: bool = False):
+        """dL/d(material parameters) of L = the sum of the views' losses."""
+        if want_replay and self.replays is None:
+            self.replays = [self.device.alloc(c.width * c.height * 16) for c in self.cameras]
+        for v in range(len(self.cameras)):
+            self.views[v].replay_rgba = self.replays[v].ptr if want_replay else None
+        N.check(N.lib().pulse_adjoint_views(self.device.handle, self.views, len(self.cameras), C.byref(params),
+                                            self.grad.ptr))
+        self.device.sync()
+        g = self.grad.copy_to(np.zeros(self.n_materials * N.GRAD_PER_MATERIAL, np.float32))
+        g = g.reshape(self.n_materials, N.GRAD_PER_MATERIAL)
+        if want_replay:
+            return g, [r.copy_to(np.zeros(self._shape(v), np.float32)) for v, r in enumerate(self.replays)]
+        return g

@@ -33,6 +33,7 @@ STRUCTS = {
     "PulseView": ("pulse_view", N.View),
     "PulseBatchConfig": ("pulse_batch_config", N.BatchConfig),
     "PulseGBuffer": ("pulse_gbuffer", N.GBuffer),
+    "PulseAdjointView": ("pulse_adjoint_view", N.AdjointView),
 }
 SCALARS = {"u32": (4, 4), "i32": (4, 4), "f32": (4, 4), "u64": (8, 8), "f64": (8, 8), "usize": (8, 8)}
 

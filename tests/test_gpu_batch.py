@@ -108,7 +108,8 @@ def test_graph_replays_accumulate_exactly(dev, views):
     samples; each intermediate frame is checked too."""
     prm = make_params(spp=2, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=1)
     b = Batch(dev, views, prm)
-    assert b.config().use_graph == 1
+    assert b.config().use_graph == 0  # the default: direct launches
+    b.set_config(use_graph=1)
     for frame in range(3):
         b.render(2 * frame)
         imgs = b.images()
@@ -176,9 +177,9 @@ def test_batch_rejects_bad_views(dev, views):
 def test_concurrent_shadow_launches_keep_images(dev, views, loop, use_graph):
     """PULSE_OPT_CONCURRENT_SHADOW (default 1): the shadow launches of bounce
     k on the context's second stream beside bounce k+1's extension launch.
-    pulse_trace and the batch (graph replay or direct launches) give the
-    same images as with every launch in sequence, over two progressive
-    frames."""
+    pulse_trace and the batch (direct launches, with the pairs; a graph
+    replay runs its launches in sequence) give the same images as with every
+    launch in sequence, over two progressive frames."""
     prm = make_params(spp=2, max_depth=4, mode=N.MODE_GGX_NEE, rr_start_depth=1)
     assert dev.get_option(N.OPT_CONCURRENT_SHADOW) == 1
     with pytest.raises(N.PulseError):

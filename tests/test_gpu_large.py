@@ -14,12 +14,14 @@ on 16 parameters, relative error <= 1e-2.
 
 Adjoint: one material per instance (1024 instances, 1025 materials x 7
 parameters: the gradient buffer exceeds the LDS and goes to device
-atomics), 6-bounce paths, 8 views.  Checked by identities that need no
-reference render: with target 0, Euler's identity for the emission
-(the image is linear in all emissions together, so sum_j Le_j dL/dLe_j =
-2 L) summed over the views, with each view's gradient added by
-PULSE_FLAG_ADJOINT_ACCUMULATE; and the accumulated gradient for nonzero
-targets equals the sum of the per-view gradients.  The deep-path adjoint
+atomics, summed over each wave per material first), 6-bounce paths, 8
+views in one batched call (pulse_adjoint_views).  Checked by identities
+that need no reference render: with target 0, Euler's identity for the
+emission (the image is linear in all emissions together, so
+sum_j Le_j dL/dLe_j = 2 L) summed over the views; and the batched gradient
+for nonzero targets equals the sum of the per-view gradients (pulse_adjoint
+per view), as does the per-view calls' accumulation with
+PULSE_FLAG_ADJOINT_ACCUMULATE.  The deep-path adjoint
 against the oracle's central differences is in test_gpu_adjoint.py (the
 Cornell box at 6 and 8 bounces, where the f32 and f64 renders agree along
 the same decisions; in these dense fields some pixels do not, so f64
@@ -31,8 +33,8 @@ import pytest
 from oracle_lib import OracleScene, same_bits
 from pulse_amd import _native as N
 from pulse_amd import procedural as P
-from pulse_amd.path_tracer import (Device, PulseAdjoint, PulsePathTracerNode, apply_material_params, make_params,
-                                   material_params)
+from pulse_amd.path_tracer import (Device, MultiViewAdjoint, PulseAdjoint, PulsePathTracerNode, apply_material_params,
+                                   make_params, material_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -157,36 +159,43 @@ def test_multiview_adjoint_per_instance_parameters(dev, per_instance):
     prm = make_params(spp=2, max_depth=6, mode=N.MODE_GGX_NEE, rr_start_depth=6, flags=N.FLAG_HORIZON_CLAMP)
     cams = _views(camf)
     adj = PulseAdjoint(dev, cams[0].width, cams[0].height, nm)
+    mv = MultiViewAdjoint(dev, cams, nm)
     mp = material_params(scene.buffers["materials"])
+    # the default: the device-memory gradient sums each wave's lanes per material first
+    assert dev.get_option(N.OPT_ADJOINT_REDUCE_ROUNDS) == -1
 
     def pass_over_views(targets):
-        """Gradient accumulated over the views, the per-view gradients and
-        the per-view losses."""
+        """Gradient of the 8 views from one batched call, the same from
+        per-view calls accumulated with PULSE_FLAG_ADJOINT_ACCUMULATE, the
+        per-view gradients and the per-view losses."""
+        imgs = mv.forward(prm)
+        tgts = [targets(img) for img in imgs]
+        mv.set_targets(tgts)
+        batched = mv.backward(prm).astype(np.float64)
         per_view, losses, acc = [], [], None
         for v, cam in enumerate(cams):
-            img = adj.forward(cam, prm)
-            tgt = targets(img)
-            adj.set_target(tgt)
-            losses.append(adj.loss())
-            per_view.append(adj.backward(cam, prm).astype(np.float64))
+            adj.set_target(tgts[v])
+            per_view.append(adj.backward(cam, prm, image=imgs[v]).astype(np.float64))
             if v > 0:
                 adj.grad.copy_from(acc.astype(np.float32).reshape(-1))
                 p = N.Params.from_buffer_copy(prm)
                 p.flags |= N.FLAG_ADJOINT_ACCUMULATE  # add this view's gradient to the others'
-                acc = adj.backward(cam, p)
+                acc = adj.backward(cam, p, image=imgs[v])
             else:
                 acc = per_view[0].astype(np.float32)
-        return acc.astype(np.float64), per_view, losses
+        return batched, acc.astype(np.float64), per_view, mv.losses()
 
     # Euler's identity for the emission, summed over the views
-    acc, per_view, losses = pass_over_views(lambda img: np.zeros_like(img))
-    euler = float(np.sum(mp[:, 4:7] * acc[:, 4:7]))
+    batched, acc, per_view, losses = pass_over_views(lambda img: np.zeros_like(img))
+    euler = float(np.sum(mp[:, 4:7] * batched[:, 4:7]))
     assert sum(losses) > 0.0
     assert abs(euler - 2.0 * sum(losses)) <= 1e-4 * 2.0 * sum(losses), (euler, 2.0 * sum(losses))
-    # accumulation: the sum of the per-view gradients
-    acc, per_view, _ = pass_over_views(lambda img: (img * np.float32(0.7) + np.float32(0.02)).astype(np.float32))
+    # the batched gradient and the accumulated one: the sum of the per-view gradients
+    batched, acc, per_view, _ = pass_over_views(
+        lambda img: (img * np.float32(0.7) + np.float32(0.02)).astype(np.float32))
     total = np.sum(per_view, axis=0)
     scale = np.abs(total).max()
+    assert np.all(np.abs(batched - total) <= 1e-5 * scale + 1e-4 * np.abs(total))
     assert np.all(np.abs(acc - total) <= 1e-5 * scale + 1e-4 * np.abs(total))
     # every instance is a parameter: many of them receive gradient
     assert np.count_nonzero(total[:1024, 0]) > 200

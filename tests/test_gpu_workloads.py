@@ -102,8 +102,7 @@ def test_batch_32_views(dev):
         assert not px.any()
         assert acc[rows][..., :3].mean() > 0
     batch.close()
-    dev.bind_scene(None)
-    for s in scenes:
+    for s in scenes:  # (destroying the bound scene unbinds it)
         s.close()
 
 
