@@ -51,10 +51,11 @@ B_TRIANGLE = 48
 B_RAY_EXTEND = 32 + 20     # ray in, hit out
 B_RAY_SHADOW = 48 + 16     # ray + threshold + colour in, colour out
 # The 4-wide traversal (loop 2): a node visit reads one 64 B quantised node
-# (the roots included); an entered instance reads its index, world_object,
-# object_world, wide root and index offset.
+# (the roots included); an entered instance reads the two 64 B lines of its
+# instance record (world_object rows 0-2 and the BLAS root on entry;
+# object_world rows 0-2, index offset and instance when the ray leaves it).
 B_WIDE_NODE = 64
-B_INSTANCE_WIDE = 4 + 64 + 64 + 4 + 4
+B_INSTANCE_WIDE = 64 + 64
 
 
 def bytes_per_ray(work: dict, io: int) -> float:

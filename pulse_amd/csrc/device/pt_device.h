@@ -50,7 +50,7 @@ struct DevScene {
     const float *strength_cdf;          // light_emission_strength_cdf (may be short: checked on the host)
     const float4 *wide_tlas;            // 4-wide TLAS, 4 float4 per quantised node (null: not built)
     const float4 *wide_blas;            // all meshes' 4-wide BLASes
-    const uint32_t *wide_root;          // per instance: first wide node of its BLAS
+    const uint4 *wide_inst;             // per instance_indices entry: 128-byte instance record (below)
     const float4 *wide_tri;             // tri_bvh rows, refined BLAS leaves reordered (wide.h: mt_select)
     uint32_t instance_count;
     uint32_t light_count;
@@ -59,6 +59,36 @@ struct DevScene {
     // range in which the slab test may divide by reciprocal multiplication)
     bool fast_div;
 };
+
+// Instance record of the 4-wide traversal, one per instance_indices entry
+// (the 4-wide TLAS leaves index these records directly), 32 words in two
+// 64-byte lines: the first is read when a ray enters the instance, the
+// second when it leaves it.  Columns 0-3, rows 0-2 of the column-major
+// matrices: the bottom row of every matrix is 0 0 0 1 (the 4-wide BVHs are
+// built only then), so transform_position's w is 1 and the 3x4 part gives
+// its result bit for bit.
+constexpr int kInstRecordWords = 32;
+constexpr int kRecWorldObject = 0;   // words 0-11: world_object
+constexpr int kRecRoot = 12;         // first wide node of the instance's BLAS (words 13-15 zero)
+constexpr int kRecObjectWorld = 16;  // words 16-27: object_world
+constexpr int kRecIndexOffset = 28;  // the instance's index_offset
+constexpr int kRecInstance = 29;     // the instance (instances[] index); words 30-31 zero
+// The matrix words of a record: columns 0-3, rows 0-2 of m (16 floats,
+// column-major) at rec[0 .. 11].
+__host__ __device__ inline void record_matrix(const float *m, uint32_t *rec) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) {
+            float v = m[4 * c + r];
+            uint32_t u;
+            __builtin_memcpy(&u, &v, 4);
+            rec[3 * c + r] = u;
+        }
+}
+// A bottom row of 0 0 0 1: transform_position's w is then 1 for every
+// finite point.
+__host__ __device__ inline bool affine_matrix(const float *m) {
+    return m[3] == 0.0f && m[7] == 0.0f && m[11] == 0.0f && m[15] == 1.0f;
+}
 
 // ---------------------------------------------------------------------------
 // 3-vectors with the reference's evaluation order.  Templated on the scalar:

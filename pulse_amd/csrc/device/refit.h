@@ -116,6 +116,27 @@ __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse
         }
 }
 
+// The 4-wide traversal's instance records (pt_device.h: kInstRecordWords
+// per instance_indices entry) after k_refit_instances: a thread per entry
+// rewrites the two matrices of the record from its instance (the root,
+// index offset and instance words stay); a bottom row other than 0 0 0 1 is
+// an error (the records hold rows 0-2).
+__global__ void __launch_bounds__(64) k_refit_records(const pulse_mesh_instance *inst, const uint32_t *iidx,
+                                                      uint32_t n_iidx, uint32_t *rec, uint32_t *err) {
+    const uint32_t k = blockIdx.x * 64u + threadIdx.x;
+    if (k >= n_iidx) return;
+    const pulse_mesh_instance &mi = inst[iidx[k]];
+    float ow[16], wo[16];
+    for (int j = 0; j < 16; ++j) {
+        ow[j] = mi.object_world[j];
+        wo[j] = mi.world_object[j];
+    }
+    if (!affine_matrix(ow) || !affine_matrix(wo)) atomicOr(err, kErrRefit);
+    uint32_t *r = rec + static_cast<size_t>(kInstRecordWords) * k;
+    record_matrix(wo, r + kRecWorldObject);
+    record_matrix(ow, r + kRecObjectWorld);
+}
+
 // One block: the boxes of every child of every wide TLAS node into
 // child_box[24 node + 6 k ..] and node_box[6 node ..], one level of the tree
 // at a time, deepest first (`levels`: {n_levels, level starts, node

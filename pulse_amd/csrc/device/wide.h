@@ -170,6 +170,27 @@ __device__ __forceinline__ void mt_select(V3 o, V3 d, const float4 *tri, uint32_
     }
 }
 
+// transform_position / transform_direction (utilities.wgsl:751-759) with a
+// matrix whose bottom row is 0 0 0 1, from its columns 0-3, rows 0-2 packed
+// in three 16-byte words (an instance record line): mat_vec's sums in its
+// order, with w = 1 (m[12 + r] * 1 is m[12 + r]) or w = 0.
+__device__ __forceinline__ V3 affine_point(uint4 a, uint4 b, uint4 c, V3 p) {
+    const float m0 = __uint_as_float(a.x), m1 = __uint_as_float(a.y), m2 = __uint_as_float(a.z);
+    const float m4 = __uint_as_float(a.w), m5 = __uint_as_float(b.x), m6 = __uint_as_float(b.y);
+    const float m8 = __uint_as_float(b.z), m9 = __uint_as_float(b.w), m10 = __uint_as_float(c.x);
+    const float m12 = __uint_as_float(c.y), m13 = __uint_as_float(c.z), m14 = __uint_as_float(c.w);
+    return mk(m0 * p.x + m4 * p.y + m8 * p.z + m12, m1 * p.x + m5 * p.y + m9 * p.z + m13,
+              m2 * p.x + m6 * p.y + m10 * p.z + m14);
+}
+__device__ __forceinline__ V3 affine_direction(uint4 a, uint4 b, uint4 c, V3 p) {
+    const float m0 = __uint_as_float(a.x), m1 = __uint_as_float(a.y), m2 = __uint_as_float(a.z);
+    const float m4 = __uint_as_float(a.w), m5 = __uint_as_float(b.x), m6 = __uint_as_float(b.y);
+    const float m8 = __uint_as_float(b.z), m9 = __uint_as_float(b.w), m10 = __uint_as_float(c.x);
+    const float m12 = __uint_as_float(c.y), m13 = __uint_as_float(c.z), m14 = __uint_as_float(c.w);
+    return mk(m0 * p.x + m4 * p.y + m8 * p.z + m12 * 0.0f, m1 * p.x + m5 * p.y + m9 * p.z + m13 * 0.0f,
+              m2 * p.x + m6 * p.y + m10 * p.z + m14 * 0.0f);
+}
+
 __device__ __forceinline__ void wide_cswap(float &da, uint32_t &ea, float &db, uint32_t &eb) {
     const bool s = db < da;
     const float d = s ? db : da;
@@ -192,7 +213,7 @@ struct WideTraversal {
     // primitive ranges in progress as next | remaining << 26 (one add steps
     // both): instances of a TLAS leaf, triangles of a BLAS leaf
     uint32_t leaf, tri;
-    uint32_t inst;
+    uint32_t inst;               // instance record (instance_indices position) in progress
     bool want_advance;           // next: leave the BLAS and/or enter the next instance
     float safe_t;
     ObjHit h;
@@ -368,12 +389,11 @@ struct WideTraversal {
     // of the TLAS leaf or returns to the TLAS.
     __device__ __forceinline__ bool advance(const DevScene &sc, LdsStack &stk) {
         if (in_blas) {
-            const pulse_mesh_instance &mi = sc.instances[inst];
+            // the record's second line: object_world, index_offset, instance
+            const uint4 *rb = sc.wide_inst + 8u * inst + 4u;
+            const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3];
             if (!h.found) h.t = kTFar;
-            float ow[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) ow[i] = mi.object_world[i];
-            V3 hit_world = transform_position(ow, add(cur.o, scale(h.t, cur.d)));
+            const V3 hit_world = affine_point(b0, b1, b2, add(cur.o, scale(h.t, cur.d)));
             float new_t = length(sub(hit_world, wo));
             if (ANY_HIT) {
                 if (new_t < threshold) {
@@ -382,22 +402,21 @@ struct WideTraversal {
                 }
             } else if (new_t < rec_t) {
                 rec_t = new_t;
-                rec_inst = inst;
-                rec_tri = h.tri - mi.index_offset;
+                rec_inst = b3.y;
+                rec_tri = h.tri - b3.x;
                 rec_u = h.u;
                 rec_v = h.v;
             }
         }
         if (leaf >= kRangeOne) {
-            inst = sc.instance_indices[leaf & kRangeMask];
+            inst = leaf & kRangeMask;
             leaf += kRangeStep;
             n_inst += 1;
-            const pulse_mesh_instance &mi = sc.instances[inst];
-            float wm[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) wm[i] = mi.world_object[i];
-            V3 oo = transform_position(wm, wo);
-            V3 dd = transform_direction(wm, wd);
+            // the record's first line: world_object and the BLAS root
+            const uint4 *ra = sc.wide_inst + 8u * inst;
+            const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];
+            V3 oo = affine_point(a0, a1, a2, wo);
+            V3 dd = affine_direction(a0, a1, a2, wd);
             const float dlen = length(dd);
             cur = wide_ray(oo, divs(dd, dlen));
             float start_t = kTFar;
@@ -415,7 +434,7 @@ struct WideTraversal {
             h.tri = 0;
             h.u = 0.0f;
             h.v = 0.0f;
-            level = reinterpret_cast<const uint4 *>(sc.wide_blas) + 4u * sc.wide_root[inst];
+            level = reinterpret_cast<const uint4 *>(sc.wide_blas) + 4u * a3.x;
             in_blas = true;
             sp_base = sp;
             // (no triangles remain here: every leaf runs to its end)

@@ -46,7 +46,7 @@ struct DevBuf {
 
 enum SceneBuf {
     B_PRIMS, B_TRI_BVH, B_NRM_BVH, B_BLAS, B_TLAS, B_INST_IDX, B_INSTANCES, B_MATERIALS,
-    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_WTLAS, B_WBLAS, B_WROOT, B_WTRI, B_COUNT
+    B_CDFS, B_AREAS, B_LIGHTS, B_SCDF, B_WTLAS, B_WBLAS, B_WINST, B_WTRI, B_COUNT
 };
 
 // Queue counter words of the single-view path: extension and shadow queue
@@ -87,6 +87,10 @@ struct SceneData {
     // triangle_count, index_offset, node_offset}, and the light table
     std::vector<uint32_t> inst_ranges;
     std::vector<pulse_light_data_index> light_indices;
+    // per instance: the first 4-wide node of its BLAS (the instance records'
+    // root word, rewritten by instance updates)
+    std::vector<uint32_t> wide_root;
+    uint32_t n_iidx = 0;         // instance_indices entries (instance records)
     void release() {
         for (auto &b : buf) b.release();
         for (auto &b : refit) b.release();

@@ -101,7 +101,8 @@ int bind_scene(pulse_ctx *ctx, SceneData *sd) {
     if (st == 0) {
         if (lv == 2 && !sd->has_wide)
             PULSE_FAIL(PULSE_E_UNSUPPORTED, "the scene has no 4-wide BVH (a leaf starts at slot 2^26 - 32 or later, "
-                       "or a node coordinate is not finite or beyond 2^40)");
+                       "a node coordinate is not finite or beyond 2^40, or an instance matrix's bottom row is not "
+                       "0 0 0 1)");
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "BVH too deep: TLAS depth %u + BLAS depth %u exceeds the %s stack",
                    sd->tlas_depth, sd->blas_depth, lv == 0 ? "nested loop's" : lv == 1 ? "single loop's" : "4-wide");
     }
@@ -390,7 +391,7 @@ DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
     s.strength_cdf = sd.buf[B_SCDF].as<float>();
     s.wide_tlas = sd.has_wide ? sd.buf[B_WTLAS].as<float4>() : nullptr;
     s.wide_blas = sd.has_wide ? sd.buf[B_WBLAS].as<float4>() : nullptr;
-    s.wide_root = sd.has_wide ? sd.buf[B_WROOT].as<uint32_t>() : nullptr;
+    s.wide_inst = sd.has_wide ? sd.buf[B_WINST].as<uint4>() : nullptr;
     s.wide_tri = sd.has_wide ? sd.buf[B_WTRI].as<float4>() : nullptr;
     s.instance_count = sd.uniform.instance_count;
     s.light_count = sd.uniform.light_count;
@@ -400,6 +401,27 @@ DevScene dev_scene(const SceneData &sd, uint32_t *errors) {
 }
 
 // Validation failures leave `sd` unchanged.
+// The 4-wide traversal's instance records (kInstRecordWords per
+// instance_indices entry, device/pt_device.h), from instance matrices and
+// per-instance BLAS roots.  False if a matrix's bottom row is not 0 0 0 1
+// (the records hold rows 0-2 only).  iidx entries are below n_inst.
+bool instance_records(const pulse_mesh_instance *inst, size_t n_inst, const uint32_t *iidx, size_t n_iidx,
+                      const uint32_t *root, std::vector<uint32_t> &out) {
+    for (size_t i = 0; i < n_inst; ++i)
+        if (!affine_matrix(inst[i].object_world) || !affine_matrix(inst[i].world_object)) return false;
+    out.assign(kInstRecordWords * std::max<size_t>(n_iidx, 1), 0u);
+    for (size_t k = 0; k < n_iidx; ++k) {
+        const uint32_t i = iidx[k];
+        uint32_t *r = &out[kInstRecordWords * k];
+        record_matrix(inst[i].world_object, r + kRecWorldObject);
+        r[kRecRoot] = root[i];
+        record_matrix(inst[i].object_world, r + kRecObjectWorld);
+        r[kRecIndexOffset] = inst[i].index_offset;
+        r[kRecInstance] = i;
+    }
+    return true;
+}
+
 int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     if (!d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
     int rc = PULSE_OK;
@@ -612,6 +634,8 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
             wtri[3 * g + 2] = make_float4(tri_bvh[3 * p + 2].x, f[0], f[1], 0.0f);
         }
     }
+    std::vector<uint32_t> wrec;
+    if (has_wide && !instance_records(inst, n_inst, iidx, n_iidx, wroot.data(), wrec)) has_wide = false;
     const uint32_t wide_need = wtlas.stack_need + wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
 
@@ -634,7 +658,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         return rc;
     if (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) ||
                      (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
-                     (rc = up(B_WROOT, wroot.data(), wroot.size() * 4)) ||
+                     (rc = up(B_WINST, wrec.data(), wrec.size() * 4)) ||
                      (rc = up(B_WTRI, wtri.data(), wtri.size() * sizeof(float4)))))
         return rc;
     {
@@ -653,6 +677,8 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     }
     sd.cut_boxes = std::move(cut_boxes);
     sd.cut_range = std::move(cut_range);
+    sd.wide_root = std::move(wroot);
+    sd.n_iidx = static_cast<uint32_t>(n_iidx);
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
     sd.wide_blas_need = wide_blas_need;
@@ -745,6 +771,10 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
         refit_wide_tlas(wtlas, iidx, ibox.data());
     }
     has_wide = has_wide && quantize_wide(wtlas, qtlas);
+    std::vector<uint32_t> wrec;
+    if (has_wide && !instance_records(inst, n_inst, iidx, n_iidx, sd.wide_root.data(), wrec))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_scene_update_instances: an instance matrix's bottom row is not 0 0 0 1; "
+                   "the scene's 4-wide BVHs hold affine transforms only (upload the scene instead)");
     const uint32_t wide_need = wtlas.stack_need + sd.wide_blas_need + 1;
     if (pick_stack(wide_need, 2) == 0) has_wide = false;
     int rc;
@@ -758,13 +788,15 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     };
     if ((rc = up(B_INSTANCES, inst, d->instances.bytes)) || (rc = up(B_TLAS, tlas, d->tlas_nodes.bytes)) ||
         (rc = up(B_INST_IDX, iidx, d->instance_indices.bytes)) ||
-        (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) || (rc = up_levels())))) {
+        (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) ||
+                      (rc = up(B_WINST, wrec.data(), wrec.size() * 4)) || (rc = up_levels())))) {
         sd.valid = false;  // partially replaced: unusable until the next upload
         return rc;
     }
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
     sd.n_wide_tlas = static_cast<uint32_t>(qtlas.size() / kQuantNodeWords);
+    sd.n_iidx = static_cast<uint32_t>(n_iidx);
     sd.binary_stale = false;
     sd.tlas_depth = tlas_depth;
     sd.stack_need = need;
@@ -802,6 +834,10 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
     hipLaunchKernelGGL(k_refit_instances, dim3(n), dim3(64), 0, stream, sd.refit[0].as<float>(),
                        sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.refit[5].as<float>(),
                        sd.refit[6].as<uint32_t>(), sd.refit[1].as<float>(), n, err);
+    if (sd.n_iidx)
+        hipLaunchKernelGGL(k_refit_records, dim3((sd.n_iidx + 63) / 64), dim3(64), 0, stream,
+                           sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.buf[B_INST_IDX].as<uint32_t>(),
+                           sd.n_iidx, sd.buf[B_WINST].as<uint32_t>(), err);
     hipLaunchKernelGGL(k_refit_boxes, dim3(1), dim3(256), 0, stream, sd.buf[B_WTLAS].as<uint32_t>(),
                        sd.refit[7].as<uint32_t>(), sd.buf[B_INST_IDX].as<uint32_t>(), sd.refit[1].as<float>(),
                        sd.refit[2].as<float>(), sd.refit[3].as<float>());
@@ -813,8 +849,8 @@ int scene_refit_instances(SceneData &sd, const float *object_world, uint32_t n, 
     HIP_TRY(hipStreamSynchronize(stream));
     if (flags) {
         sd.valid = false;  // partly rewritten: unusable until the next upload
-        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: a singular transform or a world box beyond "
-                   "2^40 (the scene must be uploaded again)");
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_refit_instances: a singular transform, a bottom row other than "
+                   "0 0 0 1, or a world box beyond 2^40 (the scene must be uploaded again)");
     }
     sd.binary_stale = true;
     sd.generation += 1;

@@ -114,3 +114,47 @@ def test_batch_sees_the_refit(dev):
     dev.bind_scene(sa)
     assert same_bits(got, _render(dev, cam, prm)).all()
     batch.close()
+
+
+def _projective(n=24, seed=1, which=0):
+    t = _transforms(n, seed)
+    t[which] = t[which].copy()
+    t[which][3, 3] = 2.0  # w = 2: transform_position halves the point
+    return t
+
+
+def test_projective_instance_falls_back_to_the_reference_order():
+    """The 4-wide traversal's instance records hold matrix rows 0-2 only:
+    a scene with a projective instance matrix gets no 4-wide BVHs, the
+    default traversal is then the reference's order (loop 1), bit-exact,
+    and an explicit loop 2 is refused."""
+    s = _make(_projective())
+    cam = _cam()
+    prm = make_params(spp=1, max_depth=3, mode=N.MODE_GGX_NEE)
+    d = Device(0)
+    try:
+        d.upload_scene(s)
+        o, _, _ = OracleScene(s).render(cam, prm)
+        assert same_bits(_render(d, cam, prm), o).all()
+        with pytest.raises(N.PulseError, match="UNSUPPORTED"):
+            d.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+    finally:
+        d.close()
+
+
+def test_update_and_refit_refuse_a_projective_matrix(dev):
+    a, b = _moved()
+    cam = _cam()
+    prm = make_params(spp=1, max_depth=3, mode=N.MODE_GGX_NEE)
+    dev.upload_scene(a)
+    before = _render(dev, cam, prm)
+    p = _make(_projective())
+    with pytest.raises(N.PulseError, match="bottom row"):
+        dev.update_instances(p)
+    # refused before anything was replaced: the scene renders as before
+    assert same_bits(_render(dev, cam, prm), before).all()
+    m = p.buffers["instances"]["object_world"]
+    with pytest.raises(N.PulseError, match="bottom row"):
+        dev.refit_instances(m)
+    dev.upload_scene(a)
+    assert same_bits(_render(dev, cam, prm), before).all()
