@@ -69,7 +69,8 @@ struct DevScene {
 // its result bit for bit.
 constexpr int kInstRecordWords = 32;
 constexpr int kRecWorldObject = 0;   // words 0-11: world_object
-constexpr int kRecRoot = 12;         // first wide node of the instance's BLAS (words 13-15 zero)
+constexpr int kRecRoot = 12;         // first wide node of the instance's BLAS (words 13-14 zero)
+constexpr int kRecMissOverflows = 15;  // 1: a miss's world t overflows for rays from |o| < 2^60 (miss_overflows)
 constexpr int kRecObjectWorld = 16;  // words 16-27: object_world
 constexpr int kRecIndexOffset = 28;  // the instance's index_offset
 constexpr int kRecInstance = 29;     // the instance (instances[] index); words 30-31 zero
@@ -83,6 +84,26 @@ __host__ __device__ inline void record_matrix(const float *m, uint32_t *rec) {
             __builtin_memcpy(&u, &v, 4);
             rec[3 * c + r] = u;
         }
+}
+// True when, for an affine instance with matrices m (object_world) and w
+// (world_object), the reference's world t of a BLAS miss -- the length of
+// m (o' + 1e30 d') - o with o' = w o, d' the unit object direction --
+// overflows to +inf for every ray origin o with components below 2^60, so a
+// miss can never update the closest hit (utilities.wgsl:318-325).  With the
+// 3x3 parts' Frobenius norms ||w|| <= 2^30 and ||m|| ||w|| <= 2^16 and a
+// translation below 2^40: |o'| < 2^92, so the miss point's difference from
+// o is 1e30 m d' up to rounding, of length >= 1e30 / ||w|| >= 2^69, whose
+// largest component squared alone overflows; the rounding of the f32
+// products stays below 2^-6 of it.
+__host__ __device__ inline bool miss_overflows(const float *m, const float *w) {
+    double fm = 0.0, fw = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            fm += static_cast<double>(m[4 * c + r]) * m[4 * c + r];
+            fw += static_cast<double>(w[4 * c + r]) * w[4 * c + r];
+        }
+    const bool t_ok = fabs(m[12]) <= 0x1p40f && fabs(m[13]) <= 0x1p40f && fabs(m[14]) <= 0x1p40f;
+    return t_ok && fw <= 0x1p60 && fm * fw <= 0x1p32;
 }
 // A bottom row of 0 0 0 1: transform_position's w is then 1 for every
 // finite point.
@@ -420,8 +441,10 @@ __device__ __forceinline__ bool traverse_tlas(const DevScene &sc, V3 o, V3 d, fl
                 if (ANY_HIT) {
                     if (new_t < threshold) return true;
                 } else if (new_t < rec_t) {
+                    // a miss with a finite world t: the reference's unhit
+                    // object record, instance 0 (utilities.wgsl:265)
                     rec_t = new_t;
-                    rec_inst = inst;
+                    rec_inst = h.found ? inst : 0u;
                     rec_tri = h.tri;
                     rec_u = h.u;
                     rec_v = h.v;

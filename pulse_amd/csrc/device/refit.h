@@ -118,9 +118,9 @@ __global__ void __launch_bounds__(64) k_refit_instances(const float *mats, pulse
 
 // The 4-wide traversal's instance records (pt_device.h: kInstRecordWords
 // per instance_indices entry) after k_refit_instances: a thread per entry
-// rewrites the two matrices of the record from its instance (the root,
-// index offset and instance words stay); a bottom row other than 0 0 0 1 is
-// an error (the records hold rows 0-2).
+// rewrites the two matrices of the record and its miss flag from its
+// instance (the root, index offset and instance words stay); a bottom row
+// other than 0 0 0 1 is an error (the records hold rows 0-2).
 __global__ void __launch_bounds__(64) k_refit_records(const pulse_mesh_instance *inst, const uint32_t *iidx,
                                                       uint32_t n_iidx, uint32_t *rec, uint32_t *err) {
     const uint32_t k = blockIdx.x * 64u + threadIdx.x;
@@ -134,6 +134,7 @@ __global__ void __launch_bounds__(64) k_refit_records(const pulse_mesh_instance 
     if (!affine_matrix(ow) || !affine_matrix(wo)) atomicOr(err, kErrRefit);
     uint32_t *r = rec + static_cast<size_t>(kInstRecordWords) * k;
     record_matrix(wo, r + kRecWorldObject);
+    r[kRecMissOverflows] = miss_overflows(ow, wo) ? 1u : 0u;
     record_matrix(ow, r + kRecObjectWorld);
 }
 

@@ -271,8 +271,11 @@ struct Traversal {
                     return false;
                 }
             } else if (new_t < rec_t) {
+                // a miss whose world t is finite records the reference's
+                // unhit object record: instance 0, triangle 0
+                // (utilities.wgsl:265)
                 rec_t = new_t;
-                rec_inst = inst;
+                rec_inst = h.found ? inst : 0u;
                 rec_tri = h.tri;
                 rec_u = h.u;
                 rec_v = h.v;

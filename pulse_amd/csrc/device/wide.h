@@ -388,9 +388,13 @@ struct WideTraversal {
     // Finishes the BLAS in progress (if any), then enters the next instance
     // of the TLAS leaf or returns to the TLAS.
     __device__ __forceinline__ bool advance(const DevScene &sc, LdsStack &stk) {
-        if (in_blas) {
+        // A BLAS without a hit below the bound skips the merge when its
+        // world t through the miss point (t = 1e30) certainly overflows to
+        // +inf and so never updates (utilities.wgsl:318-325): bit 31 of inst
+        // is set on entry when that is not certain (miss_overflows).
+        if (in_blas && (h.found || (inst >> 31))) {
             // the record's second line: object_world, index_offset, instance
-            const uint4 *rb = sc.wide_inst + 8u * inst + 4u;
+            const uint4 *rb = sc.wide_inst + 8u * (inst & kRangeMask) + 4u;
             const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3];
             if (!h.found) h.t = kTFar;
             const V3 hit_world = affine_point(b0, b1, b2, add(cur.o, scale(h.t, cur.d)));
@@ -401,9 +405,12 @@ struct WideTraversal {
                     return false;
                 }
             } else if (new_t < rec_t) {
+                // a miss whose world t is finite (an instance that scales
+                // down by ~1e-11 or more) records the reference's unhit
+                // object record: instance 0, triangle 0 (utilities.wgsl:265)
                 rec_t = new_t;
-                rec_inst = b3.y;
-                rec_tri = h.tri - b3.x;
+                rec_inst = h.found ? b3.y : 0u;
+                rec_tri = h.found ? h.tri - b3.x : 0u;
                 rec_u = h.u;
                 rec_v = h.v;
             }
@@ -415,6 +422,7 @@ struct WideTraversal {
             // the record's first line: world_object and the BLAS root
             const uint4 *ra = sc.wide_inst + 8u * inst;
             const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];
+            if (!(a3.w != 0u && fmaxf(fmaxf(fabsf(wo.x), fabsf(wo.y)), fabsf(wo.z)) < 0x1p60f)) inst |= 0x80000000u;
             V3 oo = affine_point(a0, a1, a2, wo);
             V3 dd = affine_direction(a0, a1, a2, wd);
             const float dlen = length(dd);

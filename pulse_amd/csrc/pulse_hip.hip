@@ -415,6 +415,7 @@ bool instance_records(const pulse_mesh_instance *inst, size_t n_inst, const uint
         uint32_t *r = &out[kInstRecordWords * k];
         record_matrix(inst[i].world_object, r + kRecWorldObject);
         r[kRecRoot] = root[i];
+        r[kRecMissOverflows] = miss_overflows(inst[i].object_world, inst[i].world_object) ? 1u : 0u;
         record_matrix(inst[i].object_world, r + kRecObjectWorld);
         r[kRecIndexOffset] = inst[i].index_offset;
         r[kRecInstance] = i;

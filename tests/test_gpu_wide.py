@@ -251,3 +251,71 @@ def test_fast_reciprocal_is_the_division(dev):
     correctly rounded division for every f32 with |a| in [2^-100, 2^126)
     (both signs, every mantissa): no difference."""
     assert dev.selftest_reciprocal() == 0
+
+
+def test_tiny_instance_miss_records(dev):
+    """An instance scaled down by 1e-12: a ray that enters its box and misses
+    its triangles has a finite world t through the miss point (1e30 in
+    object space), which the reference records with its unhit object record
+    -- instance 0, triangle 0, u = v = 0 (utilities.wgsl:265, 318-325).  The
+    4-wide traversal's instance records mark such an instance (its miss need
+    not overflow), so the merge still runs for it; every traversal loop
+    equals the oracle."""
+    from pulse_amd.scene import MaterialDesc, SceneBuilder
+
+    rng = P.CounterRng(21)
+    sb = SceneBuilder()
+    mesh = sb.add_mesh(P.displaced_icosphere(2, 1.0, rng, 7))
+    mat = sb.add_material(MaterialDesc(base_color=(0.5, 0.5, 0.5, 1.0), perceptual_roughness=0.5))
+    for k in range(6):  # ordinary instances away from the rays
+        m = np.eye(4)
+        m[:3, 3] = (20.0 + 4.0 * k, -3.0 + k, 5.0)
+        sb.add_instance(mesh, mat, m)
+    tiny = np.eye(4)
+    tiny[:3, :3] *= 1e-12  # at the world origin, where f32 resolves its 1e-12 box
+    sb.add_instance(mesh, mat, tiny)
+    scene = sb.build(2)
+    r = np.random.default_rng(5)
+    n = 2048
+    # from 1e-5 away towards a point of its box outside the sphere: in object
+    # space the aim spreads by ~0.6 units, so rays both hit and miss the mesh
+    target = np.full(3, 0.95e-12)
+    u = r.normal(size=(n, 3))
+    o = target + 1e-5 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    d = target - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, N.RAY_DTYPE)
+    rays["origin"] = o.astype(np.float32)
+    rays["dir"] = d.astype(np.float32)
+    rays["t_max"] = np.float32(1e30)
+    orc = OracleScene(scene)
+    want, _ = orc.trace_rays(rays)
+    spurious = (want["t"] < 1e30) & (want["t"] > 1e10)
+    assert spurious.sum() > n // 4, "the rays should enter the tiny instance and miss its triangles"
+    assert (want["instance_index"][spurious] == 0).all() and (want["triangle_index"][spurious] == 0).all()
+    try:
+        for loop in (1, 0):  # the reference's order: bit for bit
+            dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+            dev.upload_scene(scene)
+            g = dev.trace_rays(rays)
+            for k in ("t", "u", "v", "instance_index", "triangle_index"):
+                assert same_bits(g[k], want[k]).all(), f"loop {loop}: {k}"
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 2)
+    # The 4-wide boxes are padded where the reference's rounded box tests
+    # (here at object-space distances of 1e7) skip boxes holding a hit, so
+    # some of its misses are hits or entries here.  Every result is a real
+    # hit of the reference's triangle test, a miss, or the unhit record with
+    # the reference's miss-point t.
+    dev.upload_scene(scene)
+    g = dev.trace_rays(rays)
+    sp = (g["t"] < 1e30) & (g["t"] > 1e10)
+    assert sp.sum() > n // 8
+    assert (g["instance_index"][sp] == 0).all() and (g["triangle_index"][sp] == 0).all()
+    assert (g["u"][sp] == 0).all() and (g["v"][sp] == 0).all()
+    both = sp & spurious
+    assert both.sum() > n // 8 and same_bits(g["t"][both], want["t"][both]).all()
+    hit = g["t"] <= 1e10
+    r = orc.intersect_candidates(rays[hit], g["instance_index"][hit], g["triangle_index"][hit])
+    for k in ("t", "u", "v"):
+        assert same_bits(r[k], g[k][hit]).all(), k
