@@ -38,8 +38,9 @@ extern "C" {
                                    /* element size, or an index out of range */
 #define PULSE_E_STACK_OVERFLOW -4  /* traversal stack exceeded (device flag) */
 #define PULSE_E_ITERATION_CAP -5   /* traversal hit its loop cap (100000    */
-                                   /* steps per walk for loops 0/1, 2^22     */
-                                   /* node visits per ray for loop 2)        */
+                                   /* steps per walk of the binary BVHs,     */
+                                   /* loops 0/1; loop 2's 4-wide trees are   */
+                                   /* checked acyclic at upload instead)     */
 #define PULSE_E_NO_SCENE -6        /* trace called before scene upload       */
 #define PULSE_E_UNSUPPORTED -7     /* parameter combination not supported    */
 #define PULSE_E_BUILD -8           /* host BVH builder failed                */

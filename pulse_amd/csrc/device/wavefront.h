@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(64) k_census(DevScene sc, WfBuffers wb, int q,
     const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
     uint32_t c[4] = {0u, 0u, 0u, 0u};
     if (i < count) {
-        WideTraversal<SHADOW, 2> tr;
+        WideTraversal<SHADOW, 2, true> tr;
         const float4 oa = SHADOW ? wb.sh_o[i] : wb.ray_o[q][i];
         const float4 da = SHADOW ? wb.sh_d[i] : wb.ray_d[q][i];
         if (tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, SHADOW ? da.w : 0.0f)) {

@@ -25,7 +25,6 @@
 namespace pt {
 
 constexpr uint32_t kWideEmpty = 0xffffffffu;
-constexpr int kMaxWideVisits = 1 << 22;  // per ray, over both levels
 constexpr uint32_t kRangeMask = (1u << 26) - 1u;
 constexpr uint32_t kRangeOne = 1u << 26;            // one primitive remaining
 constexpr uint32_t kRangeStep = 1u - (1u << 26);    // next + 1, remaining - 1
@@ -201,7 +200,13 @@ __device__ __forceinline__ void wide_cswap(float &da, uint32_t &ea, float &db, u
     ea = e;
 }
 
-template <bool ANY_HIT, int TRIS_PER_STEP = 2>
+// No iteration guard: every interior entry of a 4-wide tree names a node of
+// a larger index (the collapse allocates a node before its children, and the
+// upload refuses a tree where that fails: wide_bvh.h, collapse_wide), so a
+// walk visits each node of a tree at most once per entry into the tree and
+// ends.  COUNT keeps the work census (node visits, triangle tests,
+// instances entered).
+template <bool ANY_HIT, int TRIS_PER_STEP = 2, bool COUNT = false>
 struct WideTraversal {
     V3 wo, wd;         // world ray
     float threshold, slack_thr, any_cull;  // any_cull: occlusion mode's world culling bound
@@ -209,7 +214,7 @@ struct WideTraversal {
     const uint4 *level;  // 16-byte words of the current level's tree (its root node first)
     bool in_blas;
     int sp, sp_base;
-    int iter;             // wide node visits of this ray (a guard: validated trees stay far below)
+    int iter;             // wide node visits of this ray (work census)
     // primitive ranges in progress as next | remaining << 26 (one add steps
     // both): instances of a TLAS leaf, triangles of a BLAS leaf
     uint32_t leaf, tri;
@@ -282,11 +287,7 @@ struct WideTraversal {
         }
         if (!(in_blas && tri >= kRangeOne) && !(entry >> 31)) {
             // interior wide node of the current level
-            if (iter >= kMaxWideVisits) {
-                atomicOr(sc.error_flags, kErrIterationCap);
-                return false;
-            }
-            iter += 1;
+            if (COUNT) iter += 1;
             const float cur_t = in_blas ? h.t : wcull();  // culling bound of this level
             const uint4 *p = level + 4u * entry;
             const uint4 qa = p[0], qb = p[1], qc = p[2], qd = p[3];
@@ -331,11 +332,11 @@ struct WideTraversal {
         // tri_bvh slot) becomes the mesh's slot at the merge
         mt_select<!ANY_HIT>(cur.o, cur.d, sc.wide_tri, tri & kRangeMask, h);
         tri += kRangeStep;
-        n_tri += 1;
+        if (COUNT) n_tri += 1;
         if (TRIS_PER_STEP == 2 && tri >= kRangeOne) {
             mt_select<!ANY_HIT>(cur.o, cur.d, sc.wide_tri, tri & kRangeMask, h);
             tri += kRangeStep;
-            n_tri += 1;
+            if (COUNT) n_tri += 1;
         }
         if (tri >= kRangeOne) return true;
         if (ANY_HIT && h.found && h.t < safe_t) {
@@ -418,7 +419,7 @@ struct WideTraversal {
         if (leaf >= kRangeOne) {
             inst = leaf & kRangeMask;
             leaf += kRangeStep;
-            n_inst += 1;
+            if (COUNT) n_inst += 1;
             // the record's first line: world_object and the BLAS root
             const uint4 *ra = sc.wide_inst + 8u * inst;
             const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];

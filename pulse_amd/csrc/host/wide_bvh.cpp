@@ -162,6 +162,14 @@ bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideT
         need = sub;
     }
     out.stack_need = need;
+    // every interior entry names a later node: a walk visits each node at
+    // most once per entry into the tree (the traversal has no visit guard)
+    const size_t nodes_out = out.words.size() / kWideNodeWords;
+    for (size_t i = 0; i < nodes_out && b.ok; ++i)
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t e = out.words[i * kWideNodeWords + 24 + k];
+            if (e != kWideEmpty && !(e & kWideLeafBit) && (e <= i || e >= nodes_out)) b.ok = false;
+        }
     return b.ok;
 }
 

@@ -52,7 +52,10 @@ struct WideTree {
 // the reference's node layout: min xyz, first child / first primitive, max
 // xyz, primitive count; node 0 is the root, child indices are relative to
 // `nodes`) into `out`; leaf entries hold first primitive + `first_offset`.
-// False if that is at or beyond kWideFirstLimit (the entry cannot encode it).
+// False if that is at or beyond kWideFirstLimit (the entry cannot encode it),
+// or if an interior entry does not name a later node of `out` (nodes are
+// allocated before their children, so this holds for every acyclic input;
+// the traversal relies on it instead of counting visits).
 bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideTree &out);
 
 constexpr int kQuantNodeWords = 16;

@@ -1039,8 +1039,8 @@ int pulse_sync(pulse_ctx *ctx) {
     if (rc != PULSE_OK) return rc;
     if (err & kErrStackOverflow) PULSE_FAIL(PULSE_E_STACK_OVERFLOW, "traversal stack overflow (capacity %u)", ctx->stack_entries);
     if (err & kErrIterationCap)
-        PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached its loop cap (%s)",
-                   ctx->loop_variant == 2 ? "2^22 node visits per ray" : "100000 steps per walk");
+        PULSE_FAIL(PULSE_E_ITERATION_CAP, "traversal reached its loop cap (100000 steps per walk of the "
+                   "reference's binary BVHs)");
     return PULSE_OK;
 }
 
