@@ -552,8 +552,10 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
 /* dL/dparameters of the image rendered by pulse_trace with the same camera
  * and params (accumulation_count 0, params->spp samples, mode
  * PULSE_MODE_GGX_NEE, max_depth <= 8 -- <= 4 with
- * PULSE_FLAG_ADJOINT_REPARAM --, no Russian roulette:
- * rr_start_depth >= max_depth).  Replays every path from its seed; `grad`
+ * PULSE_FLAG_ADJOINT_REPARAM --, any rr_start_depth: Russian roulette is
+ * replayed with its decisions; a random survival (p <= 1) contributes its
+ * score, a certain one (p > 1) the derivative of its 1/p).
+ * Replays every path from its seed; `grad`
  * (device, n_materials*7 floats) is overwritten, or added to with
  * PULSE_FLAG_ADJOINT_ACCUMULATE.  `replay_rgba` (device,
  * optional) receives the replayed image, which equals the forward image bit

@@ -321,6 +321,10 @@ int oracle_render_pair(oracle_scene *sc, const pulse_camera *cam, const pulse_pa
 // with every decision fixed the f64 image is a smooth function of the
 // material parameters.  params: n_materials*7 f64 values {base.rgb,
 // roughness, emissive.rgb}; ids: parameter indices (7*m + j); h: step.
+// Along replayed decisions a random roulette survival (p <= 1) keeps its
+// weight 1/p and carries the likelihood ratio of the decision to later
+// roulettes; a p above 1 (the path survives whatever the draw) follows the
+// perturbed throughput.
 // frozen_dirs = 0: the sampled directions follow the roughness (the
 // reparameterised derivative); 1: the specular half vectors sampled at the
 // unperturbed parameters (f64) are held fixed (the detached derivative);
@@ -362,6 +366,10 @@ int oracle_grad_fd_rows(oracle_scene *sc, const pulse_camera *cam, const pulse_p
     std::vector<double> p(params, params + 7 * static_cast<size_t>(n_materials));
     std::vector<std::vector<double>> dirs(frozen_dirs == 1 ? logs.size() : 0);
     DecisionTape::DirMode dmode = frozen_dirs == 1 ? DecisionTape::RECORD_DIRS : DecisionTape::DIRS_OFF;
+    // roulette probabilities of the unperturbed paths, replayed along the
+    // decisions
+    std::vector<std::vector<double>> rrp(frozen_dirs == 2 ? 0 : logs.size());
+    DecisionTape::RrMode rmode = frozen_dirs == 2 ? DecisionTape::RR_OFF : DecisionTape::RECORD_RR;
     uint64_t used = 0;
     for (uint32_t yy = 0; yy < H; ++yy)
         for (uint32_t x = 0; x < W; ++x) used += std::isnan(target[4 * (static_cast<size_t>(row0 + yy) * W + x)]) ? 0 : 1;
@@ -378,6 +386,10 @@ int oracle_grad_fd_rows(oracle_scene *sc, const pulse_camera *cam, const pulse_p
             if (frozen_dirs == 1) {
                 rep.dir_mode = dmode;
                 rep.dirs = &dirs[(static_cast<size_t>(yy) * W + x) * spp + k];
+            }
+            if (frozen_dirs != 2) {
+                rep.rr_mode = rmode;
+                rep.rr = &rrp[(static_cast<size_t>(yy) * W + x) * spp + k];
             }
             V3<R> c = trace_path<R>(s, *cam, *prm, x, row0 + yy, prm->accumulation_count + k, pc[tid], rep);
             I = accumulate(I, c, k);
@@ -413,6 +425,7 @@ int oracle_grad_fd_rows(oracle_scene *sc, const pulse_camera *cam, const pulse_p
     double l0 = loss(p, eval32 != 0);
     if (loss_out) *loss_out = l0;
     if (frozen_dirs == 1) dmode = DecisionTape::FREEZE_DIRS;
+    if (frozen_dirs != 2) rmode = DecisionTape::FREEZE_RR;
     for (uint32_t i = 0; i < n_ids; ++i) {
         std::vector<double> pp = p, pm = p;
         pp[ids[i]] += h;

@@ -569,6 +569,22 @@ struct DecisionTape {
     DirMode dir_mode = DIRS_OFF;
     std::vector<double> *dirs = nullptr;
     size_t dir_pos = 0;
+    // Frozen roulette: RECORD_RR stores each survival probability p =
+    // max(throughput); FREEZE_RR replays the decisions and treats a
+    // recorded p <= 1 (a random survival) as a decision sampled at the
+    // unperturbed parameters: the path takes the perturbed estimator's own
+    // p' and is reweighted by the likelihood ratio p' / p of its survival,
+    // so that its weight stays 1/p while the throughput later roulettes see
+    // is the perturbed estimator's (divided by p').  A recorded p > 1 (the
+    // path survives whatever the draw) follows the perturbed throughput.
+    // The product of the likelihood ratios so far (frozen directions and
+    // survivals) is `lr`: the throughput carries it, the roulette's p is
+    // that of the throughput without it.
+    enum RrMode { RR_OFF, RECORD_RR, FREEZE_RR };
+    RrMode rr_mode = RR_OFF;
+    std::vector<double> *rr = nullptr;
+    size_t rr_pos = 0;
+    double lr = 1.0;
 };
 
 constexpr uint64_t kMissCode = ~0ull;
@@ -787,6 +803,7 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
         } else if (tape.dir_mode == DecisionTape::FREEZE_DIRS && tape.dir_pos + 4 <= tape.dirs->size()) {
             ratio = pdf / static_cast<R>((*tape.dirs)[tape.dir_pos + 3]);
             tape.dir_pos += 4;
+            tape.lr *= static_cast<double>(ratio);
         }
     }
     if (tape.take_bool(pdf < R(0.0001f))) pdf = R(1);
@@ -1023,6 +1040,17 @@ bool russian_roulette(V3<R> &throughput, uint32_t &rng, DecisionTape &tape) {
     R p = max3(throughput);
     R e = static_cast<R>(rand_f(rng));
     if (tape.take_bool(e > p)) return false;
+    if (tape.rr_mode == DecisionTape::RECORD_RR) tape.rr->push_back(static_cast<double>(p));
+    if (tape.rr_mode == DecisionTape::FREEZE_RR && tape.rr_pos < tape.rr->size()) {
+        const double q = (*tape.rr)[tape.rr_pos++];
+        const R own = p / static_cast<R>(tape.lr);
+        if (q <= 1.0) {
+            tape.lr *= static_cast<double>(own) / q;
+            p = static_cast<R>(q);
+        } else {
+            p = own;
+        }
+    }
     R ip = R(1) / p;
     throughput = throughput * ip;
     return true;

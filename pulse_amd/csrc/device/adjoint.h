@@ -15,10 +15,24 @@
 // The derivative is that of the estimator with every decision held fixed
 // (closest-hit ids, lobe, occlusion, fold of the light sample), i.e. the
 // derivative a finite difference with the same random numbers measures away
-// from those discontinuities.  Russian roulette is not differentiated (its
-// probability depends on the albedo): the adjoint requires it off
-// (rr_start_depth >= max_depth); metallic materials are rejected on upload
+// from those discontinuities.  Russian roulette (path_tracer.wgsl:104-108)
+// is replayed with its decision.  Its p = max(throughput) divides the
+// throughput: where p > 1 the path survives whatever the draw and 1/p is
+// differentiated like any weight; where p <= 1 the survival is random and
+// its score d ln p is added, which cancels 1/p's own derivative in every
+// later contribution (E[1{u < p} / p] = 1) but not in later roulettes: a
+// later p > 1 is measured on the throughput divided by this p, so its
+// 1/p' carries this p's derivative back.  Metallic materials are rejected
 // for the adjoint (their lobe probability would depend on the albedo).
+//
+// Reverse sweep: with a_k = dL/d ln(beta_k) per channel (beta_k the
+// throughput reaching vertex k) and S_k the suffix sum of vertex
+// contributions, a_k = g beta_k S_k + E_k, where E carries the roulette
+// terms: E_k = E_{k+1} - e_c X_k for a survived roulette at k (c the
+// channel of the maximum), X_k = sum over channels of a_{k+1} where p > 1
+// and of E_{k+1} where p <= 1 (the score removes g beta_{k+1} S_{k+1});
+// a parameter moving ln w_k (the weight of vertex k) by d then adds
+// (g beta_{k+1} S_{k+1} + E_k) . d.
 //
 // Per block the gradient is accumulated in LDS (float atomics), then added to
 // the global gradient with one atomic per entry; a gradient too large for the
@@ -68,6 +82,7 @@ struct AdjArgs {
     uint32_t horizon_clamp;
     uint32_t detached;     // roughness derivative at fixed sampled directions
     uint32_t reduce_rounds;  // distinct keys per wave summed before the atomic (0: per-lane atomics)
+    uint32_t rr_start;     // Russian roulette from this depth on (as the forward, shade.h)
 };
 
 template <class S>
@@ -159,6 +174,7 @@ __device__ __forceinline__ void grad_add_wave(float *buf, uint32_t key, const fl
 template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1, int MAXV = kAdjMaxVertices>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
     static_assert(BLOCK == kAdjBlockPixels, "a block replays 128 pixels");
+    static_assert(MAXV <= 8, "the roulette record holds 4 bits per vertex in 32");
     using S = typename std::conditional<DETACHED, float, Dual<MAXV>>::type;
     __shared__ uint32_t smem[STACK * BLOCK];
     extern __shared__ float sgrad[];
@@ -204,9 +220,15 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             // stored vertex values for the reverse sweep
             V3 beta[MAXV], cvert[MAXV], wvert[MAXV];
             V3 dDdb[MAXV], dDdLe[MAXV], dDdr[MAXV];
-            float dwdb[MAXV], dlogw[MAXV];
+            float dwdb[MAXV], dlogw[MAXV], dlogq[MAXV];
             uint32_t mat[MAXV], lmat[MAXV];
             bool vis[MAXV];
+            // 4 bits per vertex: 1 + the channel of the maximum at a survived
+            // roulette (0: none), | 4 if its p exceeded 1
+            uint32_t rrc = 0;
+            // roughness tangent of ln(throughput with every 1/p varying) -
+            // ln(T) (one scalar: the p <= 1 survivals hold T's 1/p fixed)
+            S nrm = sconst<S>(0.0f);
             int nv = 0;
             for (uint32_t depth = 0; depth < depth_end; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
@@ -268,6 +290,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 wvert[nv] = mk(0.0f, 0.0f, 0.0f);
                 dwdb[nv] = 0.0f;
                 dlogw[nv] = 0.0f;
+                dlogq[nv] = 0.0f;
                 const bool last = depth + 1u >= a.max_depth;
                 if (!last) {
                     SampleT<S> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
@@ -279,6 +302,37 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     wvert[nv] = val3(smp.reflectance);
                     dwdb[nv] = smp.diffuse ? smp.dbase : 0.0f;
                     dlogw[nv] = smp.dlogf_dr;
+                    dlogq[nv] = smp.dlogq_dr;
+                    if (depth >= a.rr_start) {
+                        // the forward's roulette (shade.h) on the replayed
+                        // throughput: the same draw, the same end of the
+                        // path, the same 1/p
+                        const V3 tv = val3(T);
+                        const float pr = max3(tv);
+                        if (rand_f(rng) > pr) {
+                            nv += 1;
+                            break;
+                        }
+                        const float ip = 1.0f / pr;
+                        const uint32_t c = tv.x == pr ? 0u : (tv.y == pr ? 1u : 2u);
+                        rrc |= ((c + 1u) | (pr > 1.0f ? 4u : 0u)) << (4 * nv);
+                        if constexpr (DETACHED) {
+                            T = scale(ip, T);
+                        } else {
+                            const S tc = c == 0u ? T.x : (c == 1u ? T.y : T.z);
+                            if (pr > 1.0f) {
+                                // 1/p with p = T_c times the normaliser
+                                // (value ip, as the forward)
+                                const S ipd = 1.0f / (tc * (1.0f + nrm));
+                                T = scale(ipd, T);
+                            } else {
+                                nrm = 1.0f - tc / pr;
+                                T = scale(ip, T);
+                            }
+                        }
+                        wvert[nv] = scale(ip, wvert[nv]);
+                        dwdb[nv] *= ip;
+                    }
                     d = normalize(smp.wi);
                     o = add(h.position, scale(0.001f, h.normal));
                 }
@@ -308,6 +362,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             }
             // albedo and emission: reverse sweep over the vertices
             float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
+            float Ex = 0.0f, Ey = 0.0f, Ez = 0.0f;  // E_{k+1}: the 1/p terms (zero without p > 1)
+            V3 bn = mk(0.0f, 0.0f, 0.0f);           // beta_{k+1}
             for (int k = kmax - 1; k >= 0; --k) {
                 uint32_t kb = kNoKey, ke = kNoKey, k0 = kNoKey;  // albedo+roughness, NEE emission, direct emission
                 float v4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, e3[3] = {0.0f, 0.0f, 0.0f}, d3[3] = {0.0f, 0.0f, 0.0f};
@@ -315,23 +371,39 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     const V3 b = beta[k];
                     const uint32_t m = mat[k];
                     const float dwb = dwdb[k];
-                    const float ax = gx * b.x * (dDdb[k].x + Sx * dwb);
-                    const float ay = gy * b.y * (dDdb[k].y + Sy * dwb);
-                    const float az = gz * b.z * (dDdb[k].z + Sz * dwb);
+                    const V3 wk = wvert[k];
+                    // F = E_k (the roulette terms of this vertex and later ones)
+                    float Fx = Ex, Fy = Ey, Fz = Ez;
+                    const uint32_t ck = (rrc >> (4 * k)) & 7u;
+                    if (ck != 0u) {
+                        const float es = Ex + Ey + Ez;
+                        const float as = (ck & 4u) ? gx * bn.x * Sx + gy * bn.y * Sy + gz * bn.z * Sz + es : es;
+                        Fx -= (ck & 3u) == 1u ? as : 0.0f;
+                        Fy -= (ck & 3u) == 2u ? as : 0.0f;
+                        Fz -= (ck & 3u) == 3u ? as : 0.0f;
+                    }
+                    // d ln w_k / d base = dwb / w_k per channel; F is nonzero only
+                    // in channels whose w_k is positive (they hold a later p > 1)
+                    const float ax = gx * b.x * (dDdb[k].x + Sx * dwb) + (Fx != 0.0f ? Fx * dwb / wk.x : 0.0f);
+                    const float ay = gy * b.y * (dDdb[k].y + Sy * dwb) + (Fy != 0.0f ? Fy * dwb / wk.y : 0.0f);
+                    const float az = gz * b.z * (dDdb[k].z + Sz * dwb) + (Fz != 0.0f ? Fz * dwb / wk.z : 0.0f);
                     if (ax == ax && ay == ay && az == az) {
                         kb = m * kGradPerMaterial;
                         v4[0] = ax;
                         v4[1] = ay;
                         v4[2] = az;
                     }
-                    const V3 wk = wvert[k];
                     if constexpr (DETACHED) {
                         // roughness at fixed directions: NEE (GGX evaluator) and
-                        // the sampling weight w_k (d w = w d ln f)
+                        // the sampling weight w_k with the likelihood ratio of
+                        // its direction (d (w q) = w q d ln f, every channel);
+                        // the 1/p terms follow w_k alone (p is the path's own
+                        // throughput), d ln w = d ln f - d ln q
                         const float dl = dlogw[k];
                         const float dr = gx * b.x * (dDdr[k].x + Sx * wk.x * dl) +
                                          gy * b.y * (dDdr[k].y + Sy * wk.y * dl) +
-                                         gz * b.z * (dDdr[k].z + Sz * wk.z * dl);
+                                         gz * b.z * (dDdr[k].z + Sz * wk.z * dl) +
+                                         (Fx + Fy + Fz) * (dl - dlogq[k]);
                         if (dr == dr) {
                             kb = m * kGradPerMaterial;
                             v4[3] = dr;
@@ -356,10 +428,14 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                             d3[2] = ez;
                         }
                     }
-                    const V3 ck = cvert[k];
-                    Sx = ck.x + wk.x * Sx;
-                    Sy = ck.y + wk.y * Sy;
-                    Sz = ck.z + wk.z * Sz;
+                    const V3 cv = cvert[k];
+                    Sx = cv.x + wk.x * Sx;
+                    Sy = cv.y + wk.y * Sy;
+                    Sz = cv.z + wk.z * Sz;
+                    Ex = Fx;
+                    Ey = Fy;
+                    Ez = Fz;
+                    bn = b;
                 }
                 grad_add_wave(gbuf, kb, v4, rounds);
                 grad_add_wave(gbuf, ke, e3, rounds);

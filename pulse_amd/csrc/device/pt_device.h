@@ -571,6 +571,9 @@ struct SampleT {
     // d ln(D G); when the pdf floor replaced pdf by 1, w q = brdf q, one more
     // d ln D.  Diffuse sample: 0.
     float dlogf_dr;
+    // d ln q / d roughness at the fixed sample (specular: d ln D; diffuse: 0),
+    // so that d ln w = dlogf_dr - dlogq_dr.
+    float dlogq_dr;
 };
 
 // d ln D_GGX / d roughness at a fixed microfacet normal (D = a2 / (pi b^2),
@@ -662,6 +665,7 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         pdf = sconst<S>(1.0f);
         out.dlogf_dr += dlog_d;
     }
+    out.dlogq_dr = dlog_d;
     out.wi = wi;
     out.reflectance = divs(brdf, pdf);
     out.dbase = dbase_num / val(pdf);

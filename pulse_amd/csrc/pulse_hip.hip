@@ -898,8 +898,6 @@ int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be 1..%d", who, kAdjMaxVerticesDeep);
     if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d", who, kAdjMaxVertices);
-    if (prm->rr_start_depth < prm->max_depth)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: Russian roulette must be off (rr_start_depth >= max_depth)", who);
     if (prm->accumulation_count != 0 || prm->spp == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs accumulation_count 0 and spp >= 1", who);
     if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: materials with metallic != 0", who);
@@ -927,6 +925,7 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
     a.n_materials = ctx->cur->n_materials;
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
+    a.rr_start = prm->rr_start_depth;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
     a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
     const size_t lds = a.use_lds ? lds_bytes : 0;

@@ -40,10 +40,10 @@ def dev():
     d.close()
 
 
-def _cornell_case(spp=2, depth=3, size=48):
+def _cornell_case(spp=2, depth=3, size=48, rr=None):
     scene, camf = P.cornell()
     cam = camf(size, size)
-    prm = make_params(spp=spp, max_depth=depth, mode=N.MODE_GGX_NEE, rr_start_depth=depth,
+    prm = make_params(spp=spp, max_depth=depth, mode=N.MODE_GGX_NEE, rr_start_depth=depth if rr is None else rr,
                       flags=N.FLAG_HORIZON_CLAMP)
     return scene, cam, prm
 
@@ -95,6 +95,28 @@ def test_deep_paths_replay_and_oracle_gradient(dev, depth):
     _replay_case(dev, scene, cam, prm, True)
 
 
+@pytest.mark.parametrize("loop", [1, 2])
+@pytest.mark.parametrize("extra_flags,frozen", MODES)
+def test_roulette_replay_and_oracle_gradient(dev, extra_flags, frozen, loop):
+    """Russian roulette from the first bounce on: the replay draws the same
+    numbers and ends the same paths (replay == forward bit for bit), and the
+    gradient equals the oracle's central differences along the same
+    decisions, random survivals (p <= 1) reweighted by their likelihood
+    ratio, certain ones (p > 1) divided by the perturbed p."""
+    scene, cam, prm = _cornell_case(depth=4, rr=1)
+    prm.flags |= extra_flags
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        _replay_case(dev, scene, cam, prm, frozen)
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
+def test_roulette_deep_paths(dev):
+    scene, cam, prm = _cornell_case(depth=8, rr=2)
+    _replay_case(dev, scene, cam, prm, True)
+
+
 def test_emission_euler_identity_gpu(dev):
     scene, cam, prm = _cornell_case(spp=2, depth=3, size=40)
     dev.upload_scene(scene)
@@ -130,9 +152,6 @@ def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen):
     scale = np.abs(gref).max()
     print("field: max err / scale", float((np.abs(g - gref) / scale).max()), flush=True)
     assert (np.abs(g - gref) <= 2e-3 * np.abs(gref) + 1e-4 * scale).all()
-    with pytest.raises(N.PulseError, match="UNSUPPORTED"):
-        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=0,
-                                      flags=N.FLAG_HORIZON_CLAMP))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
         adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_GGX, rr_start_depth=3))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
