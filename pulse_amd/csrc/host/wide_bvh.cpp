@@ -1,12 +1,18 @@
 // wide_bvh.cpp -- collapse of the reference's binary BVHs into 4-wide nodes
-// (layout in wide_bvh.h).  A wide node takes the two children of a binary
-// interior node and keeps replacing its largest-area interior child by that
-// child's two children until it has four children or only leaves: the
-// leaves, their order and their primitive ranges are the binary tree's.
+// (layout in wide_bvh.h).  Which binary interior nodes become wide nodes is
+// chosen by dynamic programming over the binary tree for the least summed
+// surface area of the wide nodes (the expected number of node visits of a
+// ray through the root box, up to a constant): f(n, k), the least cost of
+// covering the subtree of n by at most k subtrees, is 0 for a leaf and
+// min(f(n, 1), min_i f(a, i) + f(b, k - i)) otherwise, with f(n, 1) =
+// area(n) + min_i f(a, i) + f(b, 4 - i) (n itself a wide node over its
+// cheapest cover by four).  The leaves, their order and their primitive
+// ranges are the binary tree's.
 
 #include "wide_bvh.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -27,6 +33,11 @@ struct Builder {
     uint32_t first_offset;
     WideTree &out;
     bool ok = true;
+    // per binary node: cover cost f(n, k) for k = 1..4 (index k - 1) and
+    // the left child's budget of the best split for k = 2..4 (0: n is
+    // its own cover, k >= 2) and for n's own wide node (index 0)
+    std::vector<std::array<float, 4>> cost;
+    std::vector<std::array<uint8_t, 4>> split;
 
     Item read(uint32_t i) const {
         Item it{};
@@ -106,23 +117,70 @@ struct Builder {
         return interior_node(it.bin, need);
     }
 
+    // The cover tables for the subtree of `root` (post order, iterative).
+    void plan(uint32_t root) {
+        cost.assign(n, std::array<float, 4>{0.0f, 0.0f, 0.0f, 0.0f});
+        split.assign(n, std::array<uint8_t, 4>{0, 0, 0, 0});
+        std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+        while (!st.empty()) {
+            auto [i, done] = st.back();
+            st.pop_back();
+            const Item it = read(i);
+            if (!it.interior || it.first + 1u >= n || it.first == 0) continue;  // leaf (or a malformed entry: no cost)
+            if (!done) {
+                st.push_back({i, true});
+                st.push_back({it.first + 1u, false});
+                st.push_back({it.first, false});
+                continue;
+            }
+            const auto &fa = cost[it.first], &fb = cost[it.first + 1u];
+            auto best_split = [&](int k, uint8_t &arg) {
+                float best = 0.0f;
+                arg = 0;
+                for (int i2 = 1; i2 < k; ++i2) {
+                    const float c = fa[i2 - 1] + fb[k - i2 - 1];
+                    if (arg == 0 || c < best) {
+                        best = c;
+                        arg = static_cast<uint8_t>(i2);
+                    }
+                }
+                return best;
+            };
+            auto &f = cost[i];
+            auto &sp = split[i];
+            f[0] = half_area(it) + best_split(4, sp[0]);
+            for (int k = 2; k <= 4; ++k) {
+                uint8_t arg;
+                const float g = best_split(k, arg);
+                if (g < f[0]) {
+                    f[k - 1] = g;
+                    sp[k - 1] = arg;
+                } else {
+                    f[k - 1] = f[0];
+                    sp[k - 1] = 0;
+                }
+            }
+        }
+    }
+    // Appends the cheapest cover of the subtree of `i` by at most k items.
+    void cover(uint32_t i, int k, std::vector<Item> &items) {
+        const Item it = read(i);
+        if (!it.interior || k == 1 || split[i][k - 1] == 0) {
+            items.push_back(it);
+            return;
+        }
+        const int a = split[i][k - 1];
+        cover(it.first, a, items);
+        cover(it.first + 1u, k - a, items);
+    }
+
     // Wide node for binary interior node `b`.
     uint32_t interior_node(uint32_t b, uint32_t &need) {
         Item root = read(b);
-        std::vector<Item> items{read(root.first), read(root.first + 1u)};
-        while (items.size() < 4) {
-            int best = -1;
-            float area = -1.0f;
-            for (size_t k = 0; k < items.size(); ++k)
-                if (items[k].interior && half_area(items[k]) > area) {
-                    area = half_area(items[k]);
-                    best = static_cast<int>(k);
-                }
-            if (best < 0) break;
-            uint32_t c = items[best].first;
-            items[best] = read(c);
-            items.insert(items.begin() + best + 1, read(c + 1u));
-        }
+        std::vector<Item> items;
+        const int a = split[b][0] ? split[b][0] : 2;  // 0: not planned (checked inputs always are)
+        cover(root.first, a, items);
+        cover(root.first + 1u, 4 - a, items);
         uint32_t node = alloc();
         uint32_t deepest = 0;
         for (size_t k = 0; k < items.size(); ++k) {
@@ -141,7 +199,7 @@ struct Builder {
 bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideTree &out) {
     out.words.clear();
     out.stack_need = 0;
-    Builder b{nodes, n, first_offset, out};
+    Builder b{nodes, n, first_offset, out, true, {}, {}};
     if (n == 0) {
         b.alloc();
         return true;
@@ -153,6 +211,7 @@ bool collapse_wide(const uint32_t *nodes, size_t n, uint32_t first_offset, WideT
     }
     uint32_t need = 0;
     if (root.interior) {
+        b.plan(0);
         b.interior_node(0, need);
     } else {
         uint32_t node = b.alloc();
@@ -230,6 +289,164 @@ void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxe
     std::vector<std::thread> pool;
     for (size_t k = 0; k < t; ++k) pool.emplace_back(run, n * k / t, n * (k + 1) / t);
     for (auto &th : pool) th.join();
+}
+
+namespace {
+
+struct SahBox {
+    float lo[3], hi[3];
+    void grow(const float *b) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], b[a]);
+            hi[a] = std::max(hi[a], b[3 + a]);
+        }
+    }
+    double half_area() const {
+        const double dx = static_cast<double>(hi[0]) - lo[0], dy = static_cast<double>(hi[1]) - lo[1],
+                     dz = static_cast<double>(hi[2]) - lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+constexpr SahBox kEmptyBox = {{3.402823466e38f, 3.402823466e38f, 3.402823466e38f},
+                              {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f}};
+constexpr size_t kSweepItems = 4096;  // above: 64 centroid bins per axis
+constexpr int kSahBins = 64;
+
+}  // namespace
+
+void sah_tree(const float *boxes, size_t n, std::vector<uint32_t> &nodes) {
+    nodes.clear();
+    if (n == 0) return;
+    std::vector<uint32_t> items(n);
+    std::vector<float> cen(3 * n);
+    for (size_t j = 0; j < n; ++j) {
+        items[j] = static_cast<uint32_t>(j);
+        for (int a = 0; a < 3; ++a) cen[3 * j + a] = 0.5f * boxes[6 * j + a] + 0.5f * boxes[6 * j + 3 + a];
+    }
+    auto put = [&](uint32_t node, const SahBox &b, uint32_t first, uint32_t count) {
+        uint32_t *w = nodes.data() + 8 * static_cast<size_t>(node);
+        std::memcpy(w, b.lo, 12);
+        std::memcpy(w + 4, b.hi, 12);
+        w[3] = first;
+        w[7] = count;
+    };
+    struct Task {
+        uint32_t node, begin, end;
+    };
+    nodes.assign(8, 0u);
+    std::vector<Task> st{{0u, 0u, static_cast<uint32_t>(n)}};
+    std::vector<double> right_area(n + 1);
+    while (!st.empty()) {
+        const Task t = st.back();
+        st.pop_back();
+        const uint32_t cnt = t.end - t.begin;
+        SahBox box = kEmptyBox;
+        for (uint32_t i = t.begin; i < t.end; ++i) box.grow(boxes + 6 * static_cast<size_t>(items[i]));
+        if (cnt == 1) {
+            put(t.node, box, items[t.begin], 1u);
+            continue;
+        }
+        // best split: left = the first k items in centroid order along an axis
+        int best_axis = -1;
+        uint32_t best_k = 0;
+        double best_cost = 0.0;
+        float best_plane = 0.0f;
+        if (cnt <= kSweepItems) {
+            std::vector<uint32_t> order(items.begin() + t.begin, items.begin() + t.end);
+            for (int a = 0; a < 3; ++a) {
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](uint32_t x, uint32_t y) { return cen[3 * x + a] < cen[3 * y + a]; });
+                SahBox r = kEmptyBox;
+                for (uint32_t k = cnt; k-- > 1;) {
+                    r.grow(boxes + 6 * static_cast<size_t>(order[k]));
+                    right_area[k] = r.half_area() * (cnt - k);
+                }
+                SahBox l = kEmptyBox;
+                for (uint32_t k = 1; k < cnt; ++k) {
+                    l.grow(boxes + 6 * static_cast<size_t>(order[k - 1]));
+                    const double c = l.half_area() * k + right_area[k];
+                    if (best_axis < 0 || c < best_cost) {
+                        best_axis = a;
+                        best_k = k;
+                        best_cost = c;
+                    }
+                }
+            }
+            std::stable_sort(items.begin() + t.begin, items.begin() + t.end,
+                             [&](uint32_t x, uint32_t y) { return cen[3 * x + best_axis] < cen[3 * y + best_axis]; });
+        } else {
+            SahBox cb = kEmptyBox;
+            for (uint32_t i = t.begin; i < t.end; ++i) {
+                const float *c = &cen[3 * static_cast<size_t>(items[i])];
+                const float b6[6] = {c[0], c[1], c[2], c[0], c[1], c[2]};
+                cb.grow(b6);
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float ext = cb.hi[a] - cb.lo[a];
+                if (!(ext > 0.0f)) continue;
+                SahBox bb[kSahBins];
+                uint32_t bc[kSahBins] = {};
+                for (int k = 0; k < kSahBins; ++k) bb[k] = kEmptyBox;
+                auto bin_of = [&](uint32_t it) {
+                    const int k = static_cast<int>((cen[3 * it + a] - cb.lo[a]) / ext * kSahBins);
+                    return std::min(std::max(k, 0), kSahBins - 1);
+                };
+                for (uint32_t i = t.begin; i < t.end; ++i) {
+                    const int k = bin_of(items[i]);
+                    bb[k].grow(boxes + 6 * static_cast<size_t>(items[i]));
+                    bc[k] += 1;
+                }
+                double ra[kSahBins];
+                uint32_t rc[kSahBins];
+                SahBox r = kEmptyBox;
+                uint32_t rn = 0;
+                for (int k = kSahBins - 1; k > 0; --k) {
+                    if (bc[k]) {
+                        const float *lo = bb[k].lo, *hi = bb[k].hi;
+                        const float b6[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+                        r.grow(b6);
+                    }
+                    rn += bc[k];
+                    ra[k] = rn ? r.half_area() : 0.0;
+                    rc[k] = rn;
+                }
+                SahBox l = kEmptyBox;
+                uint32_t ln = 0;
+                for (int k = 1; k < kSahBins; ++k) {
+                    if (bc[k - 1]) {
+                        const float *lo = bb[k - 1].lo, *hi = bb[k - 1].hi;
+                        const float b6[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+                        l.grow(b6);
+                    }
+                    ln += bc[k - 1];
+                    if (ln == 0 || rc[k] == 0) continue;
+                    const double c = l.half_area() * ln + ra[k] * rc[k];
+                    if (best_axis < 0 || c < best_cost) {
+                        best_axis = a;
+                        best_k = ln;
+                        best_cost = c;
+                        best_plane = cb.lo[a] + ext * static_cast<float>(k) / kSahBins;
+                    }
+                }
+            }
+            if (best_axis >= 0) {
+                const int a = best_axis;
+                auto mid = std::stable_partition(items.begin() + t.begin, items.begin() + t.end, [&](uint32_t it) {
+                    const float ext = cb.hi[a] - cb.lo[a];
+                    const int k = std::min(std::max(static_cast<int>((cen[3 * it + a] - cb.lo[a]) / ext * kSahBins), 0),
+                                           kSahBins - 1);
+                    return cb.lo[a] + ext * static_cast<float>(k) / kSahBins < best_plane;
+                });
+                best_k = static_cast<uint32_t>(mid - (items.begin() + t.begin));
+            }
+        }
+        if (best_axis < 0 || best_k == 0 || best_k >= cnt) best_k = cnt / 2;  // coincident centroids: halves
+        const uint32_t c0 = static_cast<uint32_t>(nodes.size() / 8);
+        nodes.resize(nodes.size() + 16, 0u);
+        put(t.node, box, c0, 0u);
+        st.push_back({c0 + 1u, t.begin + best_k, t.end});
+        st.push_back({c0, t.begin, t.begin + best_k});
+    }
 }
 
 void wide_levels(const WideTree &t, std::vector<uint32_t> &out) {

@@ -87,6 +87,16 @@ void instance_world_box(const float *m, const float *boxes, size_t n, float out[
 // cut range, on up to `threads` threads.
 void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxes, const uint32_t *cut_range,
                           size_t n, float *out, int threads);
+// Binary tree over n items with boxes boxes[6 j ..] (min xyz, max xyz) in
+// the node layout collapse_wide reads (8 words: min xyz, first child or
+// first item, max xyz, item count, 0 for interior nodes; children an
+// adjacent pair; node 0 the root), one item per leaf.  Splits minimise the
+// surface-area cost over the items' box centroids: a full sweep of the
+// centroid order along each axis up to kSweepItems items, 64 centroid bins
+// per axis above; coincident centroids split in halves.  The 4-wide
+// traversal's TLAS is collapsed from it (the reference's TLAS splits at the
+// centroid midpoint, tlas.rs:192).
+void sah_tree(const float *boxes, size_t n, std::vector<uint32_t> &nodes);
 // The nodes of a 4-wide tree grouped by depth, deepest first, for a
 // level-by-level refit: out = {n_levels, level_start[0 .. n_levels],
 // node indices}; level l holds out[1 + n_levels + 1 + level_start[l] ..
