@@ -449,6 +449,27 @@ void sah_tree(const float *boxes, size_t n, std::vector<uint32_t> &nodes) {
     }
 }
 
+bool wide_tlas(const uint32_t *tlas, const uint32_t *iidx, const float *inst_box, WideTree &out) {
+    std::vector<uint32_t> pos, st{0u};
+    while (!st.empty()) {
+        const uint32_t *w = tlas + 8 * static_cast<size_t>(st.back());
+        st.pop_back();
+        if (w[7] == 0) {
+            st.push_back(w[3] + 1u);
+            st.push_back(w[3]);
+        } else {
+            for (uint32_t c = 0; c < w[7]; ++c) pos.push_back(w[3] + c);
+        }
+    }
+    std::vector<float> pb(6 * pos.size());
+    for (size_t j = 0; j < pos.size(); ++j) std::memcpy(&pb[6 * j], inst_box + 6 * static_cast<size_t>(iidx[pos[j]]), 24);
+    std::vector<uint32_t> nodes;
+    sah_tree(pb.data(), pos.size(), nodes);
+    for (size_t i = 0; i < nodes.size() / 8; ++i)
+        if (nodes[8 * i + 7] != 0) nodes[8 * i + 3] = pos[nodes[8 * i + 3]];
+    return collapse_wide(nodes.data(), nodes.size() / 8, 0u, out);
+}
+
 void wide_levels(const WideTree &t, std::vector<uint32_t> &out) {
     const size_t n = t.words.size() / kWideNodeWords;
     std::vector<uint32_t> depth(n, 0u), frontier{0u};
@@ -473,41 +494,6 @@ void wide_levels(const WideTree &t, std::vector<uint32_t> &out) {
     for (uint32_t l = 0; l < levels; ++l) out[2 + l] = out[1 + l] + count[l];
     std::vector<uint32_t> fill(out.begin() + 1, out.begin() + 1 + levels);
     for (size_t i = 0; i < n; ++i) out[2 + levels + fill[deepest - depth[i]]++] = static_cast<uint32_t>(i);
-}
-
-void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box) {
-    const size_t n = t.words.size() / kWideNodeWords;
-    std::vector<float> nb(6 * n);
-    for (size_t i = n; i-- > 0;) {
-        uint32_t *w = t.words.data() + i * kWideNodeWords;
-        float box[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
-                        -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t e = w[24 + k];
-            if (e == kWideEmpty) continue;
-            float cb[6] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f,
-                           -3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
-            if (e & kWideLeafBit) {
-                const uint32_t first = e & ((1u << 26) - 1u), count = ((e >> 26) & 31u) + 1u;
-                for (uint32_t j = first; j < first + count; ++j) {
-                    const float *b = inst_box + 6 * static_cast<size_t>(iidx[j]);
-                    for (int a = 0; a < 3; ++a) {
-                        cb[a] = std::min(cb[a], b[a]);
-                        cb[3 + a] = std::max(cb[3 + a], b[3 + a]);
-                    }
-                }
-            } else {
-                std::memcpy(cb, &nb[6 * static_cast<size_t>(e)], sizeof(cb));
-            }
-            for (int a = 0; a < 3; ++a) {
-                std::memcpy(w + 8 * a + k, &cb[a], 4);
-                std::memcpy(w + 8 * a + 4 + k, &cb[3 + a], 4);
-                box[a] = std::min(box[a], cb[a]);
-                box[3 + a] = std::max(box[3 + a], cb[3 + a]);
-            }
-        }
-        std::memcpy(&nb[6 * i], box, sizeof(box));
-    }
 }
 
 bool quantize_wide(const WideTree &in, std::vector<uint32_t> &out) {

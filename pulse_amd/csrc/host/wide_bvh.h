@@ -92,20 +92,22 @@ void instance_world_boxes(const float *m, size_t m_stride, const float *cut_boxe
 // first item, max xyz, item count, 0 for interior nodes; children an
 // adjacent pair; node 0 the root), one item per leaf.  Splits minimise the
 // surface-area cost over the items' box centroids: a full sweep of the
-// centroid order along each axis up to kSweepItems items, 64 centroid bins
+// centroid order along each axis up to 4096 items, 64 centroid bins
 // per axis above; coincident centroids split in halves.  The 4-wide
 // traversal's TLAS is collapsed from it (the reference's TLAS splits at the
 // centroid midpoint, tlas.rs:192).
 void sah_tree(const float *boxes, size_t n, std::vector<uint32_t> &nodes);
+// The 4-wide traversal's TLAS: sah_tree over the instance_indices entries
+// that the leaves of the binary TLAS `tlas` (8 words per node, node 0 the
+// root, not the empty-scene sentinel; child and instance ranges checked)
+// hold -- the same instances, one leaf each -- bounded by
+// inst_box[6 * iidx[entry]], collapsed by collapse_wide.
+bool wide_tlas(const uint32_t *tlas, const uint32_t *iidx, const float *inst_box, WideTree &out);
 // The nodes of a 4-wide tree grouped by depth, deepest first, for a
 // level-by-level refit: out = {n_levels, level_start[0 .. n_levels],
 // node indices}; level l holds out[1 + n_levels + 1 + level_start[l] ..
 // level_start[l + 1]).  Every child lies in an earlier level than its parent.
 void wide_levels(const WideTree &t, std::vector<uint32_t> &out);
-// Recomputes the f32 child boxes of a 4-wide TLAS (`t`, collapsed from the
-// binary TLAS) from per-instance world boxes inst_box[6 * instance] over
-// instance_indices `iidx`, deepest nodes first.
-void refit_wide_tlas(WideTree &t, const uint32_t *iidx, const float *inst_box);
 
 // Leaf refinement of the 4-wide BLASes.  The reference's builder stops
 // splitting at 8 triangles or fewer (scene/blas.rs:97); the 4-wide

@@ -423,34 +423,6 @@ bool instance_records(const pulse_mesh_instance *inst, size_t n_inst, const uint
     return true;
 }
 
-// The 4-wide traversal's TLAS: a surface-area tree over the instance_indices
-// entries the reference's TLAS leaves hold (the same instances, one leaf
-// each), bounded by the instances' world boxes `ibox` (6 floats per
-// instance), collapsed to 4-wide nodes.  The reference's own TLAS splits at
-// the centroid midpoint (tlas.rs:192) and serves traversal loops 0 and 1.
-// `tlas` was walked and its ranges checked; iidx entries are below the
-// instance count.
-bool wide_tlas(const pulse_tlas_node *tlas, const uint32_t *iidx, const float *ibox, WideTree &out) {
-    std::vector<uint32_t> pos, st{0u};
-    while (!st.empty()) {
-        const pulse_tlas_node &nd = tlas[st.back()];
-        st.pop_back();
-        if (nd.instance_count == 0) {
-            st.push_back(nd.a_or_first_instance + 1u);
-            st.push_back(nd.a_or_first_instance);
-        } else {
-            for (uint32_t c = 0; c < nd.instance_count; ++c) pos.push_back(nd.a_or_first_instance + c);
-        }
-    }
-    std::vector<float> pb(6 * pos.size());
-    for (size_t j = 0; j < pos.size(); ++j) std::memcpy(&pb[6 * j], ibox + 6 * static_cast<size_t>(iidx[pos[j]]), 24);
-    std::vector<uint32_t> nodes;
-    sah_tree(pb.data(), pos.size(), nodes);
-    for (size_t i = 0; i < nodes.size() / 8; ++i)
-        if (nodes[8 * i + 7] != 0) nodes[8 * i + 3] = pos[nodes[8 * i + 3]];
-    return collapse_wide(nodes.data(), nodes.size() / 8, 0u, out);
-}
-
 int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     if (!d) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_upload: null argument");
     int rc = PULSE_OK;
@@ -598,7 +570,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     WideTree wtlas;
     std::vector<uint32_t> qtlas, wblas;
     bool has_wide = edges_small && (empty_tlas ? collapse_wide(reinterpret_cast<const uint32_t *>(tlas), n_tlas, 0u, wtlas)
-                                               : wide_tlas(tlas, iidx, ibox.data(), wtlas));
+                                               : wide_tlas(reinterpret_cast<const uint32_t *>(tlas), iidx, ibox.data(), wtlas));
     has_wide = has_wide && quantize_wide(wtlas, qtlas);
     if (cut_boxes.empty()) cut_boxes.assign(6, 0.0f);
     std::vector<uint32_t> wroot(std::max<size_t>(n_inst, 1), 0u);
@@ -799,7 +771,7 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
         std::vector<float> ibox(6 * n_inst);
         instance_world_boxes(inst[0].object_world, sizeof(pulse_mesh_instance) / sizeof(float), sd.cut_boxes.data(),
                              sd.cut_range.data(), n_inst, ibox.data(), kHostThreads);
-        has_wide = wide_tlas(tlas, iidx, ibox.data(), wtlas);
+        has_wide = wide_tlas(reinterpret_cast<const uint32_t *>(tlas), iidx, ibox.data(), wtlas);
     }
     has_wide = has_wide && quantize_wide(wtlas, qtlas);
     std::vector<uint32_t> wrec;

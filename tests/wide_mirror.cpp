@@ -121,11 +121,13 @@ std::vector<float> instance_boxes(const Scene &s) {
 
 bool build(const Scene &s, Layout &L) {
     WideTree t;
-    if (!pulse_host::collapse_wide(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u, t)) return false;
     const bool empty = s.tlas_nodes[0].a_or_first_instance == 0u && s.tlas_nodes[0].instance_count == 0u;
-    if (!empty) {
+    if (empty) {
+        if (!pulse_host::collapse_wide(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.n_tlas, 0u, t)) return false;
+    } else {
         const std::vector<float> ib = instance_boxes(s);
-        pulse_host::refit_wide_tlas(t, s.instance_indices, ib.data());
+        if (!pulse_host::wide_tlas(reinterpret_cast<const uint32_t *>(s.tlas_nodes), s.instance_indices, ib.data(), t))
+            return false;
     }
     if (!pulse_host::quantize_wide(t, L.tlas)) return false;
     uint32_t blas_need = 0;
@@ -316,11 +318,17 @@ int wide_mirror_check(const pulse_scene_desc *d, uint64_t *out) {
     auto check_tree = [&](const uint32_t *bin, size_t nb, uint32_t off, bool tlas) {
         WideTree t;
         std::vector<uint32_t> q, refined;
+        const bool empty_tlas = tlas && bin[3] == 0 && bin[7] == 0;
         if (!tlas) pulse_host::refine_leaves(bin, nb, off, rows.data(), pulse_host::kRefineLeaf, refs, refined, perm.data());
-        if (!pulse_host::collapse_wide(tlas ? bin : refined.data(), tlas ? nb : refined.size() / 8, off, t)) return false;
-        if (tlas && !(nb == 1 && bin[3] == 0 && bin[7] == 0)) {
+        if (tlas && !empty_tlas) {
+            // the library's TLAS: a surface-area tree over the same instances
             const std::vector<float> ib = instance_boxes(s);
-            pulse_host::refit_wide_tlas(t, s.instance_indices, ib.data());
+            if (!pulse_host::wide_tlas(bin, s.instance_indices, ib.data(), t)) return false;
+        } else if (!pulse_host::collapse_wide(tlas ? bin : refined.data(), tlas ? nb : refined.size() / 8, off, t)) {
+            return false;
+        }
+        if (tlas && !empty_tlas) {
+            const std::vector<float> ib = instance_boxes(s);
             for (size_t i = 0; i < s.n_instances; ++i) {
                 const float *b = &ib[6 * i];
                 // every triangle vertex of the instance, transformed, lies in its box
