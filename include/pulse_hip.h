@@ -559,7 +559,9 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
  * (device, n_materials*7 floats) is overwritten, or added to with
  * PULSE_FLAG_ADJOINT_ACCUMULATE.  `replay_rgba` (device,
  * optional) receives the replayed image, which equals the forward image bit
- * for bit.  Requires metallic == 0 for every material. */
+ * for bit.  Metallic materials are differentiated in their albedo (which
+ * moves f0 and with it the lobe probability; metallic itself is not a
+ * parameter). */
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params, const float *image,
                   const float *target, float *grad, float *replay_rgba);
 

@@ -558,28 +558,30 @@ struct DecisionTape {
     }
     bool take_bool(bool computed) { return take(computed ? 1u : 0u) != 0; }
     // Frozen half vectors (the specular lobe's sampled microfacet normal):
-    // RECORD_DIRS stores them with the sampling density q0 they were drawn
-    // from; FREEZE_DIRS replays them instead of mapping the random numbers
-    // through the current roughness, and reweights the sample by the
-    // likelihood ratio q(r)/q0 (importance reweighting keeps the estimator's
-    // expectation that of the perturbed parameters).  The roughness
-    // derivative of a path so replayed is the detached (score-function)
-    // estimator of the derivative of the expected image.
+    // RECORD_DIRS stores them with the density q0 they were drawn from
+    // within the lobe; FREEZE_DIRS replays them instead of mapping the
+    // random numbers through the current roughness, and reweights the
+    // sample by the likelihood ratio q(r)/q0 (importance reweighting keeps
+    // the estimator's expectation that of the perturbed parameters).  The
+    // roughness derivative of a path so replayed is the detached
+    // (score-function) estimator of the derivative of the expected image.
     enum DirMode { DIRS_OFF, RECORD_DIRS, FREEZE_DIRS };
     DirMode dir_mode = DIRS_OFF;
     std::vector<double> *dirs = nullptr;
     size_t dir_pos = 0;
-    // Frozen roulette: RECORD_RR stores each survival probability p =
-    // max(throughput); FREEZE_RR replays the decisions and treats a
-    // recorded p <= 1 (a random survival) as a decision sampled at the
-    // unperturbed parameters: the path takes the perturbed estimator's own
-    // p' and is reweighted by the likelihood ratio p' / p of its survival,
-    // so that its weight stays 1/p while the throughput later roulettes see
-    // is the perturbed estimator's (divided by p').  A recorded p > 1 (the
+    // Frozen roulette and lobe choices: RECORD_RR stores, in path order,
+    // each roulette survival probability p = max(throughput) and each
+    // GGX sample's lobe probability (F0max or 1 - F0max); FREEZE_RR
+    // replays the decisions and treats a recorded p <= 1 (a random
+    // survival) and a lobe choice as decisions sampled at the unperturbed
+    // parameters: the path takes the perturbed estimator's own p' and is
+    // reweighted by the likelihood ratio p' / p of the decision, so that
+    // its weight stays 1/p while the throughput later roulettes see is the
+    // perturbed estimator's (divided by p').  A recorded roulette p > 1 (the
     // path survives whatever the draw) follows the perturbed throughput.
-    // The product of the likelihood ratios so far (frozen directions and
-    // survivals) is `lr`: the throughput carries it, the roulette's p is
-    // that of the throughput without it.
+    // The product of the likelihood ratios so far (frozen directions,
+    // survivals and lobes) is `lr`: the throughput carries it, the
+    // roulette's p is that of the throughput without it.
     enum RrMode { RR_OFF, RECORD_RR, FREEZE_RR };
     RrMode rr_mode = RR_OFF;
     std::vector<double> *rr = nullptr;
@@ -794,17 +796,25 @@ SampleResult<R> importance_sample_ggx_d(V3<R> n, V3<R> wo, const MaterialR<R> &m
         pdf_d = R(kPI) * NdotI;
     }
     R pdf = ((R(1) - F0_max) * pdf_d) + (F0_max * pdf_s);
-    R ratio = R(1);  // likelihood ratio of a frozen specular sample
+    R ratio = R(1);  // likelihood ratio of a frozen specular sample and of the replayed lobe
     if (specular) {
         if (tape.dir_mode == DecisionTape::RECORD_DIRS) {
-            // wm was just sampled; pdf is the density it was drawn from
+            // wm was just sampled; pdf_s is the density it was drawn from
             tape.dirs->insert(tape.dirs->end(), {static_cast<double>(wm_out.x), static_cast<double>(wm_out.y),
-                                                 static_cast<double>(wm_out.z), static_cast<double>(pdf)});
+                                                 static_cast<double>(wm_out.z), static_cast<double>(pdf_s)});
         } else if (tape.dir_mode == DecisionTape::FREEZE_DIRS && tape.dir_pos + 4 <= tape.dirs->size()) {
-            ratio = pdf / static_cast<R>((*tape.dirs)[tape.dir_pos + 3]);
+            ratio = pdf_s / static_cast<R>((*tape.dirs)[tape.dir_pos + 3]);
             tape.dir_pos += 4;
             tape.lr *= static_cast<double>(ratio);
         }
+    }
+    const R p_lobe = specular ? F0_max : R(1) - F0_max;
+    if (tape.rr_mode == DecisionTape::RECORD_RR) {
+        tape.rr->push_back(static_cast<double>(p_lobe));
+    } else if (tape.rr_mode == DecisionTape::FREEZE_RR && tape.rr_pos < tape.rr->size()) {
+        const R lobe_ratio = p_lobe / static_cast<R>((*tape.rr)[tape.rr_pos++]);
+        ratio = ratio * lobe_ratio;
+        tape.lr *= static_cast<double>(lobe_ratio);
     }
     if (tape.take_bool(pdf < R(0.0001f))) pdf = R(1);
     return {wi, brdf / pdf * ratio, NdotO, NdotI_out};

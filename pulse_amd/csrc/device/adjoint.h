@@ -22,8 +22,12 @@
 // its score d ln p is added, which cancels 1/p's own derivative in every
 // later contribution (E[1{u < p} / p] = 1) but not in later roulettes: a
 // later p > 1 is measured on the throughput divided by this p, so its
-// 1/p' carries this p's derivative back.  Metallic materials are rejected
-// for the adjoint (their lobe probability would depend on the albedo).
+// 1/p' carries this p's derivative back.
+//
+// The GGX lobe choice is a decision of the same kind (probability P = F0max
+// or 1 - F0max, which follows the albedo of a metallic material and, with
+// moving directions, the roughness): the weight holds P fixed, the terms
+// that do not cancel come in like those of a random survival.
 //
 // Reverse sweep: with a_k = dL/d ln(beta_k) per channel (beta_k the
 // throughput reaching vertex k) and S_k the suffix sum of vertex
@@ -220,7 +224,10 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             // stored vertex values for the reverse sweep
             V3 beta[MAXV], cvert[MAXV], wvert[MAXV];
             V3 dDdb[MAXV], dDdLe[MAXV], dDdr[MAXV];
-            float dwdb[MAXV], dlogw[MAXV], dlogq[MAXV];
+            float dwdb[MAXV], dlogw[MAXV], dlogq[MAXV], dlogp[MAXV];
+            // 3 bits per vertex: the channel of F0max at the lobe choice | 4 if
+            // the pdf floor replaced the density
+            uint32_t lobe = 0;
             uint32_t mat[MAXV], lmat[MAXV];
             bool vis[MAXV];
             // 4 bits per vertex: 1 + the channel of the maximum at a survived
@@ -291,6 +298,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 dwdb[nv] = 0.0f;
                 dlogw[nv] = 0.0f;
                 dlogq[nv] = 0.0f;
+                dlogp[nv] = 0.0f;
                 const bool last = depth + 1u >= a.max_depth;
                 if (!last) {
                     SampleT<S> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
@@ -300,9 +308,16 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     }
                     T = mul(T, smp.reflectance);
                     wvert[nv] = val3(smp.reflectance);
-                    dwdb[nv] = smp.diffuse ? smp.dbase : 0.0f;
+                    dwdb[nv] = smp.dbase;
                     dlogw[nv] = smp.dlogf_dr;
                     dlogq[nv] = smp.dlogq_dr;
+                    dlogp[nv] = smp.dlogp_db;
+                    lobe |= (smp.lobe_c0 | (smp.floored ? 4u : 0u)) << (3 * nv);
+                    if constexpr (!DETACHED) {
+                        // the weight holds the lobe probability P fixed: the
+                        // throughput with every factor varying has d ln P less
+                        if (val(smp.lobe_p) > 0.0f) nrm = nrm + (1.0f - smp.lobe_p / val(smp.lobe_p));
+                    }
                     if (depth >= a.rr_start) {
                         // the forward's roulette (shade.h) on the replayed
                         // throughput: the same draw, the same end of the
@@ -384,9 +399,21 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     }
                     // d ln w_k / d base = dwb / w_k per channel; F is nonzero only
                     // in channels whose w_k is positive (they hold a later p > 1)
-                    const float ax = gx * b.x * (dDdb[k].x + Sx * dwb) + (Fx != 0.0f ? Fx * dwb / wk.x : 0.0f);
-                    const float ay = gy * b.y * (dDdb[k].y + Sy * dwb) + (Fy != 0.0f ? Fy * dwb / wk.y : 0.0f);
-                    const float az = gz * b.z * (dDdb[k].z + Sz * dwb) + (Fz != 0.0f ? Fz * dwb / wk.z : 0.0f);
+                    float ax = gx * b.x * (dDdb[k].x + Sx * dwb) + (Fx != 0.0f ? Fx * dwb / wk.x : 0.0f);
+                    float ay = gy * b.y * (dDdb[k].y + Sy * dwb) + (Fy != 0.0f ? Fy * dwb / wk.y : 0.0f);
+                    float az = gz * b.z * (dDdb[k].z + Sz * dwb) + (Fz != 0.0f ? Fz * dwb / wk.z : 0.0f);
+                    if (dlogp[k] != 0.0f) {
+                        // the lobe probability (metallic: it follows base_c0): -d ln P
+                        // in the weight reaches the 1/p terms only; where the
+                        // pdf floor left P out of the weight, the choice's score
+                        // reaches every later contribution
+                        const uint32_t lb = (lobe >> (3 * k)) & 7u;
+                        const float t = (lb & 4u) ? (gx * bn.x * Sx + gy * bn.y * Sy + gz * bn.z * Sz) * dlogp[k]
+                                                  : -(Fx + Fy + Fz) * dlogp[k];
+                        ax += (lb & 3u) == 0u ? t : 0.0f;
+                        ay += (lb & 3u) == 1u ? t : 0.0f;
+                        az += (lb & 3u) == 2u ? t : 0.0f;
+                    }
                     if (ax == ax && ay == ay && az == az) {
                         kb = m * kGradPerMaterial;
                         v4[0] = ax;

@@ -562,7 +562,7 @@ template <class S>
 struct SampleT {
     V3T<S> wi, reflectance;
     bool diffuse;     // lobe taken
-    float dbase;      // d reflectance_c / d base_c (diffuse lobe; 0 for specular)
+    float dbase;      // d reflectance_c / d base_c at P held (specular: through f0, 0 unless metallic)
     float NdotO, NdotI;  // values, for the optional horizon clamp
     // Detached roughness derivative of the sampling weight w = brdf / pdf
     // with the sample held fixed and the density it was drawn from, q,
@@ -574,6 +574,18 @@ struct SampleT {
     // d ln q / d roughness at the fixed sample (specular: d ln D; diffuse: 0),
     // so that d ln w = dlogf_dr - dlogq_dr.
     float dlogq_dr;
+    // The lobe choice (probability P = F0max for the specular lobe, 1 -
+    // F0max for the diffuse one) is a decision: the weight divides by P held
+    // at its value (P's own tangent is dropped: with the choice's score it
+    // cancels in every later contribution), `lobe_p` keeps P with its
+    // tangent for what does not cancel; d ln P / d base_c0 (c0 the channel
+    // of F0max; nonzero for metallic materials only) and whether the pdf
+    // floor replaced the density (then the weight holds no P and the
+    // score stays).
+    S lobe_p;
+    float dlogp_db;
+    uint32_t lobe_c0;
+    bool floored;
 };
 
 // d ln D_GGX / d roughness at a fixed microfacet normal (D = a2 / (pi b^2),
@@ -612,7 +624,7 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
     S pdf_s = sconst<S>(0.0f), pdf_d = sconst<S>(0.0f);
     SampleT<S> out;
     out.diffuse = !(e0 <= val(F0_max));
-    float dbase_num = 0.0f, dlog_d = 0.0f;
+    float dbase_num = 0.0f, dlog_d = 0.0f, dspec = 0.0f;
     out.NdotO = val(NdotO);
     out.dlogf_dr = 0.0f;
     if (!out.diffuse) {
@@ -652,6 +664,8 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         out.NdotI = val(NdotI);
         dlog_d = dlog_d_dr(val(m.roughness), val(NdotM));
         out.dlogf_dr = dlog_d + dlog_g_dr(val(m.roughness), val(NdotO), val(NdotI));
+        // d brdf_c / d base_c = metallic (1 - (1 - o.m)^5) D G / denom (through f0)
+        if (m.metallic != 0.0f) dspec = m.metallic * (1.0f - pt_pow5f(1.0f - val(OdotM))) * (val(D) * val(G) / val(denom));
     } else {
         wi = sample_cosine_hemisphere(n, e1, e2);
         S NdotI = dot(n, wi);
@@ -660,15 +674,26 @@ __device__ __forceinline__ SampleT<S> importance_sample_ggx_d(const V3T<S> &n, c
         pdf_d = kPI * NdotI;
         dbase_num = val(kPI * NdotI);
     }
-    S pdf = ((1.0f - F0_max) * pdf_d) + (F0_max * pdf_s);
-    if (val(pdf) < 0.0001f) {
+    const S F0_held = sconst<S>(val(F0_max));
+    S pdf = ((1.0f - F0_held) * pdf_d) + (F0_held * pdf_s);
+    out.floored = val(pdf) < 0.0001f;
+    if (out.floored) {
         pdf = sconst<S>(1.0f);
         out.dlogf_dr += dlog_d;
     }
     out.dlogq_dr = dlog_d;
     out.wi = wi;
     out.reflectance = divs(brdf, pdf);
-    out.dbase = dbase_num / val(pdf);
+    out.lobe_p = out.diffuse ? 1.0f - F0_max : F0_max;
+    if constexpr (!std::is_same<S, float>::value) {
+        // floored: the weight holds no P, the choice's score d ln P stays (value x1)
+        if (out.floored && val(out.lobe_p) > 0.0f) out.reflectance = scale(out.lobe_p / val(out.lobe_p), out.reflectance);
+    }
+    out.dbase = (out.diffuse ? dbase_num : dspec) / val(pdf);
+    const float F0m = val(F0_max);
+    out.lobe_c0 = val(F0.x) == F0m ? 0u : (val(F0.y) == F0m ? 1u : 2u);
+    const float dF0 = m.metallic * (1.0f - pt_pow5f(1.0f - val(NdotO)));
+    out.dlogp_db = m.metallic == 0.0f ? 0.0f : (out.diffuse ? -dF0 / (1.0f - F0m) : dF0 / F0m);
     return out;
 }
 
@@ -699,7 +724,7 @@ struct NeeSampleT {
     V3T<S> origin, dir;  // shadow ray
     S threshold;         // occluded iff closest t < threshold
     V3T<S> contrib;      // direct light if unoccluded (before * throughput)
-    V3 d_base;           // d contrib_c / d base_c (Lambert evaluator)
+    V3 d_base;           // d contrib_c / d base_c (Lambert evaluator; GGX through f0 when metallic)
     V3 d_emissive;       // d contrib_c / d emissive_c of the light's material
     V3 d_rough;          // d contrib_c / d roughness at fixed light sample (GGX evaluator)
     uint32_t light_material;
@@ -755,7 +780,7 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
     S cos_r = dot(ns.dir, n0);
     S cos_e = dot(neg(ns.dir), nl);
     V3T<S> brdf;
-    float dlog_r = 0.0f;
+    float dlog_r = 0.0f, dspec = 0.0f;
     if (!ggx) {
         brdf = lift<S>(mk(m.base.x * kINV_PI, m.base.y * kINV_PI, m.base.z * kINV_PI));
     } else {
@@ -772,6 +797,8 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
         S denom = 4.0f * NdotO * NdotI;
         brdf = {F.x * D * G / denom, F.y * D * G / denom, F.z * D * G / denom};
         dlog_r = dlog_d_dr(val(m.roughness), val(NdotM)) + dlog_g_dr(val(m.roughness), val(NdotO), val(NdotI));
+        // d brdf_c / d base_c through f0 (metallic materials)
+        if (m.metallic != 0.0f) dspec = m.metallic * (1.0f - pt_pow5f(1.0f - val(OdotM))) * (val(D) * val(G) / val(denom));
     }
     ns.light_material = mi.material_index;
     V3 le = ld3(sc.materials[mi.material_index].emissive);
@@ -788,9 +815,9 @@ __device__ __forceinline__ NeeSampleT<S> nee_sample(const DevScene &sc, const V3
                        val(dl.z) > 0.0f ? val(brdf.z) * g : 0.0f);
     ns.d_rough = mk(val(dl.x) > 0.0f ? val(dl.x) * dlog_r : 0.0f, val(dl.y) > 0.0f ? val(dl.y) * dlog_r : 0.0f,
                     val(dl.z) > 0.0f ? val(dl.z) * dlog_r : 0.0f);
-    ns.d_base = ggx ? mk(0.0f, 0.0f, 0.0f)
-                    : mk(val(dl.x) > 0.0f ? kINV_PI * le.x * g : 0.0f, val(dl.y) > 0.0f ? kINV_PI * le.y * g : 0.0f,
-                         val(dl.z) > 0.0f ? kINV_PI * le.z * g : 0.0f);
+    const float db = ggx ? dspec : kINV_PI;
+    ns.d_base = mk(val(dl.x) > 0.0f ? db * le.x * g : 0.0f, val(dl.y) > 0.0f ? db * le.y * g : 0.0f,
+                   val(dl.z) > 0.0f ? db * le.z * g : 0.0f);
     return ns;
 }
 using NeeSample = NeeSampleT<float>;

@@ -80,7 +80,6 @@ struct SceneData {
     uint32_t n_strength_cdf = 0;  // entries of light_emission_strength_cdf
     bool fast_div = false;       // node coordinates allow the exact fast division
     bool fast_div_blas = false;  // the BLAS part of fast_div
-    bool any_metallic = false;   // some material has metallic != 0 (no adjoint)
     bool valid = false;
     uint64_t generation = 0;     // bumped by every upload or instance update
     // host copies checked by instance updates: per instance {triangle_offset,

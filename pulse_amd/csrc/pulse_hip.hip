@@ -689,9 +689,6 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     sd.n_strength_cdf = static_cast<uint32_t>(n_scdf);
     sd.stack_need = need;
     sd.n_materials = static_cast<uint32_t>(n_mat);
-    sd.any_metallic = false;
-    for (size_t k = 0; k < n_mat; ++k)
-        if (static_cast<const pulse_material *>(d->materials.data)[k].metallic != 0.0f) sd.any_metallic = true;
     sd.fast_div = fast_div;
     sd.fast_div_blas = fast_div_blas;
     sd.tlas_depth = tlas_depth;
@@ -902,7 +899,6 @@ int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d", who, kAdjMaxVertices);
     if (prm->accumulation_count != 0 || prm->spp == 0)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs accumulation_count 0 and spp >= 1", who);
-    if (ctx->cur->any_metallic) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: materials with metallic != 0", who);
     if (prm->flags & PULSE_FLAG_LIGHT_POWER)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_LIGHT_POWER (the light pdf depends on emission)", who);
     if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: BVH too deep for the adjoint stack", who);
@@ -1139,9 +1135,6 @@ int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (count) HIP_TRY(hipMemcpy(ctx->cur->buf[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
-    ctx->cur->any_metallic = false;
-    for (uint32_t k = 0; k < count; ++k)
-        if (materials[k].metallic != 0.0f) ctx->cur->any_metallic = true;
     return PULSE_OK;
 }
 

@@ -112,6 +112,26 @@ def test_roulette_replay_and_oracle_gradient(dev, extra_flags, frozen, loop):
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
 
 
+@pytest.mark.parametrize("loop", [1, 2])
+@pytest.mark.parametrize("extra_flags,frozen", MODES)
+def test_metallic_replay_and_oracle_gradient(dev, extra_flags, frozen, loop):
+    """Metallic walls (0.6) and boxes (1.0), roulette from the first bounce:
+    the albedo moves f0, so the GGX weights, the GGX light evaluator and
+    the lobe probability F0max; the gradient equals the oracle's central
+    differences along the same decisions, the lobe choice reweighted by its
+    likelihood ratio like a random survival."""
+    scene, cam, prm = _cornell_case(depth=4, rr=1)
+    mats = scene.buffers["materials"]
+    mats["metallic"][0] = 0.6
+    mats["metallic"][1] = 1.0
+    prm.flags |= extra_flags
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        _replay_case(dev, scene, cam, prm, frozen)
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
 def test_roulette_deep_paths(dev):
     scene, cam, prm = _cornell_case(depth=8, rr=2)
     _replay_case(dev, scene, cam, prm, True)
