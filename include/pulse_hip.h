@@ -550,7 +550,9 @@ int pulse_trace_batch(pulse_ctx *ctx, const pulse_view *views, uint32_t n_views,
 int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, uint32_t npix, double *loss);
 
 /* dL/dparameters of the image rendered by pulse_trace with the same camera
- * and params (accumulation_count 0, params->spp samples, mode
+ * and params: samples 0 .. accumulation_count + spp - 1 of every pixel,
+ * i.e. the image pulse_trace accumulates over calls with these parameters
+ * (accumulation_count 0 for a single call); mode
  * PULSE_MODE_GGX_NEE, max_depth <= 8 -- <= 4 with
  * PULSE_FLAG_ADJOINT_REPARAM --, any rr_start_depth: Russian roulette is
  * replayed with its decisions; a random survival (p <= 1) contributes its
@@ -561,7 +563,8 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
  * optional) receives the replayed image, which equals the forward image bit
  * for bit.  Metallic materials are differentiated in their albedo (which
  * moves f0 and with it the lobe probability; metallic itself is not a
- * parameter). */
+ * parameter).  PULSE_FLAG_LIGHT_POWER is allowed: the light pick's pdf is
+ * held at its value (the expected light sample does not depend on it). */
 int pulse_adjoint(pulse_ctx *ctx, const pulse_camera *camera, const pulse_params *params, const float *image,
                   const float *target, float *grad, float *replay_rgba);
 

@@ -24,6 +24,12 @@
 // later p > 1 is measured on the throughput divided by this p, so its
 // 1/p' carries this p's derivative back.
 //
+// The light picked for next event estimation is a decision too; its
+// probability (uniform, or by emission strength with
+// PULSE_FLAG_LIGHT_POWER) divides the light sample's contribution, which
+// ends there: E over the pick of contribution / P_pick does not depend on
+// P_pick, so the pdf is held at its value.
+//
 // The GGX lobe choice is a decision of the same kind (probability P = F0max
 // or 1 - F0max, which follows the albedo of a metallic material and, with
 // moving directions, the roughness): the weight holds P fixed, the terms
@@ -79,7 +85,8 @@ struct AdjArgs {
     uint32_t n_views;
     uint32_t max_depth;
     uint32_t nee_ggx;
-    uint32_t spp;
+    uint32_t spp;          // samples replayed per pixel: 0 .. spp - 1 (accumulation_count + the call's spp)
+    uint32_t light_power;  // lights picked by emission strength (its pdf is held: see below)
     float *grad;           // n_materials * 7, accumulated into
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
@@ -276,7 +283,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 dDdr[nv] = mk(0.0f, 0.0f, 0.0f);
                 lmat[nv] = 0;
                 if (sc.light_count > 0u) {
-                    NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng);
+                    NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng, a.light_power != 0u);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
                     bool occ = trace_any<LOOP, true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,

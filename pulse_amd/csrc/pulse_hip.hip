@@ -897,10 +897,8 @@ int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be 1..%d", who, kAdjMaxVerticesDeep);
     if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d", who, kAdjMaxVertices);
-    if (prm->accumulation_count != 0 || prm->spp == 0)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs accumulation_count 0 and spp >= 1", who);
-    if (prm->flags & PULSE_FLAG_LIGHT_POWER)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_LIGHT_POWER (the light pdf depends on emission)", who);
+    if (prm->spp == 0 || prm->spp > 0xffffffffu - prm->accumulation_count)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs spp >= 1 and accumulation_count + spp below 2^32", who);
     if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: BVH too deep for the adjoint stack", who);
     return PULSE_OK;
 }
@@ -918,7 +916,8 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
     AdjArgs a{};
     a.max_depth = prm->max_depth;
     a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
-    a.spp = prm->spp;
+    a.spp = prm->accumulation_count + prm->spp;  // every sample the accumulated image holds
+    a.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
     a.grad = grad;
     a.n_materials = ctx->cur->n_materials;
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;

@@ -151,12 +151,17 @@ def test_emission_euler_identity_gpu(dev):
     assert abs(euler - 2 * loss) <= 2e-4 * loss
 
 
+@pytest.mark.parametrize("light_power", [0, N.FLAG_LIGHT_POWER], ids=["uniform_lights", "light_power"])
 @pytest.mark.parametrize("extra_flags,frozen", MODES)
-def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen):
+def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen, light_power):
+    """The field's five emitters differ in area, so picking them by emission
+    strength (PULSE_FLAG_LIGHT_POWER) gives each its own pick pdf; the
+    replay holds it at its value, as does the oracle (its strength table is
+    the scene's)."""
     scene, camf = P.icosphere_field(n_meshes=6, subdiv=3, n_instances=40, n_emissive=5, box=24.0, radius=(1.0, 3.0))
     cam = camf(64, 40)
     prm = make_params(spp=2, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3,
-                      flags=N.FLAG_HORIZON_CLAMP | extra_flags)
+                      flags=N.FLAG_HORIZON_CLAMP | extra_flags | light_power)
     dev.upload_scene(scene)
     nm = len(scene.buffers["materials"])
     adj = PulseAdjoint(dev, cam.width, cam.height, nm)
@@ -175,8 +180,42 @@ def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen):
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
         adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_GGX, rr_start_depth=3))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
-        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=3,
-                                      flags=N.FLAG_HORIZON_CLAMP | N.FLAG_LIGHT_POWER))
+        adj.backward(cam, make_params(spp=2, max_depth=9, mode=N.MODE_GGX_NEE, rr_start_depth=3,
+                                      flags=N.FLAG_HORIZON_CLAMP))
+
+
+@pytest.mark.parametrize("loop", [1, 2])
+def test_accumulated_image_gradient(dev, loop):
+    """Three pulse_trace calls accumulate samples 0-1, 2-3 and 4-5 into one
+    image; the adjoint with the last call's parameters (accumulation_count
+    4, spp 2) replays all six samples: its replay equals the accumulated
+    image bit for bit and its gradient equals the oracle's central
+    differences of the 6-sample image along the same decisions."""
+    scene, cam, prm = _cornell_case(spp=2, depth=3, rr=1)
+    dev.upload_scene(scene)
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        nm = len(scene.buffers["materials"])
+        adj = PulseAdjoint(dev, cam.width, cam.height, nm)
+        for acc in (0, 2, 4):
+            prm.accumulation_count = acc
+            img = adj.forward(cam, prm)
+        target = (img * np.float32(0.6) + np.float32(0.05)).astype(np.float32)
+        adj.set_target(target)
+        g, replay = adj.backward(cam, prm, want_replay=True)
+        assert same_bits(replay, img).all()
+        whole = make_params(spp=6, max_depth=3, mode=N.MODE_GGX_NEE, rr_start_depth=1, flags=N.FLAG_HORIZON_CLAMP)
+        o = OracleScene(scene)
+        oimg, _, _ = o.render(cam, whole)
+        assert same_bits(oimg, img).all()
+        mp = material_params(scene.buffers["materials"])
+        gref, _ = o.grad_fd(cam, whole, target, mp.reshape(-1), np.arange(mp.size, dtype=np.uint32), h=1e-6,
+                            frozen=True)
+        gref = gref.reshape(nm, 7)
+        scale = np.abs(gref).max()
+        assert (np.abs(g - gref) <= 2e-3 * np.abs(gref) + 1e-4 * scale).all(), (g, gref)
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
 
 
 def _field_fd_case():
