@@ -1,6 +1,11 @@
 // adjoint.h -- replay adjoint of the L2 image loss with respect to per-
 // material albedo (base_color.rgb), GGX roughness and emissive.rgb.
 //
+// The paths are those of PULSE_MODE_GGX_NEE (emission at the first hit,
+// light sampling at every hit) or of the reference's own path_trace,
+// PULSE_MODE_REF_GGX (path_tracer.wgsl:58-124: emission at every hit, no
+// light sampling).
+//
 // Nothing of the forward paths is stored: each path is replayed from its
 // seed (the pixel index and accumulation count, path_tracer.wgsl:60-61) with
 // the same f32 traversal, so it takes exactly the forward decisions.  Along
@@ -87,6 +92,7 @@ struct AdjArgs {
     uint32_t nee_ggx;
     uint32_t spp;          // samples replayed per pixel: 0 .. spp - 1 (accumulation_count + the call's spp)
     uint32_t light_power;  // lights picked by emission strength (its pdf is held: see below)
+    uint32_t ref_ggx;      // PULSE_MODE_REF_GGX: emission at every hit, no light sampling
     float *grad;           // n_materials * 7, accumulated into
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 V3 ck = mk(0.0f, 0.0f, 0.0f);
                 beta[nv] = val3(T);
                 mat[nv] = h.material_index;
-                if (depth == 0) {
+                if (depth == 0 || a.ref_ggx) {
                     C = add(C, mul(T, m.emissive));
                     ck = m.emissive;
                 }
@@ -282,7 +288,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 dDdLe[nv] = mk(0.0f, 0.0f, 0.0f);
                 dDdr[nv] = mk(0.0f, 0.0f, 0.0f);
                 lmat[nv] = 0;
-                if (sc.light_count > 0u) {
+                if (sc.light_count > 0u && !a.ref_ggx) {
                     NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng, a.light_power != 0u);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
@@ -452,8 +458,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                             e3[2] = ez;
                         }
                     }
-                    if (k == 0) {
-                        // emission seen directly: dC/dLe of the first hit = 1
+                    if (k == 0 || a.ref_ggx) {
+                        // emission seen directly (the first hit; every hit in
+                        // the reference's path_trace): dC/dLe = the throughput
                         const float ex = gx * b.x, ey = gy * b.y, ez = gz * b.z;
                         if (ex == ex && ey == ey && ez == ez) {
                             k0 = m * kGradPerMaterial + 4;
@@ -473,7 +480,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 }
                 grad_add_wave(gbuf, kb, v4, rounds);
                 grad_add_wave(gbuf, ke, e3, rounds);
-                if (k == 0) grad_add_wave(gbuf, k0, d3, rounds);
+                if (k == 0 || a.ref_ggx) grad_add_wave(gbuf, k0, d3, rounds);
             }
             if (vw.replay) {
                 // running mean of path_tracer.wgsl:116-123 over the replayed samples

@@ -892,7 +892,8 @@ void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds, uint32_t grid)
 int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "%s: no scene uploaded", who);
     if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
-    if (prm->mode != PULSE_MODE_GGX_NEE) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs PULSE_MODE_GGX_NEE", who);
+    if (prm->mode != PULSE_MODE_GGX_NEE && prm->mode != PULSE_MODE_REF_GGX)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs PULSE_MODE_GGX_NEE or PULSE_MODE_REF_GGX", who);
     if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep))
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be 1..%d", who, kAdjMaxVerticesDeep);
     if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
@@ -918,6 +919,7 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
     a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
     a.spp = prm->accumulation_count + prm->spp;  // every sample the accumulated image holds
     a.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
+    a.ref_ggx = prm->mode == PULSE_MODE_REF_GGX ? 1u : 0u;
     a.grad = grad;
     a.n_materials = ctx->cur->n_materials;
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;

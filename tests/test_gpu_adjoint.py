@@ -132,6 +132,21 @@ def test_metallic_replay_and_oracle_gradient(dev, extra_flags, frozen, loop):
         dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
 
 
+@pytest.mark.parametrize("extra_flags,frozen,depth", [
+    pytest.param(0, True, 4, id="detached-4"), pytest.param(0, True, 5, id="detached-5"),
+    pytest.param(0, True, 8, id="detached-8"), pytest.param(N.FLAG_ADJOINT_REPARAM, False, 4, id="reparam-4")])
+def test_reference_path_trace_gradient(dev, extra_flags, frozen, depth):
+    """The reference's own path_trace (PULSE_MODE_REF_GGX: emission at every
+    hit, no light sampling; 5 bounces by default), roulette from the first
+    bounce: replay == forward bit for bit, gradient == the oracle's central
+    differences along the same decisions (reparameterised roughness up to 4
+    bounces)."""
+    scene, cam, prm = _cornell_case(depth=depth, rr=1)
+    prm.mode = N.MODE_REF_GGX
+    prm.flags |= extra_flags
+    _replay_case(dev, scene, cam, prm, frozen)
+
+
 def test_roulette_deep_paths(dev):
     scene, cam, prm = _cornell_case(depth=8, rr=2)
     _replay_case(dev, scene, cam, prm, True)
@@ -178,7 +193,7 @@ def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen, light
     print("field: max err / scale", float((np.abs(g - gref) / scale).max()), flush=True)
     assert (np.abs(g - gref) <= 2e-3 * np.abs(gref) + 1e-4 * scale).all()
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
-        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_GGX, rr_start_depth=3))
+        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_NEE_LAMBERT, rr_start_depth=3))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
         adj.backward(cam, make_params(spp=2, max_depth=9, mode=N.MODE_GGX_NEE, rr_start_depth=3,
                                       flags=N.FLAG_HORIZON_CLAMP))
