@@ -552,9 +552,11 @@ int pulse_image_loss(pulse_ctx *ctx, const float *image, const float *target, ui
 /* dL/dparameters of the image rendered by pulse_trace with the same camera
  * and params: samples 0 .. accumulation_count + spp - 1 of every pixel,
  * i.e. the image pulse_trace accumulates over calls with these parameters
- * (accumulation_count 0 for a single call); mode PULSE_MODE_GGX_NEE or
- * PULSE_MODE_REF_GGX (the reference's path_trace), max_depth <= 8 -- <= 4 with
- * PULSE_FLAG_ADJOINT_REPARAM --, any rr_start_depth: Russian roulette is
+ * (accumulation_count 0 for a single call); any mode (PULSE_MODE_GGX_NEE,
+ * the reference's path_trace PULSE_MODE_REF_GGX, or its path_traceBBB
+ * PULSE_MODE_REF_NEE_LAMBERT, whose first hit takes one vertex more); at
+ * most 8 path vertices -- max_depth <= 8, <= 7 in the Lambert mode; 4 / 3
+ * with PULSE_FLAG_ADJOINT_REPARAM --, any rr_start_depth: Russian roulette is
  * replayed with its decisions; a random survival (p <= 1) contributes its
  * score, a certain one (p > 1) the derivative of its 1/p).
  * Replays every path from its seed; `grad`

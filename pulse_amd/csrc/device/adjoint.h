@@ -2,9 +2,11 @@
 // material albedo (base_color.rgb), GGX roughness and emissive.rgb.
 //
 // The paths are those of PULSE_MODE_GGX_NEE (emission at the first hit,
-// light sampling at every hit) or of the reference's own path_trace,
+// light sampling at every hit), of the reference's own path_trace,
 // PULSE_MODE_REF_GGX (path_tracer.wgsl:58-124: emission at every hit, no
-// light sampling).
+// light sampling), or of its path_traceBBB, PULSE_MODE_REF_NEE_LAMBERT
+// (path_tracer.wgsl:128-202: cosine sampling, Lambert light sampling,
+// emission at the first hit; no roughness dependence).
 //
 // Nothing of the forward paths is stored: each path is replayed from its
 // seed (the pixel index and accumulation count, path_tracer.wgsl:60-61) with
@@ -88,11 +90,12 @@ struct AdjView {
 struct AdjArgs {
     AdjView views[kAdjMaxViews];
     uint32_t n_views;
-    uint32_t max_depth;
+    uint32_t vertices;     // path vertices: max_depth (GGX modes) or max_depth + 1 (Lambert: the first hit and max_depth bounces)
     uint32_t nee_ggx;
     uint32_t spp;          // samples replayed per pixel: 0 .. spp - 1 (accumulation_count + the call's spp)
     uint32_t light_power;  // lights picked by emission strength (its pdf is held: see below)
     uint32_t ref_ggx;      // PULSE_MODE_REF_GGX: emission at every hit, no light sampling
+    uint32_t lambert;      // PULSE_MODE_REF_NEE_LAMBERT: cosine sampling, weight = albedo
     float *grad;           // n_materials * 7, accumulated into
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
@@ -187,7 +190,7 @@ __device__ __forceinline__ void grad_add_wave(float *buf, uint32_t key, const fl
 // false, tangents through the moving directions) does not capture.
 // LOOP: the traversal of the forward pass being replayed (1: the binary
 // BVHs, 2: the 4-wide ones), so every replayed path is the forward's.
-// MAXV: path vertices the per-vertex arrays hold (max_depth at most MAXV).
+// MAXV: path vertices the per-vertex arrays hold (a.vertices at most MAXV).
 template <int BLOCK, int STACK, bool DETACHED, int LOOP = 1, int MAXV = kAdjMaxVertices>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
     static_assert(BLOCK == kAdjBlockPixels, "a block replays 128 pixels");
@@ -212,7 +215,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
     const bool live = p < vw.npix;
     const uint32_t pixel = live ? p : 0u;
     const uint32_t x = pixel % W, y = pixel / W;
-    const uint32_t depth_end = live ? a.max_depth : 0u;
+    const uint32_t depth_end = live ? a.vertices : 0u;
     const uint32_t rounds = a.reduce_rounds;
     {
         LdsStack stk{smem + threadIdx.x, BLOCK, STACK};
@@ -288,6 +291,12 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 dDdLe[nv] = mk(0.0f, 0.0f, 0.0f);
                 dDdr[nv] = mk(0.0f, 0.0f, 0.0f);
                 lmat[nv] = 0;
+                float pre0 = 0.0f, pre1 = 0.0f;
+                if (a.lambert && depth == 0) {
+                    // path_traceBBB draws the first hit's cosine sample before its light sample
+                    pre0 = rand_f(rng);
+                    pre1 = rand_f(rng);
+                }
                 if (sc.light_count > 0u && !a.ref_ggx) {
                     NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng, a.light_power != 0u);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
@@ -312,26 +321,41 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                 dlogw[nv] = 0.0f;
                 dlogq[nv] = 0.0f;
                 dlogp[nv] = 0.0f;
-                const bool last = depth + 1u >= a.max_depth;
+                const bool last = depth + 1u >= a.vertices;
                 if (!last) {
-                    SampleT<S> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
-                    if (a.horizon_clamp && below_horizon(smp)) {
-                        nv += 1;
-                        break;
+                    V3T<S> next_d;
+                    bool roulette;
+                    if (a.lambert) {
+                        // path_traceBBB (path_tracer.wgsl:128-202): the
+                        // cosine-sampled Lambert weight is the albedo; the
+                        // roulette runs from loop depth rr_start (vertex
+                        // rr_start + 1), never at the first hit
+                        T = mul(T, lift<S>(m.base));
+                        wvert[nv] = m.base;
+                        dwdb[nv] = 1.0f;
+                        roulette = depth >= 1u && depth - 1u >= a.rr_start;
+                    } else {
+                        SampleT<S> smp = importance_sample_ggx_d(h.normal, wo, m, rng);
+                        if (a.horizon_clamp && below_horizon(smp)) {
+                            nv += 1;
+                            break;
+                        }
+                        T = mul(T, smp.reflectance);
+                        wvert[nv] = val3(smp.reflectance);
+                        dwdb[nv] = smp.dbase;
+                        dlogw[nv] = smp.dlogf_dr;
+                        dlogq[nv] = smp.dlogq_dr;
+                        dlogp[nv] = smp.dlogp_db;
+                        lobe |= (smp.lobe_c0 | (smp.floored ? 4u : 0u)) << (3 * nv);
+                        if constexpr (!DETACHED) {
+                            // the weight holds the lobe probability P fixed: the
+                            // throughput with every factor varying has d ln P less
+                            if (val(smp.lobe_p) > 0.0f) nrm = nrm + (1.0f - smp.lobe_p / val(smp.lobe_p));
+                        }
+                        next_d = normalize(smp.wi);
+                        roulette = depth >= a.rr_start;
                     }
-                    T = mul(T, smp.reflectance);
-                    wvert[nv] = val3(smp.reflectance);
-                    dwdb[nv] = smp.dbase;
-                    dlogw[nv] = smp.dlogf_dr;
-                    dlogq[nv] = smp.dlogq_dr;
-                    dlogp[nv] = smp.dlogp_db;
-                    lobe |= (smp.lobe_c0 | (smp.floored ? 4u : 0u)) << (3 * nv);
-                    if constexpr (!DETACHED) {
-                        // the weight holds the lobe probability P fixed: the
-                        // throughput with every factor varying has d ln P less
-                        if (val(smp.lobe_p) > 0.0f) nrm = nrm + (1.0f - smp.lobe_p / val(smp.lobe_p));
-                    }
-                    if (depth >= a.rr_start) {
+                    if (roulette) {
                         // the forward's roulette (shade.h) on the replayed
                         // throughput: the same draw, the same end of the
                         // path, the same 1/p
@@ -361,7 +385,12 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                         wvert[nv] = scale(ip, wvert[nv]);
                         dwdb[nv] *= ip;
                     }
-                    d = normalize(smp.wi);
+                    if (a.lambert) {
+                        const float b0 = depth == 0 ? pre0 : rand_f(rng);
+                        const float b1 = depth == 0 ? pre1 : rand_f(rng);
+                        next_d = sample_cosine_hemisphere(h.normal, b0, b1);
+                    }
+                    d = next_d;
                     o = add(h.position, scale(0.001f, h.normal));
                 }
                 nv += 1;

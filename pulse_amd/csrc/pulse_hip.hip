@@ -866,7 +866,7 @@ constexpr int kAdjBlock = 128;
 static_assert(sizeof(AdjArgs) + sizeof(DevScene) <= 4096, "adjoint kernel arguments exceed 4 KB");
 template <int STACK, int LOOP>
 void launch_adjoint_loop(pulse_ctx *ctx, const AdjArgs &a, size_t lds, uint32_t grid) {
-    if (a.max_depth > static_cast<uint32_t>(kAdjMaxVertices)) {
+    if (a.vertices > static_cast<uint32_t>(kAdjMaxVertices)) {
         // deep paths: fixed-direction roughness derivative, the 64-entry stack
         hipLaunchKernelGGL((k_adjoint<kAdjBlock, 64, true, LOOP, kAdjMaxVerticesDeep>), dim3(grid), dim3(kAdjBlock),
                            lds, ctx->stream, dev_scene(ctx), a);
@@ -892,12 +892,16 @@ void launch_adjoint(pulse_ctx *ctx, const AdjArgs &a, size_t lds, uint32_t grid)
 int adjoint_check(pulse_ctx *ctx, const pulse_params *prm, const char *who) {
     if (!ctx->has_scene()) PULSE_FAIL(PULSE_E_NO_SCENE, "%s: no scene uploaded", who);
     if (int rc_ = check_traversal(*ctx->cur, ctx->loop_variant)) return rc_;
-    if (prm->mode != PULSE_MODE_GGX_NEE && prm->mode != PULSE_MODE_REF_GGX)
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs PULSE_MODE_GGX_NEE or PULSE_MODE_REF_GGX", who);
-    if (prm->max_depth == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be 1..%d", who, kAdjMaxVerticesDeep);
-    if (prm->max_depth > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
-        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %d", who, kAdjMaxVertices);
+    if (prm->mode != PULSE_MODE_GGX_NEE && prm->mode != PULSE_MODE_REF_GGX && prm->mode != PULSE_MODE_REF_NEE_LAMBERT)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: unknown mode %u", who, prm->mode);
+    // path vertices: the Lambert mode's first hit comes before its max_depth bounces
+    const uint32_t extra = prm->mode == PULSE_MODE_REF_NEE_LAMBERT ? 1u : 0u;
+    if (prm->max_depth + extra == 0 || prm->max_depth > static_cast<uint32_t>(kAdjMaxVerticesDeep) - extra)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: max_depth must be %u..%u in this mode", who, 1u - extra,
+                   static_cast<uint32_t>(kAdjMaxVerticesDeep) - extra);
+    if (prm->max_depth + extra > static_cast<uint32_t>(kAdjMaxVertices) && (prm->flags & PULSE_FLAG_ADJOINT_REPARAM))
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: PULSE_FLAG_ADJOINT_REPARAM needs max_depth <= %u in this mode", who,
+                   static_cast<uint32_t>(kAdjMaxVertices) - extra);
     if (prm->spp == 0 || prm->spp > 0xffffffffu - prm->accumulation_count)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: needs spp >= 1 and accumulation_count + spp below 2^32", who);
     if (ctx->stack_entries > 64) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: BVH too deep for the adjoint stack", who);
@@ -915,8 +919,9 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
         hipLaunchKernelGGL(k_fill_zero, dim3((nent + 255) / 256), dim3(256), 0, ctx->stream, grad, nent);
     }
     AdjArgs a{};
-    a.max_depth = prm->max_depth;
-    a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX ? 1u : 0u;
+    a.lambert = prm->mode == PULSE_MODE_REF_NEE_LAMBERT ? 1u : 0u;
+    a.vertices = prm->max_depth + a.lambert;
+    a.nee_ggx = prm->nee_eval == PULSE_NEE_GGX && !a.lambert ? 1u : 0u;
     a.spp = prm->accumulation_count + prm->spp;  // every sample the accumulated image holds
     a.light_power = (prm->flags & PULSE_FLAG_LIGHT_POWER) ? 1u : 0u;
     a.ref_ggx = prm->mode == PULSE_MODE_REF_GGX ? 1u : 0u;

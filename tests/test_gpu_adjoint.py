@@ -147,6 +147,21 @@ def test_reference_path_trace_gradient(dev, extra_flags, frozen, depth):
     _replay_case(dev, scene, cam, prm, frozen)
 
 
+@pytest.mark.parametrize("extra_flags,frozen,depth", [
+    pytest.param(0, True, 3, id="detached-3"), pytest.param(0, True, 7, id="detached-7"),
+    pytest.param(N.FLAG_ADJOINT_REPARAM, False, 3, id="reparam-3")])
+def test_lambert_path_gradient(dev, extra_flags, frozen, depth):
+    """The reference's path_traceBBB (PULSE_MODE_REF_NEE_LAMBERT: the first
+    hit with its emission and Lambert light sample, then max_depth
+    cosine-sampled bounces whose weight is the albedo), roulette from loop
+    depth 1: replay == forward bit for bit, gradient == the oracle's
+    central differences (no roughness dependence: that column is 0)."""
+    scene, cam, prm = _cornell_case(depth=depth, rr=1)
+    prm.mode = N.MODE_REF_NEE_LAMBERT
+    prm.flags |= extra_flags
+    _replay_case(dev, scene, cam, prm, frozen)
+
+
 def test_roulette_deep_paths(dev):
     scene, cam, prm = _cornell_case(depth=8, rr=2)
     _replay_case(dev, scene, cam, prm, True)
@@ -193,7 +208,7 @@ def test_field_gradient_vs_oracle_and_rejections(dev, extra_flags, frozen, light
     print("field: max err / scale", float((np.abs(g - gref) / scale).max()), flush=True)
     assert (np.abs(g - gref) <= 2e-3 * np.abs(gref) + 1e-4 * scale).all()
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
-        adj.backward(cam, make_params(spp=2, max_depth=3, mode=N.MODE_REF_NEE_LAMBERT, rr_start_depth=3))
+        adj.backward(cam, make_params(spp=2, max_depth=8, mode=N.MODE_REF_NEE_LAMBERT, rr_start_depth=3))
     with pytest.raises(N.PulseError, match="UNSUPPORTED"):
         adj.backward(cam, make_params(spp=2, max_depth=9, mode=N.MODE_GGX_NEE, rr_start_depth=3,
                                       flags=N.FLAG_HORIZON_CLAMP))

@@ -184,7 +184,7 @@ def test_views_argument_errors(dev):
     bad[0].target = mv.targets[0].ptr
     bad[0].camera.width = 0
     assert lib.pulse_adjoint_views(dev.handle, bad, 1, C.byref(prm), mv.grad.ptr) == -1  # PULSE_E_INVALID_ARG
-    ref = make_params(spp=2, max_depth=3, mode=N.MODE_REF_NEE_LAMBERT, rr_start_depth=3)
+    ref = make_params(spp=2, max_depth=9, mode=N.MODE_GGX_NEE, rr_start_depth=3)
     assert lib.pulse_adjoint_views(dev.handle, mv.views, 1, C.byref(ref), mv.grad.ptr) == -7  # PULSE_E_UNSUPPORTED
     for bad_rounds in (17, -2):
         with pytest.raises(N.PulseError, match="INVALID_ARG"):
