@@ -335,6 +335,19 @@ int pulse_scene_update_instances(pulse_ctx *ctx, pulse_scene *scene, const pulse
  * pulse_scene_update_instances or pulse_scene_upload).  Requires the scene's
  * 4-wide BVHs.  Synchronises the context's stream. */
 int pulse_scene_refit_instances(pulse_ctx *ctx, pulse_scene *scene, const float *object_world, uint32_t n);
+/* As pulse_scene_refit_instances (same arguments, same instance records and
+ * world boxes), with a new 4-wide TLAS built on the device over the moved
+ * instances instead of a refit of the last one: a refit's boxes overlap
+ * after large motion and traversal slows (the bench scene with all 800
+ * instances moved: 12.9 ms per frame refit, 5.3 ms rebuilt); the rebuild
+ * keeps it.  No counterpart in the reference, which rebuilds its TLAS on the
+ * host every frame (build_tlas, tlas.rs:29-136): this is that rebuild, for
+ * the 4-wide traversal, on the GPU.  One block sorts the entries' Morton
+ * codes and builds a binary radix tree (Karras 2012) collapsed to 4-wide
+ * nodes, so it takes scenes of at most 4096 instance_indices entries
+ * (PULSE_E_UNSUPPORTED above) whose TLAS holds every entry once.  The binary
+ * TLAS is stale afterwards, as after a refit.  Synchronous. */
+int pulse_scene_rebuild_instances(pulse_ctx *ctx, pulse_scene *scene, const float *object_world, uint32_t n);
 int pulse_scene_destroy(pulse_scene *scene);
 int pulse_scene_bind(pulse_ctx *ctx, pulse_scene *scene);
 

@@ -279,6 +279,7 @@ SIGNATURES = [
     ("pulse_scene_bind", _I, [_P, _P]),
     ("pulse_scene_update_instances", _I, [_P, _P, C.POINTER(SceneDesc)]),
     ("pulse_scene_refit_instances", _I, [_P, _P, _P, C.c_uint32]),
+    ("pulse_scene_rebuild_instances", _I, [_P, _P, _P, C.c_uint32]),
     ("pulse_sb_set_instance_transform", _I, [_P, _U32, _P]),
     ("pulse_sb_update_instances", _I, [_P]),
     ("pulse_batch_default_config", _I, [C.POINTER(BatchConfig)]),

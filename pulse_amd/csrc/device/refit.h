@@ -184,12 +184,10 @@ __global__ void __launch_bounds__(256) k_refit_boxes(const uint32_t *qnodes, con
     }
 }
 
-// One thread per wide TLAS node: quantise its children's boxes (outwards)
-// into words 0-9, as the host's quantize_wide does at upload.
-__global__ void __launch_bounds__(64) k_refit_quantize(uint32_t *qnodes, uint32_t n_nodes, const float *child_box,
-                                                       uint32_t *err) {
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-    if (i >= n_nodes) return;
+// Quantises wide TLAS node i's children's boxes (child_box[24 i + 6 k ..])
+// outwards into words 0-9 of its 16, as the host's quantize_wide does at
+// upload; a box the node cannot hold raises kErrRefit.
+__device__ inline void quantize_tlas_node(uint32_t *qnodes, uint32_t i, const float *child_box, uint32_t *err) {
     uint32_t *q = qnodes + 16u * i;
     bool valid[4];
     for (int k = 0; k < 4; ++k) valid[k] = q[10 + k] != 0xffffffffu;
@@ -240,6 +238,305 @@ __global__ void __launch_bounds__(64) k_refit_quantize(uint32_t *qnodes, uint32_
     for (int a = 0; a < 3; ++a) q[a] = __float_as_uint(origin[a]);
     q[3] = exps;
     for (int j = 0; j < 6; ++j) q[4 + j] = planes[j];
+}
+
+// One thread per wide TLAS node: quantize_tlas_node.
+__global__ void __launch_bounds__(64) k_refit_quantize(uint32_t *qnodes, uint32_t n_nodes, const float *child_box,
+                                                       uint32_t *err) {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i < n_nodes) quantize_tlas_node(qnodes, i, child_box, err);
+}
+
+// ---------------------------------------------------------------------------
+// Rebuild of the 4-wide TLAS on the device (pulse_scene_rebuild_instances).
+// A refit keeps the tree of the last build: after large motion its boxes
+// overlap and traversal slows (all 800 instances of the bench scene moved to
+// new places: 12.9 ms per frame refit against 5.3 ms rebuilt).  One block
+// builds a new tree over the instance_indices entries from the instances'
+// world boxes, in LDS and a scratch array:
+//   1. 64-bit keys: a 30-bit Morton code of the box centre within the
+//      centres' bounds, then the entry index (keys are distinct); bitonic
+//      sort in LDS;
+//   2. the binary radix tree over the sorted keys (Karras 2012, "Maximizing
+//      parallelism in the construction of BVHs, octrees, and k-d trees"):
+//      internal node i covers a key range and splits at the highest
+//      differing bit; node 0 is the root; every leaf is one entry;
+//   3. boxes bottom up (a counter per internal node: the second child to
+//      arrive computes the node), depths by walking to the root;
+//   4. wide nodes = the internal nodes at even depth, each over its
+//      grandchildren (a leaf child stays a leaf), numbered level by level
+//      (a second sort by (depth / 2, node)), so every interior entry names a
+//      later node and the traversal, which counts no visits, ends;
+//   5. quantised as at upload; the level table for later refits.
+// Every loop that follows parent links stops after n steps (a malformed
+// tree raises kErrRefit instead of looping).
+constexpr uint32_t kRebuildMax = 4096;   // entries one block sorts in LDS
+constexpr int kRebuildThreads = 1024;
+constexpr uint32_t kRbNone = 0xffffffffu;
+constexpr uint32_t kRbLeaf = 0x80000000u;
+constexpr uint32_t kRebuildScratchWords = 16;  // per entry, see k_rebuild_tlas
+
+__device__ __forceinline__ uint32_t morton_expand10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ int rb_delta(const uint64_t *keys, int n, int a, int b) {
+    if (b < 0 || b > n - 1) return -1;
+    return __clzll(static_cast<long long>(keys[a] ^ keys[b]));
+}
+// Ascending bitonic sort of keys[0 .. p), p a power of two, by the block.
+__device__ inline void rb_bitonic(uint64_t *keys, uint32_t p) {
+    for (uint32_t k = 2; k <= p; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = keys[i], b = keys[ixj];
+                    const bool up = (i & k) == 0u;
+                    if ((a > b) == up) {
+                        keys[i] = b;
+                        keys[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// n entries (1 .. kRebuildMax), entry j's box inst_box[6 iidx[j] ..];
+// scratch holds kRebuildScratchWords n words; qnodes room for max(n - 1, 1)
+// nodes, child_box 24 floats per node, levels 2 + 2 max(n - 1, 1) words.
+// info[0] = wide nodes, info[1] = wide levels.
+__global__ void __launch_bounds__(kRebuildThreads) k_rebuild_tlas(const uint32_t *iidx, uint32_t n,
+                                                                   const float *inst_box, uint32_t *scratch,
+                                                                   float *child_box, uint32_t *qnodes,
+                                                                   uint32_t *levels, uint32_t *info, uint32_t *err) {
+    __shared__ uint64_t keys[kRebuildMax];
+    __shared__ float red[6][kRebuildThreads / 64];
+    __shared__ float bounds[6];
+    __shared__ uint32_t offs[72];
+    __shared__ uint32_t counts[2];
+    const uint32_t tid = threadIdx.x, T = blockDim.x;
+    const uint32_t N = n, M = n - 1u;  // leaves, internal nodes
+    uint32_t *pos = scratch;           // N: sorted leaf -> entry
+    uint32_t *left = pos + N;          // M each below
+    uint32_t *right = left + M;
+    uint32_t *par_i = right + M;
+    uint32_t *par_l = par_i + M;       // N
+    uint32_t *flag = par_l + N;
+    uint32_t *depth = flag + M;
+    uint32_t *widx = depth + M;
+    float *ibox = reinterpret_cast<float *>(widx + M);  // 6 M
+    auto box_of = [&](uint32_t c, float *b) {
+        const float *src = (c & kRbLeaf) ? inst_box + 6u * iidx[pos[c & ~kRbLeaf]] : ibox + 6u * c;
+        for (int a = 0; a < 6; ++a) b[a] = src[a];
+    };
+    if (N == 1u) {
+        if (tid == 0) {
+            const float *b = inst_box + 6u * iidx[0];
+            uint32_t *q = qnodes;
+            for (int k = 0; k < 16; ++k) q[k] = 0u;
+            q[10] = kRbLeaf | 0u;
+            q[11] = q[12] = q[13] = kRbNone;
+            for (int a = 0; a < 24; ++a) child_box[a] = 0.0f;
+            for (int a = 0; a < 6; ++a) child_box[a] = b[a];
+            quantize_tlas_node(qnodes, 0u, child_box, err);
+            levels[0] = 1u;
+            levels[1] = 0u;
+            levels[2] = 1u;
+            levels[3] = 0u;
+            info[0] = 1u;
+            info[1] = 1u;
+        }
+        return;
+    }
+    // 1. centre bounds, keys, sort
+    float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (uint32_t j = tid; j < N; j += T) {
+        const float *b = inst_box + 6u * iidx[j];
+        for (int a = 0; a < 3; ++a) {
+            const float c = 0.5f * b[a] + 0.5f * b[3 + a];
+            mn[a] = fminf(mn[a], c);
+            mx[a] = fmaxf(mx[a], c);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+        }
+    if ((tid & 63u) == 0u)
+        for (int a = 0; a < 3; ++a) {
+            red[a][tid >> 6] = mn[a];
+            red[3 + a][tid >> 6] = mx[a];
+        }
+    __syncthreads();
+    if (tid < 6u) {
+        float v = red[tid][0];
+        for (uint32_t w = 1; w < T / 64u; ++w) v = tid < 3u ? fminf(v, red[tid][w]) : fmaxf(v, red[tid][w]);
+        bounds[tid] = v;
+    }
+    __syncthreads();
+    uint32_t P = 1;
+    while (P < N) P <<= 1;
+    for (uint32_t j = tid; j < P; j += T) {
+        uint64_t key = ~0ull;
+        if (j < N) {
+            const float *b = inst_box + 6u * iidx[j];
+            uint32_t q[3];
+            for (int a = 0; a < 3; ++a) {
+                const float ext = bounds[3 + a] - bounds[a];
+                const float c = 0.5f * b[a] + 0.5f * b[3 + a];
+                const float u = ext > 0.0f ? (c - bounds[a]) / ext * 1024.0f : 0.0f;
+                q[a] = static_cast<uint32_t>(fminf(fmaxf(u, 0.0f), 1023.0f));
+            }
+            const uint32_t m = (morton_expand10(q[0]) << 2) | (morton_expand10(q[1]) << 1) | morton_expand10(q[2]);
+            key = (static_cast<uint64_t>(m) << 32) | j;
+        }
+        keys[j] = key;
+    }
+    __syncthreads();
+    rb_bitonic(keys, P);
+    // 2. radix tree
+    for (uint32_t i = tid; i < N; i += T) pos[i] = static_cast<uint32_t>(keys[i]);
+    if (tid == 0) par_i[0] = kRbNone;
+    for (uint32_t i = tid; i < M; i += T) {
+        const int ii = static_cast<int>(i), nn = static_cast<int>(N);
+        const int d = rb_delta(keys, nn, ii, ii + 1) - rb_delta(keys, nn, ii, ii - 1) > 0 ? 1 : -1;
+        const int dmin = rb_delta(keys, nn, ii, ii - d);
+        int lmax = 2;
+        while (rb_delta(keys, nn, ii, ii + lmax * d) > dmin) lmax *= 2;
+        int l = 0;
+        for (int t = lmax / 2; t >= 1; t /= 2)
+            if (rb_delta(keys, nn, ii, ii + (l + t) * d) > dmin) l += t;
+        const int j = ii + l * d;
+        const int dnode = rb_delta(keys, nn, ii, j);
+        int s = 0, t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (rb_delta(keys, nn, ii, ii + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        const int gamma = ii + s * d + min(d, 0);
+        const uint32_t L = min(ii, j) == gamma ? (kRbLeaf | static_cast<uint32_t>(gamma)) : static_cast<uint32_t>(gamma);
+        const uint32_t R = max(ii, j) == gamma + 1 ? (kRbLeaf | static_cast<uint32_t>(gamma + 1))
+                                                   : static_cast<uint32_t>(gamma + 1);
+        left[i] = L;
+        right[i] = R;
+        if (L & kRbLeaf) par_l[L & ~kRbLeaf] = i;
+        else par_i[L] = i;
+        if (R & kRbLeaf) par_l[R & ~kRbLeaf] = i;
+        else par_i[R] = i;
+        flag[i] = 0u;
+    }
+    __syncthreads();
+    // 3. boxes bottom up, depths
+    for (uint32_t i = tid; i < N; i += T) {
+        uint32_t p = par_l[i];
+        for (uint32_t steps = 0; p != kRbNone; ++steps) {
+            if (steps > N || p >= M) {
+                atomicOr(err, kErrRefit);
+                break;
+            }
+            __threadfence();
+            if (atomicAdd(&flag[p], 1u) == 0u) break;  // the other child completes p
+            __threadfence();
+            float a[6], b[6];
+            box_of(left[p], a);
+            box_of(right[p], b);
+            for (int k = 0; k < 3; ++k) {
+                ibox[6u * p + k] = fminf(a[k], b[k]);
+                ibox[6u * p + 3 + k] = fmaxf(a[3 + k], b[3 + k]);
+            }
+            p = par_i[p];
+        }
+    }
+    for (uint32_t i = tid; i < M; i += T) {
+        uint32_t dd = 0, p = i;
+        while (p != 0u && dd <= M) {
+            p = par_i[p];
+            if (p >= M) break;
+            ++dd;
+        }
+        if (p != 0u || dd > M) atomicOr(err, kErrRefit);
+        depth[i] = dd;
+    }
+    __syncthreads();
+    // 4. wide nodes, numbered level by level
+    uint32_t P2 = 1;
+    while (P2 < M) P2 <<= 1;
+    for (uint32_t i = tid; i < P2; i += T)
+        keys[i] = (i < M && (depth[i] & 1u) == 0u) ? ((static_cast<uint64_t>(depth[i] >> 1) << 32) | i) : ~0ull;
+    __syncthreads();
+    rb_bitonic(keys, P2);
+    for (uint32_t k = tid; k < P2; k += T)
+        if (keys[k] != ~0ull && (k + 1u == P2 || keys[k + 1u] == ~0ull)) {
+            counts[0] = k + 1u;                                          // wide nodes
+            counts[1] = static_cast<uint32_t>(keys[k] >> 32) + 1u;       // wide levels
+        }
+    __syncthreads();
+    const uint32_t n_wide = counts[0], n_levels = counts[1];
+    if (n_levels > 70u) {  // deeper than 64-bit keys allow: malformed
+        if (tid == 0) atomicOr(err, kErrRefit);
+        return;
+    }
+    for (uint32_t k = tid; k < n_wide; k += T) {
+        widx[static_cast<uint32_t>(keys[k])] = k;
+        const uint32_t lv = static_cast<uint32_t>(keys[k] >> 32);
+        if (k == 0u || static_cast<uint32_t>(keys[k - 1u] >> 32) != lv) offs[lv] = k;
+    }
+    if (tid == 0) offs[n_levels] = n_wide;
+    __syncthreads();
+    for (uint32_t k = tid; k < n_wide; k += T) {
+        const uint32_t b = static_cast<uint32_t>(keys[k]);
+        uint32_t *q = qnodes + 16u * k;
+        float *cb = child_box + 24u * k;
+        int slot = 0;
+        auto put = [&](uint32_t c) {
+            float bx[6];
+            box_of(c, bx);
+            uint32_t e;
+            if (c & kRbLeaf) {
+                e = kRbLeaf | pos[c & ~kRbLeaf];  // one entry, count 1
+            } else {
+                e = widx[c];
+                if (e <= k || e >= n_wide) atomicOr(err, kErrRefit);
+            }
+            q[10 + slot] = e;
+            for (int a = 0; a < 6; ++a) cb[6 * slot + a] = bx[a];
+            ++slot;
+        };
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t c = h == 0 ? left[b] : right[b];
+            if (c & kRbLeaf) {
+                put(c);
+            } else {
+                put(left[c]);
+                put(right[c]);
+            }
+        }
+        for (; slot < 4; ++slot) {
+            q[10 + slot] = kRbNone;
+            for (int a = 0; a < 6; ++a) cb[6 * slot + a] = 0.0f;
+        }
+        q[14] = 0u;
+        q[15] = 0u;
+        quantize_tlas_node(qnodes, k, child_box, err);
+        // level table, deepest level first
+        const uint32_t lv = static_cast<uint32_t>(keys[k] >> 32);
+        levels[2u + n_levels + (n_wide - offs[lv + 1u]) + (k - offs[lv])] = k;
+    }
+    for (uint32_t l = tid; l <= n_levels; l += T) levels[1u + l] = n_wide - offs[n_levels - l];
+    if (tid == 0) {
+        levels[0] = n_levels;
+        info[0] = n_wide;
+        info[1] = n_levels;
+    }
 }
 
 }  // namespace pt

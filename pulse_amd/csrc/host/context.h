@@ -69,8 +69,10 @@ struct SceneData {
     bool has_wide = false;       // the 4-wide BVHs were built
     uint32_t n_wide_tlas = 0;    // quantised 4-wide TLAS nodes
     bool binary_stale = false;   // a device refit moved instances: the binary TLAS is out of date
-    DevBuf refit[8];             // refit: matrices, instance boxes, child boxes, node boxes, error word,
-                                 // BLAS cut boxes, per-instance cut ranges, 4-wide TLAS levels
+    bool tlas_all_entries = false;  // the TLAS leaves hold every instance_indices entry once (device rebuild)
+    DevBuf refit[10];            // refit: matrices, instance boxes, child boxes, node boxes, error word,
+                                 // BLAS cut boxes, per-instance cut ranges, 4-wide TLAS levels;
+                                 // rebuild: scratch, node / level counts
     // per instance {first, count} into cut_boxes: the boxes of its BLAS cut
     // at kCutDepth, from which the 4-wide TLAS's instance boxes are made
     std::vector<float> cut_boxes;

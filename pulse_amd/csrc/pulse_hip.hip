@@ -92,6 +92,30 @@ bool walk_tlas(const pulse_tlas_node *nodes, size_t n, size_t n_iidx, uint32_t &
     return true;
 }
 
+// True if the leaves of `nodes` (walked by walk_tlas) hold each of the n_iidx
+// instance_indices entries exactly once: the device rebuild makes a tree
+// over all entries.
+bool tlas_covers_entries(const pulse_tlas_node *nodes, size_t n_iidx) {
+    std::vector<uint8_t> seen(n_iidx, 0);
+    std::vector<uint32_t> st{0u};
+    size_t held = 0;
+    while (!st.empty()) {
+        const pulse_tlas_node &x = nodes[st.back()];
+        st.pop_back();
+        if (x.instance_count == 0) {
+            st.push_back(x.a_or_first_instance + 1u);
+            st.push_back(x.a_or_first_instance);
+            continue;
+        }
+        for (uint32_t c = 0; c < x.instance_count; ++c) {
+            if (seen[x.a_or_first_instance + c]) return false;
+            seen[x.a_or_first_instance + c] = 1;
+            ++held;
+        }
+    }
+    return held == n_iidx;
+}
+
 DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx->errors.as<uint32_t>()); }
 
 // Makes `sd` the scene of the context's launches.
@@ -657,6 +681,12 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
         (rc = up(B_LIGHTS, lights, d->light_indices.bytes)) ||
         (rc = up(B_SCDF, d->light_emission_strength_cdf.data, d->light_emission_strength_cdf.bytes)))
         return rc;
+    // room for a device rebuild's tree (at most one node per internal node of
+    // a binary tree over the entries), so that one never moves the buffer
+    if (has_wide && n_iidx <= kRebuildMax &&
+        (rc = sd.buf[B_WTLAS].ensure(std::max<size_t>(qtlas.size() * 4,
+                                                      64 * std::max<size_t>(n_iidx > 0 ? n_iidx - 1 : 1, 1)))))
+        return rc;
     if (has_wide && ((rc = up(B_WTLAS, qtlas.data(), qtlas.size() * 4)) ||
                      (rc = up(B_WBLAS, wblas.data(), wblas.size() * 4)) ||
                      (rc = up(B_WINST, wrec.data(), wrec.size() * 4)) ||
@@ -680,6 +710,7 @@ int scene_upload(SceneData &sd, const pulse_scene_desc *d) {
     sd.cut_range = std::move(cut_range);
     sd.wide_root = std::move(wroot);
     sd.n_iidx = static_cast<uint32_t>(n_iidx);
+    sd.tlas_all_entries = !empty_tlas && tlas_covers_entries(tlas, n_iidx);
     sd.has_wide = has_wide;
     sd.wide_need = wide_need;
     sd.wide_blas_need = wide_blas_need;
@@ -797,6 +828,7 @@ int scene_update_instances(SceneData &sd, const pulse_scene_desc *d) {
     sd.wide_need = wide_need;
     sd.n_wide_tlas = static_cast<uint32_t>(qtlas.size() / kQuantNodeWords);
     sd.n_iidx = static_cast<uint32_t>(n_iidx);
+    sd.tlas_all_entries = !empty_tlas && tlas_covers_entries(tlas, n_iidx);
     sd.binary_stale = false;
     sd.tlas_depth = tlas_depth;
     sd.stack_need = need;
@@ -809,6 +841,63 @@ int check_traversal(const SceneData &s, int loop) {
     if (loop != 2 && s.binary_stale)
         PULSE_FAIL(PULSE_E_UNSUPPORTED, "traversal loop %d: the scene's binary TLAS is out of date after a device "
                    "refit (pulse_scene_refit_instances); use loop 2 or pulse_scene_update_instances", loop);
+    return PULSE_OK;
+}
+
+// Instance motion with a new 4-wide TLAS built on the device
+// (k_rebuild_tlas) instead of a refit of the last one.
+int scene_rebuild_instances(SceneData &sd, const float *object_world, uint32_t n, hipStream_t stream) {
+    const char *who = "pulse_scene_rebuild_instances";
+    if (!sd.valid) PULSE_FAIL(PULSE_E_NO_SCENE, "%s: no scene uploaded", who);
+    if (!sd.has_wide) PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: the scene has no 4-wide BVHs", who);
+    if (n != sd.uniform.instance_count || 4ull * n != sd.inst_ranges.size())
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: %u matrices for %u instances", who, n, sd.uniform.instance_count);
+    if (n == 0) return PULSE_OK;
+    if (!object_world) PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: null matrices", who);
+    if (!sd.tlas_all_entries)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: the scene's TLAS does not hold every instance_indices entry once", who);
+    const uint32_t N = sd.n_iidx;
+    if (N == 0 || N > kRebuildMax)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: %u instance_indices entries (the device rebuild takes 1..%u; "
+                   "pulse_scene_refit_instances or pulse_scene_update_instances otherwise)", who, N, kRebuildMax);
+    const size_t M1 = std::max<uint32_t>(N - 1, 1);
+    if (sd.buf[B_WTLAS].cap < 64 * M1)
+        PULSE_FAIL(PULSE_E_UNSUPPORTED, "%s: the TLAS buffer has no room for a rebuilt tree", who);
+    int rc;
+    if ((rc = sd.refit[0].ensure(sizeof(float) * 16 * n)) || (rc = sd.refit[1].ensure(sizeof(float) * 6 * n)) ||
+        (rc = sd.refit[2].ensure(sizeof(float) * 24 * M1)) || (rc = sd.refit[4].ensure(sizeof(uint32_t))) ||
+        (rc = sd.refit[7].ensure(sizeof(uint32_t) * (2 + 2 * M1))) ||
+        (rc = sd.refit[8].ensure(sizeof(uint32_t) * kRebuildScratchWords * N)) ||
+        (rc = sd.refit[9].ensure(sizeof(uint32_t) * 2)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(stream));  // launches in flight read the current instances and TLAS
+    HIP_TRY(hipMemcpy(sd.refit[0].ptr, object_world, sizeof(float) * 16 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(sd.refit[4].ptr, 0, sizeof(uint32_t), stream));
+    uint32_t *err = sd.refit[4].as<uint32_t>();
+    hipLaunchKernelGGL(k_refit_instances, dim3(n), dim3(64), 0, stream, sd.refit[0].as<float>(),
+                       sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.refit[5].as<float>(),
+                       sd.refit[6].as<uint32_t>(), sd.refit[1].as<float>(), n, err);
+    hipLaunchKernelGGL(k_refit_records, dim3((N + 63) / 64), dim3(64), 0, stream,
+                       sd.buf[B_INSTANCES].as<pulse_mesh_instance>(), sd.buf[B_INST_IDX].as<uint32_t>(), N,
+                       sd.buf[B_WINST].as<uint32_t>(), err);
+    hipLaunchKernelGGL(k_rebuild_tlas, dim3(1), dim3(kRebuildThreads), 0, stream, sd.buf[B_INST_IDX].as<uint32_t>(), N,
+                       sd.refit[1].as<float>(), sd.refit[8].as<uint32_t>(), sd.refit[2].as<float>(),
+                       sd.buf[B_WTLAS].as<uint32_t>(), sd.refit[7].as<uint32_t>(), sd.refit[9].as<uint32_t>(), err);
+    HIP_TRY(hipGetLastError());
+    uint32_t flags = 0, info[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&flags, err, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(info, sd.refit[9].ptr, sizeof(info), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (flags || info[0] == 0 || info[0] > M1 || info[1] == 0 || info[1] > info[0]) {
+        sd.valid = false;  // partly rewritten: unusable until the next upload
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "%s: a singular transform, a bottom row other than 0 0 0 1, or a world box "
+                   "beyond 2^40 (the scene must be uploaded again)", who);
+    }
+    sd.n_wide_tlas = info[0];
+    // a node pushes at most 3 entries per level of the tree
+    sd.wide_need = 3 * info[1] + sd.wide_blas_need + 1;
+    sd.binary_stale = true;
+    sd.generation += 1;
     return PULSE_OK;
 }
 
@@ -1175,6 +1264,29 @@ int pulse_scene_refit_instances(pulse_ctx *ctx, pulse_scene *scene, const float 
         return rc;
     }
     if (ctx->cur == sd) ctx->generation += 1;
+    return PULSE_OK;
+}
+
+int pulse_scene_rebuild_instances(pulse_ctx *ctx, pulse_scene *scene, const float *object_world, uint32_t n) {
+    if (!ctx) PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_rebuild_instances: null ctx");
+    if (scene && scene->ctx != ctx)
+        PULSE_FAIL(PULSE_E_INVALID_ARG, "pulse_scene_rebuild_instances: scene of another context");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SceneData *sd = scene ? &scene->data : &ctx->own;
+    if ((rc = scene_rebuild_instances(*sd, object_world, n, ctx->stream)) != PULSE_OK) {
+        if (ctx->cur == sd && !sd->valid) ctx->cur = nullptr;
+        return rc;
+    }
+    if (ctx->cur == sd) {
+        uint32_t st = stack_for(*sd, ctx->loop_variant);
+        if (st == 0) {
+            ctx->cur = nullptr;
+            PULSE_FAIL(PULSE_E_UNSUPPORTED, "pulse_scene_rebuild_instances: TLAS too deep for the traversal stack");
+        }
+        ctx->stack_entries = st;
+        ctx->generation += 1;
+    }
     return PULSE_OK;
 }
 

@@ -118,6 +118,13 @@ class Device:
         return n.value
 
     # --- scene and tracing ---
+    def rebuild_instances(self, object_world: np.ndarray, target: "DeviceScene" = None) -> None:
+        """As refit_instances, with a new 4-wide TLAS built on the device
+        (pulse_scene_rebuild_instances) instead of a refit of the last one."""
+        m = np.ascontiguousarray(object_world, dtype=np.float32).reshape(-1, 16)
+        N.check(N.lib().pulse_scene_rebuild_instances(self._ctx, target.handle if target is not None else None,
+                                                      m.ctypes.data, len(m)))
+
     def upload_scene(self, scene: SceneBuffers) -> None:
         d = scene.desc()
         N.check(N.lib().pulse_scene_upload(self._ctx, C.byref(d)))
