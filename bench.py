@@ -135,7 +135,9 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
     no Russian roulette) on the same geometry with 64 materials: the target
     is rendered at 64 spp from hidden parameters (albedo +-0.3, roughness
     +-0.3 from the start values); timed: plain forward vs forward + loss +
-    adjoint."""
+    adjoint (pulse_forward_adjoint: its forward pass records the paths'
+    closest hits and shadow visibilities, which the adjoint reads instead of
+    tracing the paths again)."""
     from pulse_amd import _native as N
     from pulse_amd.path_tracer import Device, PulseAdjoint, apply_material_params, make_params, material_params
 
@@ -175,7 +177,8 @@ def bench_adjoint(scene, camf, iters: int = 5) -> dict:
     return {"forward_ms": round(fwd_ms, 3), "forward_adjoint_ms": round(fa_ms, 3),
             "ratio": round(fa_ms / fwd_ms, 3), "loss": loss, "grad_norm": float(np.linalg.norm(g)),
             "config": "512x512, 16 spp, max_depth 3, GGX + Lambert NEE, no RR, horizon clamp, "
-                      "65 materials x 7 parameters"}
+                      "65 materials x 7 parameters; the forward pass of pulse_forward_adjoint records its paths "
+                      "(closest hits, shadow visibilities) and the adjoint reads them"}
 
 
 def bench_adjoint_views(iters: int = 3) -> dict:

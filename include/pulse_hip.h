@@ -214,6 +214,13 @@ typedef struct pulse_params {
 #define PULSE_FLAG_ADJOINT_ACCUMULATE 64u /* adjoint: add to `grad` instead of     */
                                           /* overwriting it (the gradient of a sum */
                                           /* of per-view losses, one call a view)  */
+#define PULSE_FLAG_RECORD_PATHS 128u /* pulse_trace: keep every path's closest hits */
+                                     /* and shadow-ray visibilities (21 bytes per    */
+                                     /* vertex) for a pulse_adjoint of the same       */
+                                     /* image, which then reads them instead of       */
+                                     /* tracing the paths again (same gradient).      */
+                                     /* A call with accumulation_count 0 starts the   */
+                                     /* record, later calls extend it                 */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */
@@ -463,6 +470,11 @@ int pulse_trace_shadow_rays(pulse_ctx *ctx, const pulse_ray *rays, uint32_t *occ
                                            /* their own (0: per-lane atomics; 0..16). */
                                            /* -1 (default): 0 while the gradient fits */
                                            /* in LDS, 2 when it is in device memory   */
+#define PULSE_OPT_RECORDED_SAMPLES 17 /* read-only: samples per pixel of the valid  */
+                                      /* path record (PULSE_FLAG_RECORD_PATHS), 0   */
+                                      /* if none                                    */
+#define PULSE_OPT_ADJOINT_READ_RECORD 18 /* read-only: 1 if the last adjoint read   */
+                                         /* the recorded paths, 0 if it traced them */
 #define PULSE_OPT_INJECT_ERROR 12  /* write-only, for tests of error reporting: ORs */
                                    /* the value (PULSE_ERR_* bits) into the device */
                                    /* error word, as a kernel that met the          */

@@ -38,6 +38,7 @@ FLAG_ADJOINT_REPARAM = 8
 FLAG_LIGHT_POWER = 16
 FLAG_COUNT_WORK = 32
 FLAG_ADJOINT_ACCUMULATE = 64
+FLAG_RECORD_PATHS = 128
 PRESENT_RGBA32F = 0
 PRESENT_RGBA16F = 1
 PRESENT_RGBA8_SRGB = 2
@@ -57,6 +58,8 @@ OPT_PAIR_WAVES = 11
 OPT_INJECT_ERROR = 12
 OPT_PAIR_WAVES_SHADOW = 14
 OPT_ADJOINT_REDUCE_ROUNDS = 15
+OPT_RECORDED_SAMPLES = 17
+OPT_ADJOINT_READ_RECORD = 18
 ADJOINT_MAX_VIEWS = 16
 ERR_STACK_OVERFLOW = 1
 ERR_ITERATION_CAP = 2

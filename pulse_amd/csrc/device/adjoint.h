@@ -8,9 +8,12 @@
 // (path_tracer.wgsl:128-202: cosine sampling, Lambert light sampling,
 // emission at the first hit; no roughness dependence).
 //
-// Nothing of the forward paths is stored: each path is replayed from its
-// seed (the pixel index and accumulation count, path_tracer.wgsl:60-61) with
-// the same f32 traversal, so it takes exactly the forward decisions.  Along
+// Each path is replayed from its seed (the pixel index and accumulation
+// count, path_tracer.wgsl:60-61), so it takes exactly the forward
+// decisions.  Its closest hits and shadow-ray visibilities come from the
+// forward pass's record when it kept one (PULSE_FLAG_RECORD_PATHS, 21 bytes
+// per vertex: no traversal at all), else from the same f32 traversal run
+// again.  Along
 // the replay the vertex computations run in Dual<MAXV> arithmetic (MAXV the
 // kernel's vertex capacity): the tangent k is the derivative with respect to the roughness
 // used at vertex k, which moves the sampled half vector and therefore every
@@ -96,6 +99,13 @@ struct AdjArgs {
     uint32_t light_power;  // lights picked by emission strength (its pdf is held: see below)
     uint32_t ref_ggx;      // PULSE_MODE_REF_GGX: emission at every hit, no light sampling
     uint32_t lambert;      // PULSE_MODE_REF_NEE_LAMBERT: cosine sampling, weight = albedo
+    // the forward pass's recorded paths (null: trace them again): at
+    // (sample * rec_npix + pixel) * vertices + vertex, the closest hit
+    // {t, inst bits, tri bits, u}, its v, and the shadow ray's visibility
+    const float4 *rec_hit;
+    const float *rec_v;
+    const uint8_t *rec_vis;
+    uint32_t rec_npix;
     float *grad;           // n_materials * 7, accumulated into
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
@@ -256,7 +266,17 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             for (uint32_t depth = 0; depth < depth_end; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
                 uint32_t inst = 0, tri = 0;
-                trace_any<LOOP, false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
+                const uint32_t rec = (s * a.rec_npix + pixel) * a.vertices + depth;
+                if (a.rec_hit) {
+                    const float4 hr = a.rec_hit[rec];
+                    t = hr.x;
+                    inst = __float_as_uint(hr.y);
+                    tri = __float_as_uint(hr.z);
+                    u = hr.w;
+                    v = a.rec_v[rec];
+                } else {
+                    trace_any<LOOP, false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
+                }
                 if (!(t < kTFar)) break;
                 S tw, uu, vv;
                 if constexpr (DETACHED) {
@@ -301,8 +321,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng, a.light_power != 0u);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
-                    bool occ = trace_any<LOOP, true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st, su, sv,
-                                                   val(ns.threshold), stk);
+                    const bool occ = a.rec_vis ? a.rec_vis[rec] == 0u
+                                               : trace_any<LOOP, true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st,
+                                                                       su, sv, val(ns.threshold), stk);
                     if (!occ) {
                         C = add(C, mul(T, ns.contrib));
                         ck = add(ck, val3(ns.contrib));

@@ -46,6 +46,14 @@ struct WfBuffers {
     uint32_t *sh_count;   // [kMaxTraces + 1]
     uint32_t *ext_head;   // [kMaxTraces + 1] rays taken by the persistent extend launches
     uint32_t *sh_head;    // [kMaxTraces + 1] rays taken by the persistent shadow launches
+    // optional record of the paths for the adjoint (null: none): at
+    // r = (rec_base + pixel) * rec_traces + trace index, the closest hit
+    // {t, inst bits, tri bits, u}, its v, and 1 / 0 for an unoccluded /
+    // occluded shadow ray (rec_base = sample * pixels)
+    float4 *rec_hit = nullptr;
+    float *rec_v = nullptr;
+    uint8_t *rec_vis = nullptr;
+    uint32_t rec_base = 0, rec_traces = 0;
 };
 
 // generate: primary rays (path_tracer.wgsl:60-75) in 8x8 pixel tiles.
@@ -104,6 +112,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, WfBuffers wb
         float4 oa = wb.ray_o[q][i], da = wb.ray_d[q][i];
         float4 ha = wb.hit[i];
         pixel = __float_as_uint(oa.w);
+        if (wb.rec_hit) {
+            const uint32_t r = (wb.rec_base + pixel) * wb.rec_traces + static_cast<uint32_t>(k);
+            wb.rec_hit[r] = ha;
+            wb.rec_v[r] = wb.hit_v[i];
+        }
         if (ha.x < kTFar) {  // a miss ends the path (path_tracer.wgsl:83-86)
             float4 st = wb.state_t[pixel], sc4 = wb.state_c[pixel];
             V3 T = mk(st.x, st.y, st.z), C = mk(sc4.x, sc4.y, sc4.z);
@@ -143,6 +156,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
         float4 c = wb.sh_c[i];
         wb.state_c[__float_as_uint(oa.w)] = c;
     }
+    if (wb.rec_vis) wb.rec_vis[(wb.rec_base + __float_as_uint(oa.w)) * wb.rec_traces + k] = occluded ? 0u : 1u;
 }
 
 // Persistent extend (SHADOW = false, queue q at trace index k) or shadow
@@ -197,7 +211,10 @@ k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t
                         float4 oa = wb.sh_o[idx], da = wb.sh_d[idx];
                         pixel = __float_as_uint(oa.w);
                         alive = tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, da.w);
-                        if (!alive) wb.state_c[pixel] = wb.sh_c[idx];  // empty scene: visible
+                        if (!alive) {
+                            wb.state_c[pixel] = wb.sh_c[idx];  // empty scene: visible
+                            if (wb.rec_vis) wb.rec_vis[(wb.rec_base + pixel) * wb.rec_traces + k] = 1u;
+                        }
                     } else {
                         float4 oa = wb.ray_o[q][idx], da = wb.ray_d[q][idx];
                         alive = tr.begin(sc, mk(oa.x, oa.y, oa.z), mk(da.x, da.y, da.z), kTFar, 0.0f);
@@ -222,6 +239,7 @@ k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t
             alive = false;
             if (SHADOW) {
                 if (!tr.occluded) wb.state_c[pixel] = wb.sh_c[slot];
+                if (wb.rec_vis) wb.rec_vis[(wb.rec_base + pixel) * wb.rec_traces + k] = tr.occluded ? 0u : 1u;
             } else {
                 wb.hit[slot] = make_float4(tr.rec_t, __uint_as_float(tr.rec_inst), __uint_as_float(tr.rec_tri), tr.rec_u);
                 wb.hit_v[slot] = tr.rec_v;

@@ -116,6 +116,15 @@ bool tlas_covers_entries(const pulse_tlas_node *nodes, size_t n_iidx) {
     return held == n_iidx;
 }
 
+// The parameters a path record depends on (everything but the sample range
+// and the flags that do not change paths).
+bool same_paths(const pulse_params &a, const pulse_params &b) {
+    const uint32_t keep = PULSE_FLAG_HORIZON_CLAMP | PULSE_FLAG_LIGHT_POWER;
+    return a.max_depth == b.max_depth && a.mode == b.mode && a.nee_eval == b.nee_eval &&
+           a.rr_start_depth == b.rr_start_depth && (a.flags & keep) == (b.flags & keep);
+}
+bool same_camera(const pulse_camera &a, const pulse_camera &b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
 DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx->errors.as<uint32_t>()); }
 
 // Makes `sd` the scene of the context's launches.
@@ -1019,6 +1028,17 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
     a.rr_start = prm->rr_start_depth;
+    // the forward pass's recorded paths, when they are this image's
+    if (n == 1 && ctx->rec_valid && ctx->rec_scene == ctx->cur && ctx->rec_generation == ctx->cur->generation &&
+        ctx->rec_loop == ctx->loop_variant && ctx->rec_samples >= a.spp && ctx->rec_traces == a.vertices &&
+        static_cast<uint64_t>(ctx->rec_npix) == static_cast<uint64_t>(views[0].camera.width) * views[0].camera.height &&
+        same_camera(ctx->rec_cam, views[0].camera) && same_paths(ctx->rec_prm, *prm)) {
+        a.rec_hit = ctx->rec_hit.as<float4>();
+        a.rec_v = ctx->rec_v.as<float>();
+        a.rec_vis = ctx->rec_vis.as<uint8_t>();
+        a.rec_npix = ctx->rec_npix;
+    }
+    ctx->rec_read = a.rec_hit != nullptr;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
     a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
     const size_t lds = a.use_lds ? lds_bytes : 0;
@@ -1103,6 +1123,10 @@ int pulse_destroy(pulse_ctx *ctx) {
     ctx->errors.release();
     ctx->loss_acc.release();
     ctx->spill.release();
+    ctx->census.release();
+    ctx->rec_hit.release();
+    ctx->rec_v.release();
+    ctx->rec_vis.release();
     if (ctx->events_created)
         for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->events2_created)
@@ -1230,6 +1254,7 @@ int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (count) HIP_TRY(hipMemcpy(ctx->cur->buf[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
+    if (ctx->rec_scene == ctx->cur) ctx->rec_valid = false;  // materials steer the paths
     return PULSE_OK;
 }
 
@@ -1311,6 +1336,7 @@ int pulse_scene_destroy(pulse_scene *scene) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->cur == &scene->data) ctx->cur = ctx->own.valid ? &ctx->own : nullptr;
+    if (ctx->rec_scene == &scene->data) ctx->rec_valid = false;
     scene->data.release();
     delete scene;
     return PULSE_OK;
@@ -1383,6 +1409,56 @@ int frame_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
 }
 }  // namespace
 
+// Path record of a pulse_trace call (PULSE_FLAG_RECORD_PATHS, or the forward
+// pass of pulse_forward_adjoint): accumulation_count 0 starts one, a call
+// that continues the recorded samples with the same camera, parameters and
+// scene state extends it (the older samples are kept when the buffers
+// grow); anything else leaves no valid record.  Sets wb's record pointers
+// (its rec_base is set per sample) or leaves them null.
+int record_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, uint32_t traces, size_t npix,
+                 WfBuffers &wb) {
+    const bool want = (prm->flags & PULSE_FLAG_RECORD_PATHS) || ctx->rec_force;
+    if (!want) return PULSE_OK;
+    const uint64_t first = prm->accumulation_count, total = first + std::max(prm->spp, 1u);
+    const bool extend = first > 0 && ctx->rec_valid && ctx->rec_samples == first && ctx->rec_npix == npix &&
+                        ctx->rec_traces == traces && ctx->rec_scene == ctx->cur &&
+                        ctx->rec_generation == ctx->cur->generation && ctx->rec_loop == ctx->loop_variant &&
+                        same_camera(ctx->rec_cam, *cam) && same_paths(ctx->rec_prm, *prm);
+    ctx->rec_valid = false;
+    const uint64_t entries = total * npix * traces;
+    if ((first > 0 && !extend) || entries >= (1ull << 32) || entries * 21 > (16ull << 30)) return PULSE_OK;
+    const size_t keep = extend ? static_cast<size_t>(first) * npix * traces : 0;
+    auto grow = [&](pulse_host::DevBuf &b, size_t elem) -> int {
+        const size_t bytes = static_cast<size_t>(entries) * elem;
+        if (b.ptr && b.cap >= bytes) return PULSE_OK;
+        void *np = nullptr;
+        HIP_TRY(hipMalloc(&np, bytes));
+        if (keep && b.ptr) HIP_TRY(hipMemcpyAsync(np, b.ptr, keep * elem, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        b.release();
+        b.ptr = np;
+        b.cap = bytes;
+        return PULSE_OK;
+    };
+    int rc;
+    if ((rc = grow(ctx->rec_hit, sizeof(float4))) || (rc = grow(ctx->rec_v, sizeof(float))) ||
+        (rc = grow(ctx->rec_vis, sizeof(uint8_t))))
+        return rc;
+    wb.rec_hit = ctx->rec_hit.as<float4>();
+    wb.rec_v = ctx->rec_v.as<float>();
+    wb.rec_vis = ctx->rec_vis.as<uint8_t>();
+    wb.rec_traces = traces;
+    ctx->rec_samples = static_cast<uint32_t>(total);
+    ctx->rec_npix = static_cast<uint32_t>(npix);
+    ctx->rec_traces = traces;
+    ctx->rec_scene = ctx->cur;
+    ctx->rec_generation = ctx->cur->generation;
+    ctx->rec_loop = ctx->loop_variant;
+    ctx->rec_cam = *cam;
+    ctx->rec_prm = *prm;
+    return PULSE_OK;  // valid once the call has queued every sample
+}
+
 extern "C" {
 
 int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, float *accum_rgba) {
@@ -1394,6 +1470,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     const size_t npix = static_cast<size_t>(cam->width) * cam->height;
     if ((rc = ensure_wavefront(ctx, npix)) != PULSE_OK) return rc;
     WfBuffers wb = wf_buffers(ctx);
+    if ((rc = record_setup(ctx, cam, prm, traces, npix, wb)) != PULSE_OK) return rc;
     const bool timing = (prm->flags & PULSE_FLAG_STAGE_TIMING) != 0;
     if (timing && !ctx->events_created) {
         for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
@@ -1423,6 +1500,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     std::vector<uint32_t> counts(2 * (kMaxTraces + 1));
     for (uint32_t s = 0; s < std::max(prm->spp, 1u); ++s) {
         fa.accumulation_count = prm->accumulation_count + s;
+        wb.rec_base = fa.accumulation_count * static_cast<uint32_t>(npix);
         HIP_TRY(hipMemsetAsync(wb.ext_count, 0, sizeof(uint32_t) * kCounterWords, ctx->stream));
         if (concurrent) {
             // timing events (ctx->ev2): [0] start; extend k: [1 + 2k], [2 + 2k];
@@ -1567,6 +1645,7 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
+    if (wb.rec_hit) ctx->rec_valid = true;
     return PULSE_OK;
 }
 
@@ -1797,7 +1876,10 @@ int pulse_adjoint_views(pulse_ctx *ctx, const pulse_adjoint_view *views, uint32_
 
 int pulse_forward_adjoint(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, const float *target,
                           float *image_out, float *grad, double *loss) {
+    // the forward pass keeps its paths: the adjoint reads them back
+    if (ctx) ctx->rec_force = true;
     int rc = pulse_trace(ctx, cam, prm, image_out);
+    if (ctx) ctx->rec_force = false;
     if (rc) return rc;
     if (loss) {
         if ((rc = pulse_image_loss(ctx, image_out, target, cam->width * cam->height, loss)) != PULSE_OK) return rc;
@@ -1884,6 +1966,8 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_PAIR_WAVES: *value = ctx->eff_pair_waves(); return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES_SHADOW: *value = ctx->eff_pair_waves_shadow(); return PULSE_OK;
         case PULSE_OPT_ADJOINT_REDUCE_ROUNDS: *value = ctx->adjoint_reduce_rounds; return PULSE_OK;
+        case PULSE_OPT_RECORDED_SAMPLES: *value = ctx->rec_valid ? ctx->rec_samples : 0; return PULSE_OK;
+        case PULSE_OPT_ADJOINT_READ_RECORD: *value = ctx->rec_read ? 1 : 0; return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;
         case PULSE_OPT_STACK_ENTRIES: *value = ctx->stack_entries; return PULSE_OK;

@@ -162,6 +162,94 @@ def test_lambert_path_gradient(dev, extra_flags, frozen, depth):
     _replay_case(dev, scene, cam, prm, frozen)
 
 
+def _with_flags(prm, extra):
+    out = make_params(spp=prm.spp, max_depth=prm.max_depth, mode=prm.mode, rr_start_depth=prm.rr_start_depth,
+                      flags=prm.flags | extra)
+    out.accumulation_count = prm.accumulation_count
+    out.nee_eval = prm.nee_eval
+    return out
+
+
+@pytest.mark.parametrize("loop", [1, 2])
+@pytest.mark.parametrize("extra_flags,frozen", MODES)
+def test_recorded_paths_give_the_traced_gradient(dev, extra_flags, frozen, loop):
+    """PULSE_FLAG_RECORD_PATHS: the forward keeps every path's closest hits
+    and shadow visibilities and the adjoint reads them instead of tracing
+    the paths again: the replay equals the forward bit for bit and the
+    gradient equals the traced adjoint's (the same terms; only the order of
+    the float atomics that sum them differs)."""
+    scene, cam, prm = _cornell_case(depth=4, rr=1)
+    prm.flags |= extra_flags
+    dev.set_option(N.OPT_TRAVERSAL_LOOP, loop)
+    try:
+        dev.upload_scene(scene)
+        nm = len(scene.buffers["materials"])
+        adj = PulseAdjoint(dev, cam.width, cam.height, nm)
+        img = adj.forward(cam, prm)
+        target = (img * np.float32(0.6) + np.float32(0.05)).astype(np.float32)
+        adj.set_target(target)
+        g_traced = adj.backward(cam, prm)
+        assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 0
+        img2 = adj.forward(cam, _with_flags(prm, N.FLAG_RECORD_PATHS))
+        assert same_bits(img2, img).all()
+        assert dev.get_option(N.OPT_RECORDED_SAMPLES) == prm.spp
+        g_rec, replay = adj.backward(cam, prm, want_replay=True)
+        assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 1
+        assert same_bits(replay, img).all()
+        scale = np.abs(g_traced).max()
+        np.testing.assert_allclose(g_rec, g_traced, rtol=1e-5, atol=1e-6 * scale)
+        # forward + adjoint in one call records by itself
+        loss, g_fa = adj.forward_backward(cam, prm)
+        assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 1
+        np.testing.assert_allclose(g_fa, g_traced, rtol=1e-5, atol=1e-6 * scale)
+    finally:
+        dev.set_option(N.OPT_TRAVERSAL_LOOP, 1)
+
+
+def test_path_record_lifetime(dev):
+    """The record holds one image: accumulated calls with the same camera and
+    parameters extend it (the adjoint of samples 0-5 reads it); a material
+    update, another camera or a call that does not continue its samples
+    leaves no valid record, and the adjoint then traces (same gradient)."""
+    scene, cam, prm = _cornell_case(spp=2, depth=3, rr=1)
+    dev.upload_scene(scene)
+    nm = len(scene.buffers["materials"])
+    adj = PulseAdjoint(dev, cam.width, cam.height, nm)
+    rec = _with_flags(prm, N.FLAG_RECORD_PATHS)
+    for acc in (0, 2, 4):
+        rec.accumulation_count = acc
+        img = adj.forward(cam, rec)
+    assert dev.get_option(N.OPT_RECORDED_SAMPLES) == 6
+    target = (img * np.float32(0.6) + np.float32(0.05)).astype(np.float32)
+    adj.set_target(target)
+    last = _with_flags(prm, 0)
+    last.accumulation_count = 4
+    g_rec, replay = adj.backward(cam, last, want_replay=True)
+    assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 1
+    assert same_bits(replay, img).all()
+    # the same adjoint traced: a material update (to the same values) drops the record
+    adj.set_materials(scene.buffers["materials"])
+    assert dev.get_option(N.OPT_RECORDED_SAMPLES) == 0
+    g_traced = adj.backward(cam, last)
+    assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 0
+    scale = np.abs(g_traced).max()
+    np.testing.assert_allclose(g_rec, g_traced, rtol=1e-5, atol=1e-6 * scale)
+    # a call that does not continue the record (accumulation_count 3 after 0-1)
+    rec.accumulation_count = 0
+    adj.forward(cam, rec)
+    assert dev.get_option(N.OPT_RECORDED_SAMPLES) == 2
+    rec.accumulation_count = 3
+    adj.forward(cam, rec)
+    assert dev.get_option(N.OPT_RECORDED_SAMPLES) == 0
+    # another camera for the adjoint: the record is not read
+    rec.accumulation_count = 0
+    adj.forward(cam, rec)
+    other = P.cornell()[1](48, 48)
+    other.world_position[0] += 0.01
+    adj.backward(other, prm)
+    assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == 0
+
+
 def test_roulette_deep_paths(dev):
     scene, cam, prm = _cornell_case(depth=8, rr=2)
     _replay_case(dev, scene, cam, prm, True)
