@@ -262,6 +262,19 @@ def bench_adjoint_views(iters: int = 3) -> dict:
         dev.set_option(N.OPT_ADJOINT_REDUCE_ROUNDS, 0)
         per_lane_ms = best(batched)
         dev.set_option(N.OPT_ADJOINT_REDUCE_ROUNDS, -1)
+        # the views' forward passes keep their paths (one record per camera)
+        # and the batched adjoint reads them instead of tracing again
+        rec = N.Params.from_buffer_copy(prm)
+        rec.flags |= N.FLAG_RECORD_PATHS
+
+        def forward_recorded():
+            for cam, img in zip(mv.cameras, mv.images):
+                N.check(lib.pulse_trace(dev.handle, C.byref(cam), C.byref(rec), img.ptr))
+
+        fwd_rec_ms = best(forward_recorded)
+        batched_rec_ms = best(batched)
+        read_record = dev.get_option(N.OPT_ADJOINT_READ_RECORD)
+        g_rec = mv.grad.copy_to(np.zeros(nm * 7, np.float32))
         losses = mv.losses()
     finally:
         dev.close()
@@ -269,6 +282,10 @@ def bench_adjoint_views(iters: int = 3) -> dict:
             "adjoint_sequential_ms": round(seq_ms, 3), "adjoint_batched_per_lane_atomics_ms": round(per_lane_ms, 3),
             "batched_speedup": round(seq_ms / batched_ms, 3),
             "forward_plus_batched_adjoint_ratio": round((fwd_ms + batched_ms) / fwd_ms, 3),
+            "recorded_paths": {"forward_8_views_ms": round(fwd_rec_ms, 3),
+                               "adjoint_batched_ms": round(batched_rec_ms, 3), "read_record": int(read_record),
+                               "forward_plus_batched_adjoint_ratio": round((fwd_rec_ms + batched_rec_ms) / fwd_ms, 3),
+                               "grad_rel_diff": float(np.abs(g_rec - g).max() / max(np.abs(g).max(), 1e-30))},
             "loss_sum": float(sum(losses)), "grad_norm": float(np.linalg.norm(g)),
             "scene_build_s": round(build_s, 1),
             "config": "256 BLAS x 1024 instances (21M unique triangles), 1025 materials x 7 parameters, "

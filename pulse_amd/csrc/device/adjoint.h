@@ -88,6 +88,12 @@ struct AdjView {
     float grad_scale;      // dL/dI = grad_scale * (I - T): 2 / (3 npix) of this view
     uint32_t npix;         // width * height
     uint32_t block0;       // the view's first block
+    // the forward pass's recorded paths (null: trace them again): at
+    // (sample * npix + pixel) * vertices + vertex, the closest hit
+    // {t, inst bits, tri bits, u}, its v, and the shadow ray's visibility
+    const float4 *rec_hit;
+    const float *rec_v;
+    const uint8_t *rec_vis;
 };
 
 struct AdjArgs {
@@ -99,13 +105,7 @@ struct AdjArgs {
     uint32_t light_power;  // lights picked by emission strength (its pdf is held: see below)
     uint32_t ref_ggx;      // PULSE_MODE_REF_GGX: emission at every hit, no light sampling
     uint32_t lambert;      // PULSE_MODE_REF_NEE_LAMBERT: cosine sampling, weight = albedo
-    // the forward pass's recorded paths (null: trace them again): at
-    // (sample * rec_npix + pixel) * vertices + vertex, the closest hit
-    // {t, inst bits, tri bits, u}, its v, and the shadow ray's visibility
-    const float4 *rec_hit;
-    const float *rec_v;
-    const uint8_t *rec_vis;
-    uint32_t rec_npix;
+
     float *grad;           // n_materials * 7, accumulated into
     uint32_t n_materials;
     uint32_t use_lds;      // gradient buffer fits in LDS
@@ -266,14 +266,14 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
             for (uint32_t depth = 0; depth < depth_end; ++depth) {
                 float t = kTFar, u = 0.0f, v = 0.0f;
                 uint32_t inst = 0, tri = 0;
-                const uint32_t rec = (s * a.rec_npix + pixel) * a.vertices + depth;
-                if (a.rec_hit) {
-                    const float4 hr = a.rec_hit[rec];
+                const uint32_t rec = (s * vw.npix + pixel) * a.vertices + depth;
+                if (vw.rec_hit) {
+                    const float4 hr = vw.rec_hit[rec];
                     t = hr.x;
                     inst = __float_as_uint(hr.y);
                     tri = __float_as_uint(hr.z);
                     u = hr.w;
-                    v = a.rec_v[rec];
+                    v = vw.rec_v[rec];
                 } else {
                     trace_any<LOOP, false>(sc, val3(o), val3(d), t, inst, tri, u, v, 0.0f, stk);
                 }
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(DevScene sc, AdjArgs a) {
                     NeeSampleT<S> ns = nee_sample(sc, h.position, h.normal, m, wo, a.nee_ggx != 0u, rng, a.light_power != 0u);
                     float tt = kTFar, su = 0.0f, sv = 0.0f;
                     uint32_t si = 0, st = 0;
-                    const bool occ = a.rec_vis ? a.rec_vis[rec] == 0u
+                    const bool occ = vw.rec_vis ? vw.rec_vis[rec] == 0u
                                                : trace_any<LOOP, true>(sc, val3(ns.origin), val3(ns.dir), tt, si, st,
                                                                        su, sv, val(ns.threshold), stk);
                     if (!occ) {

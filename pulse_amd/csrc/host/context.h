@@ -166,19 +166,32 @@ struct pulse_ctx {
     // distinct materials per wave summed before their atomic; -1: none for a
     // gradient in LDS, 2 for one in device memory (the measured best)
     int adjoint_reduce_rounds = -1;
-    // PULSE_FLAG_RECORD_PATHS: per (sample, pixel, trace index) the closest
-    // hit, its v and the shadow ray's visibility, of samples 0 .. rec_samples
-    // - 1 of one camera, parameter set and scene state
-    pulse_host::DevBuf rec_hit, rec_v, rec_vis;
-    bool rec_valid = false;
+    // PULSE_FLAG_RECORD_PATHS: up to kMaxRecords images (one per camera),
+    // each per (sample, pixel, trace index) the closest hit, its v and the
+    // shadow ray's visibility, of samples 0 .. samples - 1 of one camera,
+    // parameter set and scene state
+    struct PathRecord {
+        pulse_host::DevBuf hit, v, vis;
+        bool valid = false;
+        uint32_t samples = 0, npix = 0, traces = 0;
+        uint64_t generation = 0, used = 0;
+        const void *scene = nullptr;
+        int loop = -1;
+        pulse_camera cam{};
+        pulse_params prm{};
+        void release() {
+            hit.release();
+            v.release();
+            vis.release();
+            valid = false;
+        }
+    };
+    static constexpr int kMaxRecords = 16;
+    PathRecord recs[kMaxRecords];
+    int rec_pending = -1;          // the record the pulse_trace call in progress fills
+    uint64_t rec_clock = 0;
     bool rec_force = false;        // pulse_forward_adjoint records its forward pass
-    bool rec_read = false;         // the last adjoint read the record
-    uint32_t rec_samples = 0, rec_npix = 0, rec_traces = 0;
-    uint64_t rec_generation = 0;
-    const void *rec_scene = nullptr;
-    int rec_loop = -1;
-    pulse_camera rec_cam{};
-    pulse_params rec_prm{};
+    bool rec_read = false;         // the last adjoint read recorded paths for every view
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev[pt::kMaxTraces] = {}, join_ev[pt::kMaxTraces] = {};
     hipEvent_t ev2[6 * pt::kMaxTraces + 4] = {};

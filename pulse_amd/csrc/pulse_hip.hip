@@ -125,6 +125,13 @@ bool same_paths(const pulse_params &a, const pulse_params &b) {
 }
 bool same_camera(const pulse_camera &a, const pulse_camera &b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
 
+// A path record made with this camera and path parameters on the scene
+// state now bound.
+bool record_matches(const pulse_ctx *ctx, const pulse_ctx::PathRecord &r, const pulse_camera &cam,
+                    const pulse_params &prm) {
+    return r.scene == ctx->cur && r.generation == ctx->cur->generation && r.loop == ctx->loop_variant &&
+           same_camera(r.cam, cam) && same_paths(r.prm, prm);
+}
 DevScene dev_scene(pulse_ctx *ctx) { return pulse_host::dev_scene(*ctx->cur, ctx->errors.as<uint32_t>()); }
 
 // Makes `sd` the scene of the context's launches.
@@ -1028,17 +1035,7 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
     a.horizon_clamp = (prm->flags & PULSE_FLAG_HORIZON_CLAMP) ? 1u : 0u;
     a.detached = (prm->flags & PULSE_FLAG_ADJOINT_REPARAM) ? 0u : 1u;
     a.rr_start = prm->rr_start_depth;
-    // the forward pass's recorded paths, when they are this image's
-    if (n == 1 && ctx->rec_valid && ctx->rec_scene == ctx->cur && ctx->rec_generation == ctx->cur->generation &&
-        ctx->rec_loop == ctx->loop_variant && ctx->rec_samples >= a.spp && ctx->rec_traces == a.vertices &&
-        static_cast<uint64_t>(ctx->rec_npix) == static_cast<uint64_t>(views[0].camera.width) * views[0].camera.height &&
-        same_camera(ctx->rec_cam, views[0].camera) && same_paths(ctx->rec_prm, *prm)) {
-        a.rec_hit = ctx->rec_hit.as<float4>();
-        a.rec_v = ctx->rec_v.as<float>();
-        a.rec_vis = ctx->rec_vis.as<uint8_t>();
-        a.rec_npix = ctx->rec_npix;
-    }
-    ctx->rec_read = a.rec_hit != nullptr;
+    ctx->rec_read = true;
     const size_t lds_bytes = static_cast<size_t>(nent) * sizeof(float);
     a.use_lds = lds_bytes <= 16384 ? 1u : 0u;
     const size_t lds = a.use_lds ? lds_bytes : 0;
@@ -1060,6 +1057,19 @@ int adjoint_run(pulse_ctx *ctx, const pulse_params *prm, const pulse_adjoint_vie
                                                  static_cast<unsigned long long>(npix));
             w.grad_scale = static_cast<float>(2.0 / (3.0 * static_cast<double>(npix)));
             w.npix = static_cast<uint32_t>(npix);
+            // the forward pass's recorded paths, when they are this view's image
+            w.rec_hit = nullptr;
+            w.rec_v = nullptr;
+            w.rec_vis = nullptr;
+            for (auto &r : ctx->recs)
+                if (r.valid && record_matches(ctx, r, v.camera, *prm) && r.samples >= a.spp &&
+                    r.traces == a.vertices && r.npix == npix) {
+                    w.rec_hit = r.hit.as<float4>();
+                    w.rec_v = r.v.as<float>();
+                    w.rec_vis = r.vis.as<uint8_t>();
+                    r.used = ++ctx->rec_clock;
+                }
+            if (!w.rec_hit) ctx->rec_read = false;
             w.block0 = static_cast<uint32_t>(blocks);
             blocks += (npix + kAdjBlockPixels - 1) / kAdjBlockPixels;
         }
@@ -1124,9 +1134,7 @@ int pulse_destroy(pulse_ctx *ctx) {
     ctx->loss_acc.release();
     ctx->spill.release();
     ctx->census.release();
-    ctx->rec_hit.release();
-    ctx->rec_v.release();
-    ctx->rec_vis.release();
+    for (auto &r : ctx->recs) r.release();
     if (ctx->events_created)
         for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->events2_created)
@@ -1254,7 +1262,8 @@ int pulse_scene_update_materials(pulse_ctx *ctx, const pulse_material *materials
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (count) HIP_TRY(hipMemcpy(ctx->cur->buf[B_MATERIALS].ptr, materials, count * sizeof(pulse_material), hipMemcpyHostToDevice));
-    if (ctx->rec_scene == ctx->cur) ctx->rec_valid = false;  // materials steer the paths
+    for (auto &r : ctx->recs)
+        if (r.scene == ctx->cur) r.valid = false;  // materials steer the paths
     return PULSE_OK;
 }
 
@@ -1336,7 +1345,8 @@ int pulse_scene_destroy(pulse_scene *scene) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->cur == &scene->data) ctx->cur = ctx->own.valid ? &ctx->own : nullptr;
-    if (ctx->rec_scene == &scene->data) ctx->rec_valid = false;
+    for (auto &r : ctx->recs)
+        if (r.scene == &scene->data) r.release();
     scene->data.release();
     delete scene;
     return PULSE_OK;
@@ -1410,23 +1420,42 @@ int frame_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
 }  // namespace
 
 // Path record of a pulse_trace call (PULSE_FLAG_RECORD_PATHS, or the forward
-// pass of pulse_forward_adjoint): accumulation_count 0 starts one, a call
-// that continues the recorded samples with the same camera, parameters and
-// scene state extends it (the older samples are kept when the buffers
-// grow); anything else leaves no valid record.  Sets wb's record pointers
-// (its rec_base is set per sample) or leaves them null.
+// pass of pulse_forward_adjoint), one per camera: accumulation_count 0
+// starts one (in the slot of the same camera, else a free slot, else the
+// least recently used), a call that continues a camera's recorded samples
+// with the same parameters and scene state extends it (the older samples
+// are kept when the buffers grow); anything else leaves that camera without
+// a valid record.  Sets wb's record pointers (its rec_base is set per
+// sample) or leaves them null.
 int record_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm, uint32_t traces, size_t npix,
                  WfBuffers &wb) {
+    ctx->rec_pending = -1;
     const bool want = (prm->flags & PULSE_FLAG_RECORD_PATHS) || ctx->rec_force;
     if (!want) return PULSE_OK;
     const uint64_t first = prm->accumulation_count, total = first + std::max(prm->spp, 1u);
-    const bool extend = first > 0 && ctx->rec_valid && ctx->rec_samples == first && ctx->rec_npix == npix &&
-                        ctx->rec_traces == traces && ctx->rec_scene == ctx->cur &&
-                        ctx->rec_generation == ctx->cur->generation && ctx->rec_loop == ctx->loop_variant &&
-                        same_camera(ctx->rec_cam, *cam) && same_paths(ctx->rec_prm, *prm);
-    ctx->rec_valid = false;
+    int slot = -1;
+    for (int i = 0; i < pulse_ctx::kMaxRecords; ++i)
+        if (ctx->recs[i].valid && same_camera(ctx->recs[i].cam, *cam)) slot = i;
     const uint64_t entries = total * npix * traces;
+    bool extend = false;
+    if (slot >= 0) {
+        pulse_ctx::PathRecord &r = ctx->recs[slot];
+        extend = first > 0 && record_matches(ctx, r, *cam, *prm) && r.samples == first && r.npix == npix &&
+                 r.traces == traces;
+        r.valid = false;
+    }
     if ((first > 0 && !extend) || entries >= (1ull << 32) || entries * 21 > (16ull << 30)) return PULSE_OK;
+    if (slot < 0) {
+        for (int i = 0; i < pulse_ctx::kMaxRecords && slot < 0; ++i)
+            if (!ctx->recs[i].valid) slot = i;
+        if (slot < 0) {
+            slot = 0;
+            for (int i = 1; i < pulse_ctx::kMaxRecords; ++i)
+                if (ctx->recs[i].used < ctx->recs[slot].used) slot = i;
+        }
+    }
+    pulse_ctx::PathRecord &r = ctx->recs[slot];
+    r.valid = false;
     const size_t keep = extend ? static_cast<size_t>(first) * npix * traces : 0;
     auto grow = [&](pulse_host::DevBuf &b, size_t elem) -> int {
         const size_t bytes = static_cast<size_t>(entries) * elem;
@@ -1441,22 +1470,23 @@ int record_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *pr
         return PULSE_OK;
     };
     int rc;
-    if ((rc = grow(ctx->rec_hit, sizeof(float4))) || (rc = grow(ctx->rec_v, sizeof(float))) ||
-        (rc = grow(ctx->rec_vis, sizeof(uint8_t))))
+    if ((rc = grow(r.hit, sizeof(float4))) || (rc = grow(r.v, sizeof(float))) || (rc = grow(r.vis, sizeof(uint8_t))))
         return rc;
-    wb.rec_hit = ctx->rec_hit.as<float4>();
-    wb.rec_v = ctx->rec_v.as<float>();
-    wb.rec_vis = ctx->rec_vis.as<uint8_t>();
+    wb.rec_hit = r.hit.as<float4>();
+    wb.rec_v = r.v.as<float>();
+    wb.rec_vis = r.vis.as<uint8_t>();
     wb.rec_traces = traces;
-    ctx->rec_samples = static_cast<uint32_t>(total);
-    ctx->rec_npix = static_cast<uint32_t>(npix);
-    ctx->rec_traces = traces;
-    ctx->rec_scene = ctx->cur;
-    ctx->rec_generation = ctx->cur->generation;
-    ctx->rec_loop = ctx->loop_variant;
-    ctx->rec_cam = *cam;
-    ctx->rec_prm = *prm;
-    return PULSE_OK;  // valid once the call has queued every sample
+    r.samples = static_cast<uint32_t>(total);
+    r.npix = static_cast<uint32_t>(npix);
+    r.traces = traces;
+    r.scene = ctx->cur;
+    r.generation = ctx->cur->generation;
+    r.loop = ctx->loop_variant;
+    r.cam = *cam;
+    r.prm = *prm;
+    r.used = ++ctx->rec_clock;
+    ctx->rec_pending = slot;  // valid once the call has queued every sample
+    return PULSE_OK;
 }
 
 extern "C" {
@@ -1645,7 +1675,8 @@ int pulse_trace(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *prm
     ctx->stats.ms_shade = ms_total - ms_primary - ms_incoherent;
     ctx->stats.extend_launches = traces * std::max(prm->spp, 1u);
     ctx->stats.shadow_launches = traces * std::max(prm->spp, 1u);
-    if (wb.rec_hit) ctx->rec_valid = true;
+    if (ctx->rec_pending >= 0) ctx->recs[ctx->rec_pending].valid = true;
+    ctx->rec_pending = -1;
     return PULSE_OK;
 }
 
@@ -1966,7 +1997,14 @@ int pulse_get_option(pulse_ctx *ctx, int option, int64_t *value) {
         case PULSE_OPT_PAIR_WAVES: *value = ctx->eff_pair_waves(); return PULSE_OK;
         case PULSE_OPT_PAIR_WAVES_SHADOW: *value = ctx->eff_pair_waves_shadow(); return PULSE_OK;
         case PULSE_OPT_ADJOINT_REDUCE_ROUNDS: *value = ctx->adjoint_reduce_rounds; return PULSE_OK;
-        case PULSE_OPT_RECORDED_SAMPLES: *value = ctx->rec_valid ? ctx->rec_samples : 0; return PULSE_OK;
+        case PULSE_OPT_RECORDED_SAMPLES: {
+            // of the most recently recorded camera
+            const pulse_ctx::PathRecord *last = nullptr;
+            for (const auto &r : ctx->recs)
+                if (r.used && (!last || r.used > last->used)) last = &r;
+            *value = last && last->valid ? last->samples : 0;
+            return PULSE_OK;
+        }
         case PULSE_OPT_ADJOINT_READ_RECORD: *value = ctx->rec_read ? 1 : 0; return PULSE_OK;
         case PULSE_OPT_LDS_STACK_ENTRIES: *value = ctx->lds_entries; return PULSE_OK;
         case PULSE_OPT_TRIS_PER_STEP: *value = ctx->tris_per_step; return PULSE_OK;

@@ -57,19 +57,24 @@ def _targets(imgs, a=0.6, b=0.05):
     return out
 
 
+@pytest.mark.parametrize("record", [False, True], ids=["traced", "recorded"])
 @pytest.mark.parametrize("extra_flags,frozen", [pytest.param(0, True, id="detached"),
                                                 pytest.param(N.FLAG_ADJOINT_REPARAM, False, id="reparam")])
-def test_views_vs_oracle(dev, extra_flags, frozen):
+def test_views_vs_oracle(dev, extra_flags, frozen, record):
+    """`recorded`: the views' forward passes keep their paths
+    (PULSE_FLAG_RECORD_PATHS, one record per camera) and the batched adjoint
+    reads them for every view instead of tracing."""
     scene, _ = P.cornell()
     dev.upload_scene(scene)
     nm = len(scene.buffers["materials"])
     cams = _cornell_views()
     prm = _params(flags=extra_flags)
     mv = MultiViewAdjoint(dev, cams, nm)
-    imgs = mv.forward(prm)
+    imgs = mv.forward(_params(flags=extra_flags | (N.FLAG_RECORD_PATHS if record else 0)))
     targets = _targets(imgs)
     mv.set_targets(targets)
     g, replays = mv.backward(prm, want_replay=True)
+    assert dev.get_option(N.OPT_ADJOINT_READ_RECORD) == (1 if record else 0)
     for v, (img, rep) in enumerate(zip(imgs, replays)):
         assert same_bits(rep, img).all(), f"view {v}: replay differs from the forward image"
     o = OracleScene(scene)
