@@ -220,7 +220,11 @@ typedef struct pulse_params {
                                      /* image, which then reads them instead of       */
                                      /* tracing the paths again (same gradient).      */
                                      /* A call with accumulation_count 0 starts the   */
-                                     /* record, later calls extend it                 */
+                                     /* record, later calls extend it.  One record    */
+                                     /* per camera, up to 16 (least recently used     */
+                                     /* replaced), 32 GB in all (older records give   */
+                                     /* their memory back), none above 16 GB; a       */
+                                     /* material update drops the scene's records     */
 
 /* Ray / hit records of the debug trace entry point (RayHitRecord,
  * types.wgsl:8-21).  A miss leaves t = 1e30 and the ids at 0. */

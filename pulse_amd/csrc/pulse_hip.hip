@@ -127,6 +127,7 @@ bool same_camera(const pulse_camera &a, const pulse_camera &b) { return std::mem
 
 // A path record made with this camera and path parameters on the scene
 // state now bound.
+constexpr size_t kRecordBudget = 32ull << 30;  // bytes of path records per context
 bool record_matches(const pulse_ctx *ctx, const pulse_ctx::PathRecord &r, const pulse_camera &cam,
                     const pulse_params &prm) {
     return r.scene == ctx->cur && r.generation == ctx->cur->generation && r.loop == ctx->loop_variant &&
@@ -1456,6 +1457,20 @@ int record_setup(pulse_ctx *ctx, const pulse_camera *cam, const pulse_params *pr
     }
     pulse_ctx::PathRecord &r = ctx->recs[slot];
     r.valid = false;
+    // all records together stay within kRecordBudget: the least recently
+    // used other records give their memory back first
+    const size_t need = static_cast<size_t>(entries) * 21;
+    for (;;) {
+        size_t held = 0;
+        int lru = -1;
+        for (int i = 0; i < pulse_ctx::kMaxRecords; ++i) {
+            if (i == slot || !ctx->recs[i].hit.ptr) continue;
+            held += ctx->recs[i].hit.cap + ctx->recs[i].v.cap + ctx->recs[i].vis.cap;
+            if (lru < 0 || ctx->recs[i].used < ctx->recs[lru].used) lru = i;
+        }
+        if (held + need <= kRecordBudget || lru < 0) break;
+        ctx->recs[lru].release();
+    }
     const size_t keep = extend ? static_cast<size_t>(first) * npix * traces : 0;
     auto grow = [&](pulse_host::DevBuf &b, size_t elem) -> int {
         const size_t bytes = static_cast<size_t>(entries) * elem;
