@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene sc, WfBuffers wb, int
 // lane of the grid.  LOOP 1 walks the reference's binary BVHs, LOOP 2 the
 // 4-wide ones (wide.h).
 template <int LDS_ENTRIES, bool SHADOW, int TPS, int LOOP = 1>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SHADOW ? 7 : 6)))
 k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t gather, uint32_t period,
               int capacity, uint32_t *spill) {
     __shared__ uint32_t smem[LDS_ENTRIES * 64];
