@@ -11,6 +11,17 @@ namespace pt {
 
 constexpr int kMaxTraces = 64;
 
+// A persistent trace launch splits each queue into kQueueParts contiguous
+// parts, one per XCD: the block that runs on XCD x (blocks are dispatched
+// to the 8 XCDs in turn) takes rays from part x first, so each XCD's L2
+// serves the rays of one stretch of the queue (one region of the screen)
+// instead of all eight caching the same one; a wave whose part is drained
+// goes on to the next part.  Each part's taken-ray head has its own 64-byte
+// line, kHeadWords words per queue kind (extension, shadow).
+constexpr int kQueueParts = 8;
+constexpr int kHeadStride = 16;
+constexpr int kHeadWords = (kMaxTraces + 1) * kQueueParts * kHeadStride;
+
 // Traversal variants: 0 = the reference's nested TLAS/BLAS loops,
 // 1 = single loop over both levels (traverse.h), same results; 2 = the
 // 4-wide BVHs (wide.h), same results up to the order among exactly tied hits.

@@ -44,8 +44,10 @@ struct WfBuffers {
     // counters: ext[k] = rays traced at trace index k; sh[k] = shadow rays
     uint32_t *ext_count;  // [kMaxTraces + 1]
     uint32_t *sh_count;   // [kMaxTraces + 1]
-    uint32_t *ext_head;   // [kMaxTraces + 1] rays taken by the persistent extend launches
-    uint32_t *sh_head;    // [kMaxTraces + 1] rays taken by the persistent shadow launches
+    // rays taken by the persistent extend / shadow launches from part p of
+    // the queue at trace index k: [(k * kQueueParts + p) * kHeadStride]
+    uint32_t *ext_head;
+    uint32_t *sh_head;
     // optional record of the paths for the adjoint (null: none): at
     // r = (rec_base + pixel) * rec_traces + trace index, the closest hit
     // {t, inst bits, tri bits, u}, its v, and 1 / 0 for an unoccluded /
@@ -188,8 +190,11 @@ k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t
         stk.spill_stride = gridDim.x * 64u;
     }
     const uint32_t count = SHADOW ? wb.sh_count[k] : wb.ext_count[k];
-    uint32_t *head = SHADOW ? &wb.sh_head[k] : &wb.ext_head[k];
+    uint32_t *heads = (SHADOW ? wb.sh_head : wb.ext_head) + k * (kQueueParts * kHeadStride);
     const uint32_t lane = __lane_id();
+    // wave-uniform: the queue part rays are taken from (the block's XCD
+    // first), and how many parts were found drained
+    uint32_t part = blockIdx.x % kQueueParts, drained = 0;
     std::conditional_t<LOOP == 2, WideTraversal<SHADOW, TPS>, Traversal<SHADOW, TPS>> tr;
     bool alive = false;
     bool exhausted = false;  // wave-uniform: the queue has no rays left
@@ -199,13 +204,20 @@ k_trace_queue(DevScene sc, WfBuffers wb, int q, int k, uint32_t refill, uint32_t
         const uint64_t idle = __ballot(!alive);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
         if (!exhausted && (n_idle >= refill || n_idle == 64u)) {
+            // part p holds slots [count * p / parts, count * (p + 1) / parts)
+            const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(count) * part / kQueueParts);
+            const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(count) * (part + 1u) / kQueueParts);
             uint32_t base = 0;
-            if (lane == 0u) base = atomicAdd(head, n_idle);
-            base = __shfl(base, 0);
-            if (base + n_idle >= count) exhausted = true;
+            if (lane == 0u) base = atomicAdd(&heads[part * kHeadStride], n_idle);
+            base = lo + __shfl(base, 0);
+            if (base + n_idle >= hi) {
+                // this part is drained: later refills take from the next one
+                part = (part + 1u) % kQueueParts;
+                if (++drained == kQueueParts) exhausted = true;
+            }
             if (!alive) {
                 const uint32_t idx = base + static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
-                if (idx < count) {
+                if (idx < hi) {
                     slot = idx;
                     if (SHADOW) {
                         float4 oa = wb.sh_o[idx], da = wb.sh_d[idx];

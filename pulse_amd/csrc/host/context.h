@@ -50,8 +50,9 @@ enum SceneBuf {
 };
 
 // Queue counter words of the single-view path: extension and shadow queue
-// lengths, then the persistent launches' taken-ray heads.
-constexpr int kCounterWords = 4 * (pt::kMaxTraces + 1);
+// lengths, then the persistent launches' taken-ray heads (per trace index,
+// one per queue part).
+constexpr int kCounterWords = 2 * (pt::kMaxTraces + 1) + 2 * pt::kHeadWords;
 
 enum WfBuf {
     W_RAY_O0, W_RAY_O1, W_RAY_D0, W_RAY_D1, W_HIT, W_HIT_V, W_STATE_T, W_STATE_C, W_SH_O, W_SH_D, W_SH_C,

@@ -342,7 +342,7 @@ WfBuffers wf_buffers_in(unsigned char *base, size_t max_pixels) {
     wb.ext_count = reinterpret_cast<uint32_t *>(wb.hit_v + max_pixels);
     wb.sh_count = wb.ext_count + (kMaxTraces + 1);
     wb.ext_head = wb.ext_count + 2 * (kMaxTraces + 1);
-    wb.sh_head = wb.ext_count + 3 * (kMaxTraces + 1);
+    wb.sh_head = wb.ext_head + kHeadWords;
     return wb;
 }
 
@@ -362,7 +362,7 @@ WfBuffers wf_buffers(pulse_ctx *ctx) {
     wb.ext_count = ctx->wf[W_COUNTERS].as<uint32_t>();
     wb.sh_count = wb.ext_count + (kMaxTraces + 1);
     wb.ext_head = wb.ext_count + 2 * (kMaxTraces + 1);
-    wb.sh_head = wb.ext_count + 3 * (kMaxTraces + 1);
+    wb.sh_head = wb.ext_head + kHeadWords;
     return wb;
 }
 
