@@ -181,6 +181,19 @@ def test_wide_render_cornell(dev, cornell, mode, nee, depth):
     _check_image(g, o)
 
 
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 2), (7, 1), (9, 3), (17, 1)])
+def test_wide_render_tiny_queues(dev, cornell, w, h):
+    """Persistent launches split each queue into 8 parts, one per XCD:
+    queues shorter than 8 rays (empty parts) and parts of uneven length
+    trace every ray once, as the oracle does."""
+    scene, camf = cornell
+    cam = camf(w, h)
+    prm = make_params(spp=8, max_depth=4, mode=N.MODE_GGX_NEE)
+    g = _render(dev, scene, cam, prm)
+    o, _, _ = OracleScene(scene).render(cam, prm)
+    _check_image(g, o)
+
+
 @pytest.mark.parametrize("waves,lds,tps", [(0, 0, 1), (24, 24, 2), (16, 8, 1), (64, 16, 2)])
 def test_wide_render_field_persistent(dev, field, waves, lds, tps):
     """One ray per lane and persistent launches (stack partly in device
