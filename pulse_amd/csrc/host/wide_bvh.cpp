@@ -75,10 +75,17 @@ struct Builder {
         return kWideLeafBit | ((count - 1u) << 26) | (static_cast<uint32_t>(f) & ((1u << 26) - 1u));
     }
 
-    // Node over a primitive range of more than 32 (chunks of 32 in order);
-    // returns its index, `need` = the stack entries below it.
-    uint32_t range_node(const Item &box, uint32_t first, uint32_t count, uint32_t &need) {
-        uint32_t node = alloc();
+    // Children that become wide nodes get their nodes side by side, right
+    // after each other, before any of their own children: siblings share
+    // 128-byte lines and every node still precedes its children.
+    static constexpr uint32_t kNoNode = ~0u;
+    static bool needs_node(const Item &it) { return it.interior || it.count > kWideMaxLeaf; }
+
+    // Node over a primitive range of more than 32 (chunks of 32 in order),
+    // at `node` (kNoNode: a new one); returns its index, `need` = the stack
+    // entries below it.
+    uint32_t range_node(const Item &box, uint32_t first, uint32_t count, uint32_t &need, uint32_t node = kNoNode) {
+        if (node == kNoNode) node = alloc();
         uint32_t chunks = (count + kWideMaxLeaf - 1) / kWideMaxLeaf;
         if (chunks <= 4) {
             for (uint32_t c = 0; c < chunks; ++c) {
@@ -90,7 +97,11 @@ struct Builder {
         }
         // four groups of whole chunks
         uint32_t per = (chunks + 3) / 4, deepest = 0;
+        uint32_t pre[4] = {kNoNode, kNoNode, kNoNode, kNoNode};
         int k = 0;
+        for (uint32_t c = 0; c < chunks; c += per, ++k)
+            if (std::min(per * kWideMaxLeaf, first + count - (first + c * kWideMaxLeaf)) > kWideMaxLeaf) pre[k] = alloc();
+        k = 0;
         for (uint32_t c = 0; c < chunks; c += per, ++k) {
             uint32_t f = first + c * kWideMaxLeaf;
             uint32_t cnt = std::min(per * kWideMaxLeaf, first + count - f);
@@ -98,7 +109,7 @@ struct Builder {
             if (cnt <= kWideMaxLeaf) {
                 entry = leaf_entry(f, cnt);
             } else {
-                entry = range_node(box, f, cnt, sub_need);
+                entry = range_node(box, f, cnt, sub_need, pre[k]);
             }
             set_child(node, k, box, entry);
             deepest = std::max(deepest, sub_need);
@@ -107,14 +118,15 @@ struct Builder {
         return node;
     }
 
-    // Entry for a child item; `need` = stack entries below it.
-    uint32_t entry_of(const Item &it, uint32_t &need) {
+    // Entry for a child item (its wide node at `node` if it needs one,
+    // kNoNode: a new one); `need` = stack entries below it.
+    uint32_t entry_of(const Item &it, uint32_t &need, uint32_t node = kNoNode) {
         need = 0;
         if (!it.interior) {
             if (it.count <= kWideMaxLeaf) return leaf_entry(it.first, it.count);
-            return range_node(it, it.first, it.count, need);
+            return range_node(it, it.first, it.count, need, node);
         }
-        return interior_node(it.bin, need);
+        return interior_node(it.bin, need, node);
     }
 
     // The cover tables for the subtree of `root` (post order, iterative).
@@ -174,18 +186,21 @@ struct Builder {
         cover(it.first + 1u, k - a, items);
     }
 
-    // Wide node for binary interior node `b`.
-    uint32_t interior_node(uint32_t b, uint32_t &need) {
+    // Wide node for binary interior node `b`, at `node` (kNoNode: a new one).
+    uint32_t interior_node(uint32_t b, uint32_t &need, uint32_t node = kNoNode) {
         Item root = read(b);
         std::vector<Item> items;
         const int a = split[b][0] ? split[b][0] : 2;  // 0: not planned (checked inputs always are)
         cover(root.first, a, items);
         cover(root.first + 1u, 4 - a, items);
-        uint32_t node = alloc();
+        if (node == kNoNode) node = alloc();
+        uint32_t pre[4] = {kNoNode, kNoNode, kNoNode, kNoNode};
+        for (size_t k = 0; k < items.size(); ++k)
+            if (needs_node(items[k])) pre[k] = alloc();
         uint32_t deepest = 0;
         for (size_t k = 0; k < items.size(); ++k) {
             uint32_t sub = 0;
-            uint32_t e = entry_of(items[k], sub);
+            uint32_t e = entry_of(items[k], sub, pre[k]);
             set_child(node, static_cast<int>(k), items[k], e);
             deepest = std::max(deepest, sub);
         }
